@@ -1,0 +1,114 @@
+/* whir_gpu.h — C ABI of the MI355X (gfx950) WHIR prover hot path.
+ *
+ * Drop-in boundary for the reference's prover plug points (kilic/whir):
+ *   - TwoAdicSubgroupDft::dft_batch / dft_algebra_batch  (crates/whir/src/whir.rs:34,59)
+ *   - MerkleTree(Ext)::commit / query                    (crates/merkle/src/lib.rs:19-83)
+ *   - Whir::commit + ProverStack::eval + Whir::open       (crates/whir/src/whir.rs:127-359,
+ *                                                         crates/whir/src/sumcheck/prover.rs:64-90,
+ *                                                         driven as in crates/whir/src/test.rs:218-235)
+ * Field elements cross the boundary as uint32 words holding the Montgomery
+ * representative (x * 2^32 mod p, p = 2^31 - 2^24 + 1): the words the reference
+ * serializes. Extension elements are 4 consecutive words (coefficient 0 first).
+ * Pointers named d_* are device pointers; `stream` is a hipStream_t (NULL = the
+ * library's default stream). Every function returns a whir_status.
+ */
+#ifndef WHIR_GPU_H
+#define WHIR_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mirrors common::Error {Transcript, Verify, IO} (crates/common/src/error.rs:1-6)
+ * plus the device-side failures. */
+typedef enum {
+  WHIR_OK = 0,
+  WHIR_ETRANSCRIPT = 1, /* proof buffer too small / transcript overflow */
+  WHIR_EVERIFY = 2,
+  WHIR_EIO = 3,
+  WHIR_EINVAL = 4,      /* shape or parameter outside what the reference accepts */
+  WHIR_EOOM = 5,        /* device allocation failed */
+  WHIR_EDEVICE = 6      /* HIP runtime error */
+} whir_status;
+
+/* SecurityAssumption (crates/whir/src/params.rs:6-21). */
+enum { WHIR_UNIQUE_DECODING = 0, WHIR_JOHNSON_BOUND = 1, WHIR_CAPACITY_BOUND = 2 };
+
+/* Whir::new(k, folding, rate, initial_reduction, soundness, security_level, ..)
+ * (crates/whir/src/whir.rs:103-125) plus the number of evaluation claims drawn
+ * after the commitment (crates/whir/src/test.rs:225-227). */
+typedef struct {
+  int32_t k;
+  int32_t folding;
+  int32_t rate;
+  int32_t initial_reduction;
+  int32_t soundness;
+  int32_t security_level;
+  int32_t n_points;
+} whir_params;
+
+typedef struct whir_ctx whir_ctx;       /* device, Poseidon2 constants, twiddle tables */
+typedef struct whir_prover whir_prover; /* device workspace for one proof shape */
+
+/* Creates the context on `device`: uploads the Poseidon2 KoalaBear constants of
+ * Poseidon2KoalaBear<16|24>::new_from_rng_128(StdRng::seed_from_u64(seed))
+ * (crates/whir/src/test.rs:195-196; the reference uses seed 1000). */
+int whir_ctx_create(int device, uint64_t poseidon_seed, whir_ctx** out);
+int whir_ctx_destroy(whir_ctx* ctx);
+
+/* ---- component entry points ---- */
+
+/* Replaces TwoAdicSubgroupDft::dft_batch (whir.rs:34): each of the `width`
+ * columns of the row-major matrix d_in (n_in_rows rows; rows n_in_rows..2^log_h
+ * are the zero padding of whir.rs:32) is transformed to 2^log_h evaluations in
+ * natural order. dft_algebra_batch (whir.rs:59) is the same call with width =
+ * 4 * (extension columns). d_out has 2^log_h * width words. */
+int whir_dft_batch(whir_ctx* ctx, const uint32_t* d_in, uint64_t n_in_rows, uint32_t log_h, uint32_t width,
+                   uint32_t* d_out, void* stream);
+
+/* Replaces MerkleTree::commit / MerkleTreeExt::commit (crates/merkle/src/lib.rs:22-29,
+ * 55-62): leaves are PaddingFreeSponge<P24,24,16,8> of each row (row_words base
+ * words, extension rows flattened coefficient-major), internal nodes
+ * TruncatedPermutation<P16,2,8,16> of split-half pairs. d_layers receives all
+ * 2 * 2^log_h - 1 digests (8 words each), leaves first; root_host (8 words) the root. */
+int whir_merkle_commit(whir_ctx* ctx, const uint32_t* d_rows, uint32_t log_h, uint32_t row_words,
+                       uint32_t* d_layers, uint32_t* root_host, void* stream);
+
+/* Poseidon2 permutation of n independent states (width 16 or 24), in place. */
+int whir_poseidon2_permute(whir_ctx* ctx, int width, uint32_t* d_states, uint64_t n, void* stream);
+
+/* ---- whole prover (Whir::commit + claims + Whir::open) ---- */
+
+int whir_prover_create(whir_ctx* ctx, const whir_params* params, whir_prover** out);
+int whir_prover_destroy(whir_prover* prover);
+/* Upper bound of the serialized proof length for this shape, in bytes. */
+size_t whir_prover_proof_capacity(const whir_prover* prover);
+/* Device buffer of 2^k words owned by the prover, usable as d_poly. */
+uint32_t* whir_prover_input(whir_prover* prover);
+/* Enqueues the whole proof of d_poly (2^k base coefficients) on `stream`; no host
+ * synchronisation. The proof is the PoseidonWriter byte stream of
+ * run_whir_poseidon's prover (crates/whir/src/test.rs:218-235). */
+int whir_prover_enqueue(whir_prover* prover, const uint32_t* d_poly, void* stream);
+/* Waits for the enqueued proof and copies it out: `proof` (capacity `cap` bytes),
+ * its length, the test's checkpoint draw (test.rs:233) and the commitment root. */
+int whir_prover_result(whir_prover* prover, uint8_t* proof, size_t cap, size_t* len, uint32_t* checkpoint,
+                       uint32_t* root8);
+
+/* Host-to-host convenience: upload, prove, download. */
+int whir_prove_host(whir_ctx* ctx, const whir_params* params, const uint32_t* poly, uint8_t* proof, size_t cap,
+                    size_t* len, uint32_t* checkpoint, uint32_t* root8);
+
+/* ---- device memory helpers (plain HIP allocations) ---- */
+int whir_device_alloc(size_t bytes, void** out);
+int whir_device_free(void* ptr);
+int whir_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int whir_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int whir_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WHIR_GPU_H */

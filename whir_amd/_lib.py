@@ -1,0 +1,86 @@
+"""Loads libwhir_amd.so (built in-tree by whir_amd/build.py) and declares the
+C ABI of include/whir_gpu.h for ctypes."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(_HERE, "libwhir_amd.so")
+
+STATUS = {
+    0: "WHIR_OK",
+    1: "WHIR_ETRANSCRIPT",
+    2: "WHIR_EVERIFY",
+    3: "WHIR_EIO",
+    4: "WHIR_EINVAL",
+    5: "WHIR_EOOM",
+    6: "WHIR_EDEVICE",
+}
+
+
+class WhirError(RuntimeError):
+    def __init__(self, code, where):
+        super().__init__(f"{where}: {STATUS.get(code, code)}")
+        self.code = code
+
+
+class whir_params(ctypes.Structure):
+    _fields_ = [
+        ("k", ctypes.c_int32),
+        ("folding", ctypes.c_int32),
+        ("rate", ctypes.c_int32),
+        ("initial_reduction", ctypes.c_int32),
+        ("soundness", ctypes.c_int32),
+        ("security_level", ctypes.c_int32),
+        ("n_points", ctypes.c_int32),
+    ]
+
+
+# (name, restype, argtypes) for every entry point declared in include/whir_gpu.h
+_vp = ctypes.c_void_p
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+SIGNATURES = [
+    ("whir_ctx_create", ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    ("whir_ctx_destroy", ctypes.c_int, [_vp]),
+    ("whir_dft_batch", ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp]),
+    ("whir_merkle_commit", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp, _u32p, _vp]),
+    ("whir_poseidon2_permute", ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint64, _vp]),
+    ("whir_prover_create", ctypes.c_int, [_vp, ctypes.POINTER(whir_params), ctypes.POINTER(_vp)]),
+    ("whir_prover_destroy", ctypes.c_int, [_vp]),
+    ("whir_prover_proof_capacity", ctypes.c_size_t, [_vp]),
+    ("whir_prover_input", _vp, [_vp]),
+    ("whir_prover_enqueue", ctypes.c_int, [_vp, _vp, _vp]),
+    ("whir_prover_result", ctypes.c_int,
+     [_vp, _u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
+    ("whir_prove_host", ctypes.c_int,
+     [_vp, ctypes.POINTER(whir_params), _u32p, _u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
+    ("whir_device_alloc", ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    ("whir_device_free", ctypes.c_int, [_vp]),
+    ("whir_memcpy_h2d", ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    ("whir_memcpy_d2h", ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    ("whir_device_synchronize", ctypes.c_int, []),
+]
+
+_lib = None
+
+
+def lib():
+    """The loaded library. Raises if it has not been built: there is no fallback."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(library_path):
+            raise RuntimeError(
+                f"{library_path} is missing; build it with `python whir_amd/build.py` "
+                "(or __graft_entry__.build()). The WHIR path has no CPU fallback.")
+        L = ctypes.CDLL(library_path)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, where):
+    if rc != 0:
+        raise WhirError(rc, where)

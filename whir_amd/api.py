@@ -1,0 +1,189 @@
+"""Python mirror of the reference's prover API over the HIP C ABI."""
+import ctypes
+
+import numpy as np
+
+from ._lib import check, lib, whir_params
+
+UNIQUE_DECODING, JOHNSON_BOUND, CAPACITY_BOUND = 0, 1, 2
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+def _u32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+class DeviceBuffer:
+    """A plain hipMalloc allocation owned by Python."""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        check(lib().whir_device_alloc(max(int(nbytes), 4), ctypes.byref(p)), "whir_device_alloc")
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        check(lib().whir_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes), "whir_memcpy_h2d")
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        check(lib().whir_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes), "whir_memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().whir_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """whir_ctx: device, Poseidon2 constants (seed 1000 by default), twiddles."""
+
+    def __init__(self, device=0, poseidon_seed=1000):
+        p = ctypes.c_void_p()
+        check(lib().whir_ctx_create(device, poseidon_seed, ctypes.byref(p)), "whir_ctx_create")
+        self.ptr = p.value
+
+    def close(self):
+        if self.ptr:
+            lib().whir_ctx_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context()
+    return _default_ctx
+
+
+def dft_batch(mat, log_h, ctx=None):
+    """Column DFT of `mat` (n_in rows x width, Montgomery words) zero-padded to
+    2^log_h rows (TwoAdicSubgroupDft::dft_batch, whir.rs:34)."""
+    ctx = ctx or default_context()
+    m = _u32(mat)
+    n_in, width = m.shape
+    din = DeviceBuffer(m.nbytes)
+    din.upload(m)
+    dout = DeviceBuffer((width << log_h) * 4)
+    check(lib().whir_dft_batch(ctx.ptr, din.ptr, n_in, log_h, width, dout.ptr, None), "whir_dft_batch")
+    check(lib().whir_device_synchronize(), "sync")
+    return dout.download(np.uint32, width << log_h).reshape(1 << log_h, width)
+
+
+def merkle_commit(rows, ctx=None):
+    """Poseidon2 Merkle tree of `rows` (height x row_words); returns (root, layers)."""
+    ctx = ctx or default_context()
+    r = _u32(rows)
+    h, w = r.shape
+    log_h = h.bit_length() - 1
+    assert 1 << log_h == h
+    dr = DeviceBuffer(r.nbytes)
+    dr.upload(r)
+    dl = DeviceBuffer((2 * h - 1) * 32)
+    root = np.zeros(8, np.uint32)
+    check(lib().whir_merkle_commit(ctx.ptr, dr.ptr, log_h, w, dl.ptr, root.ctypes.data_as(_u32p), None),
+          "whir_merkle_commit")
+    layers = dl.download(np.uint32, (2 * h - 1) * 8).reshape(2 * h - 1, 8)
+    return root, layers
+
+
+def poseidon2_permute(states, width, cooperative=False, ctx=None):
+    """Poseidon2-{16,24} of each row of `states` (n x width)."""
+    ctx = ctx or default_context()
+    s = _u32(states).reshape(-1, width)
+    d = DeviceBuffer(s.nbytes)
+    d.upload(s)
+    w = -16 if cooperative else width
+    check(lib().whir_poseidon2_permute(ctx.ptr, w, d.ptr, s.shape[0], None), "whir_poseidon2_permute")
+    check(lib().whir_device_synchronize(), "sync")
+    return d.download(np.uint32, s.size).reshape(s.shape)
+
+
+class Prover:
+    """Device workspace for one proof shape (whir_prover)."""
+
+    def __init__(self, params, ctx=None):
+        self.ctx = ctx or default_context()
+        self.params = params
+        p = ctypes.c_void_p()
+        check(lib().whir_prover_create(self.ctx.ptr, ctypes.byref(params), ctypes.byref(p)), "whir_prover_create")
+        self.ptr = p.value
+        self.capacity = lib().whir_prover_proof_capacity(self.ptr)
+        self.input_ptr = lib().whir_prover_input(self.ptr)
+
+    def upload(self, poly):
+        a = _u32(poly)
+        assert a.size == 1 << self.params.k
+        check(lib().whir_memcpy_h2d(self.input_ptr, a.ctypes.data, a.nbytes), "whir_memcpy_h2d")
+
+    def enqueue(self, d_poly=None, stream=None):
+        check(lib().whir_prover_enqueue(self.ptr, d_poly or self.input_ptr, stream), "whir_prover_enqueue")
+
+    def result(self):
+        buf = np.zeros(self.capacity, np.uint8)
+        ln = ctypes.c_size_t()
+        cp = np.zeros(1, np.uint32)
+        root = np.zeros(8, np.uint32)
+        check(lib().whir_prover_result(self.ptr, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
+                                       cp.ctypes.data_as(_u32p), root.ctypes.data_as(_u32p)), "whir_prover_result")
+        return bytes(buf[: ln.value]), int(cp[0]), root
+
+    def close(self):
+        if self.ptr:
+            lib().whir_prover_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Whir:
+    """Mirror of ``Whir::new`` (crates/whir/src/whir.rs:103-125) with the
+    reference's Poseidon2 trees and transcript built in."""
+
+    def __init__(self, k, folding, rate, initial_reduction, soundness=JOHNSON_BOUND, security_level=123,
+                 ctx=None):
+        self.k, self.folding, self.rate = k, folding, rate
+        self.initial_reduction = initial_reduction
+        self.soundness, self.security_level = soundness, security_level
+        self.ctx = ctx or default_context()
+        self._provers = {}
+
+    def params(self, n_points):
+        return whir_params(self.k, self.folding, self.rate, self.initial_reduction, self.soundness,
+                           self.security_level, n_points)
+
+    def prover(self, n_points):
+        if n_points not in self._provers:
+            self._provers[n_points] = Prover(self.params(n_points), self.ctx)
+        return self._provers[n_points]
+
+    def prove(self, poly, n_points=1):
+        """commit + n_points evaluation claims + open. Returns (proof bytes,
+        checkpoint draw, commitment root)."""
+        pr = self.prover(n_points)
+        pr.upload(poly)
+        pr.enqueue()
+        return pr.result()

@@ -1,0 +1,587 @@
+// Transcript-ordered ("serial") kernels of the WHIR prover. Each one is a single
+// workgroup whose first wavefront runs the device transcript (tx.h) while the
+// other threads do the small parallel arithmetic around it. Together with the
+// data-parallel kernels they reproduce Whir::commit / Whir::open
+// (crates/whir/src/whir.rs:127-359) and ProverStack / ProverLayout /
+// SumcheckProver (crates/whir/src/sumcheck/prover.rs:31-561) without any host
+// round trip.
+#pragma once
+#include "field.h"
+#include "tx.h"
+#include "sumcheck.h"
+
+namespace kb {
+
+constexpr int MAX_K = 32;
+constexpr int MAX_F = 5;
+constexpr int MAX_SVO_CLAIMS = 10;
+constexpr int MAX_STIR = 1024;
+constexpr int SERIAL_THREADS = 256;
+
+struct ClaimSlot {
+  Ext point[MAX_K];
+  Ext compressed[1 << MAX_F];
+  Ext eval;
+  uint32_t k;
+  uint32_t pad[3];
+};
+
+struct JobScalars {
+  Ext alpha;
+  Ext rs[MAX_K + 8];  // sumcheck challenges in draw order (SVO first)
+  Ext eq_rs[1 << MAX_F];
+  uint32_t pending_words;  // proof words reserved by the last opening kernel
+  uint32_t stir_count;
+  uint32_t checkpoint;
+  uint32_t pad;
+};
+
+// Per-query state (structure of arrays, MAX_STIR entries each).
+struct QueryArrays {
+  uint32_t* index;
+  uint32_t* ws;     // omega^index, squared each fold round (prover.rs:463-469)
+  uint32_t* ts;     // omega^-(index mod mid)            (prover.rs:471-481)
+  uint32_t* var;    // pow-claim variable omega^index     (whir.rs:242-246)
+  Ext* scale;       // alpha^(n_eq + q), updated by fold rounds
+  Ext* G;           // fold-with-code values, 2^d per query
+};
+
+__device__ __forceinline__ Ext wave_sum_ext(Ext v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v.c[c] = add(v.c[c], (uint32_t)__shfl_xor((int)v.c[c], m));
+  }
+  return v;
+}
+
+__device__ __forceinline__ bool is_wave0() { return threadIdx.x < 64; }
+
+__device__ __forceinline__ void tx_begin(WaveTx& tx, DevTx* t, uint32_t* log, JobScalars* js) {
+  tx.load(t, log);
+  const uint32_t pend = js->pending_words;
+  tx.log_words += pend;
+  if (tx.log_words > tx.log_cap) tx.overflow = 1;
+}
+__device__ __forceinline__ void tx_end(WaveTx& tx, DevTx* t, JobScalars* js) {
+  tx.store(t);
+  if (threadIdx.x == 0) js->pending_words = 0;
+}
+
+// lagrange_weights_012 (sumcheck/mod.rs:17-23)
+__device__ __forceinline__ void lagrange012(const Ext& r, Ext (&l)[3]) {
+  const Ext one = eone(), two = efrom(TWO);
+  l[0] = emul_base(emul(esub(r, one), esub(r, two)), HALF);
+  l[1] = emul(r, esub(two, r));
+  l[2] = emul_base(emul(r, esub(r, one)), HALF);
+}
+
+__global__ void tx_init_kernel(DevTx* t, JobScalars* js, uint32_t cap) {
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < 16; ++i) t->state[i] = 0;
+    for (int i = 0; i < 8; ++i) t->in_buf[i] = t->out_buf[i] = 0;
+    t->in_len = t->out_len = 0;
+    t->log_words = 0;
+    t->log_cap = cap;
+    t->overflow = 0;
+    js->pending_words = 0;
+    js->stir_count = 0;
+  }
+}
+
+// transcript.write(root) (poseidon_packed.rs:136-140)
+__global__ void __launch_bounds__(64) write_root_kernel(DevTx* t, uint32_t* log, JobScalars* js, const uint32_t* root) {
+  WaveTx tx;
+  tx_begin(tx, t, log, js);
+  uint32_t d[8];
+  for (int j = 0; j < 8; ++j) d[j] = root[j];
+  tx.write_digest(d);
+  tx_end(tx, t, js);
+}
+
+// Draw the claim point: point = expand(k, draw()) (test.rs:225-226, prover.rs:240,
+// whir.rs:227-229), then build the eq tables of point[fsv..k) split at m0.
+__global__ void __launch_bounds__(SERIAL_THREADS) claim_begin_kernel(DevTx* t, uint32_t* log, JobScalars* js,
+                                                                     ClaimSlot* slot, Ext* eq0, Ext* eq1, uint32_t k,
+                                                                     uint32_t fsv, uint32_t m0, uint32_t m1) {
+  __shared__ Ext s_pt[MAX_K];
+  if (is_wave0()) {
+    WaveTx tx;
+    tx_begin(tx, t, log, js);
+    Ext z = tx.sample_ext();
+    tx_end(tx, t, js);
+    if (threadIdx.x == 0) {
+      for (uint32_t i = 0; i < k; ++i) {
+        s_pt[i] = z;
+        slot->point[i] = z;
+        z = esqr(z);
+      }
+      slot->k = k;
+    }
+  }
+  __syncthreads();
+  build_eq_block(eq0, s_pt + fsv, (int)m0, eone());
+  build_eq_block(eq1, s_pt + fsv + m0, (int)m1, eone());
+}
+
+// Reduce the evaluation partials and write the claim's value (prover.rs:87,276;
+// whir.rs:230). fsv > 0: partials hold 2^fsv compressed values per block
+// (SvoPoint::eval, svo.rs:77-91); fsv == 0: one value per block (eval_ext).
+__global__ void __launch_bounds__(SERIAL_THREADS) claim_end_kernel(DevTx* t, uint32_t* log, JobScalars* js,
+                                                                   ClaimSlot* slot, const Ext* partials,
+                                                                   uint32_t n_part, uint32_t fsv) {
+  __shared__ Ext sh[SERIAL_THREADS];
+  __shared__ Ext s_comp[1 << MAX_F];
+  __shared__ Ext s_eq[1 << MAX_F];
+  const uint32_t W = 1u << fsv;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lanes_per = SERIAL_THREADS / W;
+  {
+    const uint32_t i = tid % W, p0 = tid / W;
+    Ext acc = ezero();
+    for (uint32_t p = p0; p < n_part; p += lanes_per) acc = eadd(acc, ld_ext(partials + (uint64_t)p * W + i));
+    sh[tid] = acc;
+    __syncthreads();
+    if (tid < W) {
+      Ext s = ezero();
+      for (uint32_t r = 0; r < lanes_per; ++r) s = eadd(s, sh[r * W + tid]);
+      s_comp[tid] = s;
+      slot->compressed[tid] = s;
+    }
+    __syncthreads();
+  }
+  Ext eval;
+  if (fsv == 0) {
+    eval = s_comp[0];
+  } else {
+    if (tid < 64) {
+      // eq(svo)[i] for i < W, svo = point[0..fsv)
+      for (uint32_t i = tid; i < W; i += 64) {
+        Ext e = eone();
+        for (uint32_t j = 0; j < fsv; ++j) {
+          const Ext z = slot->point[j];
+          e = emul(e, ((i >> j) & 1) ? z : esub(eone(), z));
+        }
+        s_eq[i] = e;
+      }
+    }
+    __syncthreads();
+    Ext part = ezero();
+    if (tid < 64)
+      for (uint32_t i = tid; i < W; i += 64) part = eadd(part, emul(s_comp[i], s_eq[i]));
+    eval = wave_sum_ext(part);  // meaningful in wave 0
+  }
+  if (is_wave0()) {
+    if (tid == 0) slot->eval = eval;
+    WaveTx tx;
+    tx_begin(tx, t, log, js);
+    tx.write_e(eval);
+    tx_end(tx, t, js);
+  }
+}
+
+// ProverLayout::new_prover (prover.rs:279-353) with Svo::calculate_accumulators
+// (svo.rs:224-278) and the expression scales (prover.rs:184-208):
+//   draw alpha; f SVO rounds (write c0, c2; draw r); eq(rs) into js->eq_rs;
+//   eq1 tables of every claim scaled by alpha^c * eq(svo_c, rs).
+// Claims are in layout order (user claims, then the OOD "virtual" claims).
+__global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t* log, JobScalars* js,
+                                                             const ClaimSlot* slots, uint32_t n_claims, uint32_t f,
+                                                             Ext* pool, uint64_t pool_stride, uint32_t m0,
+                                                             uint32_t m1) {
+  // partial evals pe[c][2^l - 2 + a] (l = 1..f), accumulators acc[c][ri][side][u]
+  constexpr int PE_PER = (2 << MAX_F) - 2;
+  constexpr int ACC_PER = 243 - 1;  // sum_{ri<5} 2 * 3^ri
+  __shared__ Ext s_pe[MAX_SVO_CLAIMS * PE_PER];
+  __shared__ Ext s_acc[MAX_SVO_CLAIMS * ACC_PER];
+  __shared__ Ext s_tot[ACC_PER];
+  __shared__ Ext s_apow[MAX_SVO_CLAIMS];
+  __shared__ Ext s_rs[MAX_F];
+  __shared__ Ext s_lag[MAX_F][3];
+  __shared__ Ext s_scale[MAX_SVO_CLAIMS];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t W = 1u << f;
+  uint32_t acc_per = 0, pow3 = 1;
+  for (uint32_t ri = 0; ri < f; ++ri) {
+    acc_per += 2 * pow3;
+    pow3 *= 3;
+  }
+  const uint32_t pe_per = 2 * W - 2;
+
+  // (1) partial evals: pe[l][a] = sum_b comp[b*2^l + a] * eq(svo[l..f))[b]
+  for (uint32_t it = tid; it < n_claims * pe_per; it += SERIAL_THREADS) {
+    const uint32_t c = it / pe_per, o = it % pe_per;
+    uint32_t l = 1;
+    while ((2u << l) - 2 <= o) ++l;
+    const uint32_t a = o - ((1u << l) - 2);
+    const ClaimSlot& sl = slots[c];
+    Ext s = ezero();
+    for (uint32_t b = 0; b < (1u << (f - l)); ++b) {
+      Ext e = eone();
+      for (uint32_t j = 0; j < f - l; ++j) {
+        const Ext z = sl.point[l + j];
+        e = emul(e, ((b >> j) & 1) ? z : esub(eone(), z));
+      }
+      s = eadd(s, emul(sl.compressed[(b << l) + a], e));
+    }
+    s_pe[c * PE_PER + o] = s;
+  }
+  __syncthreads();
+
+  // (2) accumulators per claim (weight 1)
+  for (uint32_t it = tid; it < n_claims * acc_per; it += SERIAL_THREADS) {
+    const uint32_t c = it / acc_per;
+    uint32_t o = it % acc_per;
+    uint32_t ri = 0, p3 = 1;
+    while (o >= 2 * p3) {
+      o -= 2 * p3;
+      p3 *= 3;
+      ++ri;
+    }
+    const uint32_t side = o / p3, u = o % p3;
+    const uint32_t l = ri + 1;
+    uint32_t pt[MAX_F];
+    uint32_t uu = u;
+    for (uint32_t j = 0; j < ri; ++j) {
+      pt[j] = uu % 3;
+      uu /= 3;
+    }
+    pt[ri] = side ? 2u : 0u;
+    const ClaimSlot& sl = slots[c];
+    // eval_eq_xy(pt, svo[0..l)) with pt_j in {0,1,2}: 1-y, y, 3y-1
+    Ext exy = eone();
+    for (uint32_t j = 0; j < l; ++j) {
+      const Ext y = sl.point[j];
+      Ext term;
+      if (pt[j] == 0) term = esub(eone(), y);
+      else if (pt[j] == 1) term = y;
+      else term = esub(eadd(edbl(y), y), eone());
+      exy = emul(exy, term);
+    }
+    // <pe[l], eq(pt)>; eq(pt)[a] = prod_j (a_j ? pt_j : 1 - pt_j), small integers
+    Ext dot = ezero();
+    for (uint32_t a = 0; a < (1u << l); ++a) {
+      int32_t coef = 1;
+      for (uint32_t j = 0; j < l; ++j) coef *= ((a >> j) & 1) ? (int32_t)pt[j] : 1 - (int32_t)pt[j];
+      if (coef == 0) continue;
+      const uint32_t mag = from_canonical((uint32_t)(coef < 0 ? -coef : coef));
+      const Ext term = emul_base(s_pe[c * PE_PER + (1u << l) - 2 + a], mag);
+      dot = coef < 0 ? esub(dot, term) : eadd(dot, term);
+    }
+    s_acc[c * ACC_PER + (it % acc_per)] = emul(exy, dot);
+  }
+  __syncthreads();
+
+  // (3) draw alpha, powers alpha^c
+  WaveTx tx;
+  if (is_wave0()) {
+    tx_begin(tx, t, log, js);
+    const Ext alpha = tx.sample_ext();
+    if (tid == 0) {
+      Ext a = eone();
+      for (uint32_t c = 0; c < n_claims; ++c) {
+        s_apow[c] = a;
+        a = emul(a, alpha);
+      }
+      js->alpha = alpha;
+    }
+  }
+  __syncthreads();
+  // (4) combined accumulators
+  for (uint32_t o = tid; o < acc_per; o += SERIAL_THREADS) {
+    Ext s = ezero();
+    for (uint32_t c = 0; c < n_claims; ++c) s = eadd(s, emul(s_apow[c], s_acc[c * ACC_PER + o]));
+    s_tot[o] = s;
+  }
+  __syncthreads();
+  // (5) SVO rounds
+  if (is_wave0()) {
+    uint32_t off = 0, p3 = 1;
+    for (uint32_t ri = 0; ri < f; ++ri) {
+      Ext c0 = ezero(), c2 = ezero();
+      for (uint32_t u = tid; u < p3; u += 64) {
+        Ext w = eone();
+        uint32_t uu = u;
+        for (uint32_t j = 0; j < ri; ++j) {
+          w = emul(w, s_lag[j][uu % 3]);
+          uu /= 3;
+        }
+        c0 = eadd(c0, emul(s_tot[off + u], w));
+        c2 = eadd(c2, emul(s_tot[off + p3 + u], w));
+      }
+      c0 = wave_sum_ext(c0);
+      c2 = wave_sum_ext(c2);
+      tx.write_e(c0);
+      tx.write_e(c2);
+      const Ext r = tx.sample_ext();
+      if (tid == 0) {
+        s_rs[ri] = r;
+        js->rs[ri] = r;
+        Ext l[3];
+        lagrange012(r, l);
+        s_lag[ri][0] = l[0];
+        s_lag[ri][1] = l[1];
+        s_lag[ri][2] = l[2];
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      off += 2 * p3;
+      p3 *= 3;
+    }
+    tx_end(tx, t, js);
+  }
+  __syncthreads();
+  // (6) eq(rs) and claim scales alpha^c * eq_xy(svo_c, rs)
+  if (tid < W) {
+    Ext e = eone();
+    for (uint32_t j = 0; j < f; ++j) e = emul(e, ((tid >> j) & 1) ? s_rs[j] : esub(eone(), s_rs[j]));
+    js->eq_rs[tid] = e;
+  }
+  if (tid < n_claims) {
+    Ext e = s_apow[tid];
+    for (uint32_t j = 0; j < f; ++j) {
+      const Ext x = slots[tid].point[j], y = s_rs[j];
+      // (y x) * 2 - x - y + 1
+      e = emul(e, eadd(esub(esub(edbl(emul(y, x)), x), y), eone()));
+    }
+    s_scale[tid] = e;
+  }
+  __syncthreads();
+  const uint64_t n0 = 1ull << m0, n1 = 1ull << m1;
+  for (uint64_t it = tid; it < (uint64_t)n_claims * n1; it += SERIAL_THREADS) {
+    const uint64_t c = it / n1, x = it % n1;
+    Ext* p = pool + c * pool_stride + n0 + x;
+    st_ext(p, emul(ld_ext(p), s_scale[c]));
+  }
+}
+
+// STIR indexes: n draws of `bits` bits, sorted and deduplicated (whir.rs:63-77).
+__global__ void __launch_bounds__(SERIAL_THREADS) stir_sample_kernel(DevTx* t, uint32_t* log, JobScalars* js,
+                                                                     uint32_t n, uint32_t bits, uint32_t* idx_out,
+                                                                     uint32_t* count_out) {
+  __shared__ uint32_t s_v[MAX_STIR];
+  __shared__ uint8_t s_first[MAX_STIR];
+  __shared__ uint32_t s_count;
+  const uint32_t tid = threadIdx.x;
+  if (is_wave0()) {
+    WaveTx tx;
+    tx_begin(tx, t, log, js);
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t v = tx.sample_bits(bits);
+      if (tid == 0) s_v[i] = v;
+    }
+    tx_end(tx, t, js);
+  }
+  if (tid == 0) s_count = 0;
+  __syncthreads();
+  for (uint32_t i = tid; i < n; i += SERIAL_THREADS) {
+    const uint32_t v = s_v[i];
+    bool first = true;
+    for (uint32_t j = 0; j < i; ++j) first &= s_v[j] != v;
+    s_first[i] = first;
+    if (first) atomicAdd(&s_count, 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < n; i += SERIAL_THREADS) {
+    if (!s_first[i]) continue;
+    const uint32_t v = s_v[i];
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < n; ++j) rank += (s_first[j] && s_v[j] < v) ? 1u : 0u;
+    idx_out[rank] = v;
+  }
+  if (tid == 0) {
+    *count_out = s_count;
+    js->stir_count = s_count;
+  }
+}
+
+// Merkle openings of the sorted queries (MerkleTree::query, poseidon_packed.rs:
+// 145-233): per query the flattened row, then `depth` sibling digests bottom-up,
+// as transcript hints. One block per query; offsets are fixed per query, so
+// blocks write in parallel and the next transcript kernel advances the log.
+// mode 1 (round 0, fold_with_code): also G[t] = <eq(rs), row(index mod M + t M)>
+// for t < 2^d (prover.rs:483-506) and ws/ts (prover.rs:463-481).
+// mode 2 (later rounds): pow-claim variable omega^index (whir.rs:242-246).
+__global__ void __launch_bounds__(SERIAL_THREADS) stir_query_kernel(const DevTx* t, JobScalars* js, uint32_t* log,
+                                                                    const uint32_t* idx, const uint32_t* count_ptr,
+                                                                    const uint32_t* cw, uint32_t row_words,
+                                                                    const uint32_t* tree, uint32_t depth, int mode,
+                                                                    uint32_t d, QueryArrays qa) {
+  const uint32_t count = *count_ptr;
+  const uint32_t q = blockIdx.x;
+  const uint32_t per = row_words + 8 * depth;
+  if (blockIdx.x == 0 && threadIdx.x == 0) js->pending_words = count * per;
+  if (q >= count) return;
+  const uint32_t index = idx[q];
+  const uint64_t base = (uint64_t)t->log_words + (uint64_t)q * per;
+  const bool fits = base + per <= t->log_cap;
+  if (fits) {
+    for (uint32_t w = threadIdx.x; w < row_words; w += SERIAL_THREADS)
+      log[base + w] = cw[(uint64_t)index * row_words + w];
+    for (uint32_t e = threadIdx.x; e < 8 * depth; e += SERIAL_THREADS) {
+      const uint32_t i = e >> 3, w = e & 7;
+      const uint32_t mid = 1u << (depth - i - 1);
+      const uint32_t cur = index & ((mid << 1) - 1);
+      const uint32_t sib = cur ^ mid;
+      const uint64_t layer_off = (2ull << depth) - (2ull << (depth - i));  // sum_{j<i} 2^(depth-j)
+      log[base + row_words + e] = tree[(layer_off + sib) * 8 + w];
+    }
+  }
+  if (mode == 1) {
+    // rows index mod M + t*M, M = 2^(depth - d); each row has 2^f = row_words base words
+    const uint32_t M = 1u << (depth - d);
+    const uint32_t b = index & (M - 1);
+    for (uint32_t tt = threadIdx.x; tt < (1u << d); tt += SERIAL_THREADS) {
+      const uint32_t* row = cw + (uint64_t)(b + tt * M) * row_words;
+      Ext acc = ezero();
+      for (uint32_t i = 0; i < row_words; ++i) acc = eadd(acc, emul_base(js->eq_rs[i], row[i]));
+      qa.G[q * (1u << MAX_F) + tt] = acc;
+    }
+    if (threadIdx.x == 0) {
+      const uint32_t om = two_adic_generator(depth);
+      qa.index[q] = index;
+      qa.ws[q] = pow(om, index);
+      qa.ts[q] = inv(pow(om, index & ((1u << (depth - 1)) - 1)));
+    }
+  } else if (mode == 2) {
+    if (threadIdx.x == 0) {
+      qa.index[q] = index;
+      qa.var[q] = pow(two_adic_generator(depth), index);
+    }
+  }
+}
+
+// SumcheckProver::fold / fold_with_code prologue (prover.rs:377-384, 399-419):
+// draw alpha; scale the eq1 table of OOD claim c by alpha^c; query scales
+// alpha^(n_eq + q).
+__global__ void __launch_bounds__(SERIAL_THREADS) fold_begin_kernel(DevTx* t, uint32_t* log, JobScalars* js,
+                                                                    Ext* eq_pool, uint64_t eq_stride, uint32_t n_eq,
+                                                                    uint32_t m0, uint32_t m1, QueryArrays qa,
+                                                                    const uint32_t* count_ptr) {
+  __shared__ Ext s_alpha;
+  __shared__ Ext s_apow[16];
+  const uint32_t tid = threadIdx.x;
+  if (is_wave0()) {
+    WaveTx tx;
+    tx_begin(tx, t, log, js);
+    const Ext a = tx.sample_ext();
+    tx_end(tx, t, js);
+    if (tid == 0) {
+      s_alpha = a;
+      js->alpha = a;
+      Ext p = eone();
+      for (uint32_t c = 0; c < n_eq && c < 16; ++c) {
+        s_apow[c] = p;
+        p = emul(p, a);
+      }
+    }
+  }
+  __syncthreads();
+  const uint64_t n0 = 1ull << m0, n1 = 1ull << m1;
+  for (uint64_t it = tid; it < (uint64_t)n_eq * n1; it += SERIAL_THREADS) {
+    const uint64_t c = it / n1, x = it % n1;
+    if (c == 0) continue;
+    Ext* p = eq_pool + c * eq_stride + n0 + x;
+    st_ext(p, emul(ld_ext(p), s_apow[c]));
+  }
+  const uint32_t count = count_ptr ? *count_ptr : 0;
+  for (uint32_t q = tid; q < count; q += SERIAL_THREADS) qa.scale[q] = epow(s_alpha, (uint64_t)n_eq + q);
+}
+
+// Reduce the (c0, c2) partials, add the fold_with_code query terms
+// (prover.rs:509-534), write c0, c2, draw r (expr.rs:52-70), and advance the
+// per-query state (prover.rs:542-553).
+__global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(
+    DevTx* t, uint32_t* log, JobScalars* js, const Ext* partials, uint32_t n_part, uint32_t rs_index, int fwc,
+    QueryArrays qa, const uint32_t* count_ptr, uint32_t d, uint32_t k_data, uint32_t rho, const uint32_t* tw_lo,
+    const uint32_t* tw_hi) {
+  __shared__ Ext sh[SERIAL_THREADS];
+  __shared__ Ext s_r;
+  const uint32_t tid = threadIdx.x;
+  Ext c0 = ezero(), c2 = ezero();
+  for (uint32_t p = tid; p < n_part; p += SERIAL_THREADS) {
+    c0 = eadd(c0, ld_ext(partials + 2 * p));
+    c2 = eadd(c2, ld_ext(partials + 2 * p + 1));
+  }
+  const uint32_t count = fwc ? *count_ptr : 0;
+  const uint32_t GS = 1u << MAX_F;
+  if (fwc) {
+    const uint32_t half = 1u << (d - rho - 1);  // t_hi = t_lo + half
+    for (uint32_t q = tid; q < count; q += SERIAL_THREADS) {
+      const uint32_t index = qa.index[q];
+      const uint32_t t_lo = (index >> (k_data - d)) & (half - 1);
+      const Ext g_lo = qa.G[q * GS + t_lo], g_hi = qa.G[q * GS + t_lo + half];
+      const Ext even = emul_base(eadd(g_lo, g_hi), HALF);
+      const Ext odd = emul_base(emul_base(esub(g_lo, g_hi), HALF), qa.ts[q]);
+      const Ext s = qa.scale[q];
+      c0 = eadd(c0, emul(s, even));
+      const uint32_t wt = sub(dbl(qa.ws[q]), ONE);
+      c2 = eadd(c2, emul_base(emul(s, esub(edbl(odd), even)), wt));
+    }
+  }
+  const Ext C0 = block_reduce_ext<SERIAL_THREADS>(c0, sh);
+  __syncthreads();
+  const Ext C2 = block_reduce_ext<SERIAL_THREADS>(c2, sh);
+  if (is_wave0()) {
+    WaveTx tx;
+    tx_begin(tx, t, log, js);
+    tx.write_e(C0);
+    tx.write_e(C2);
+    const Ext r = tx.sample_ext();
+    tx_end(tx, t, js);
+    if (tid == 0) {
+      s_r = r;
+      js->rs[rs_index] = r;
+    }
+  }
+  __syncthreads();
+  if (!fwc) return;
+  const Ext r = s_r;
+  for (uint32_t q = tid; q < count; q += SERIAL_THREADS) {
+    const uint32_t index = qa.index[q];
+    const uint32_t ws = qa.ws[q];
+    // s *= 1 + r (w - 1)
+    qa.scale[q] = emul(qa.scale[q], eadd(eone(), emul(r, esub(efrom(ws), eone()))));
+    qa.ws[q] = sqr(ws);
+    uint32_t ts = sqr(qa.ts[q]);
+    if (rho + 2 <= k_data && ((index >> (k_data - rho - 2)) & 1)) ts = neg(ts);
+    qa.ts[q] = ts;
+    // G_{rho+1}[tt] = fold(G[tt], G[tt + h], omega_{2^(k_data-rho)}^{-i}, r), i = b + tt*M
+    if (rho + 1 < d) {
+      const uint32_t h = 1u << (d - rho - 1);
+      const uint32_t M = 1u << (k_data - d);
+      const uint32_t b = index & (M - 1);
+      const uint32_t lg = k_data - rho;
+      for (uint32_t tt = 0; tt < h; ++tt) {
+        const uint32_t i = b + tt * M;
+        const uint32_t e = ((1u << lg) - i) & ((1u << lg) - 1);
+        const uint32_t e24 = e << (24 - lg);
+        const uint32_t tw = mul(tw_lo[e24 & 4095u], tw_hi[e24 >> 12]);
+        const Ext g_lo = qa.G[q * GS + tt], g_hi = qa.G[q * GS + tt + h];
+        const Ext even = emul_base(eadd(g_lo, g_hi), HALF);
+        const Ext odd = emul_base(emul_base(esub(g_lo, g_hi), HALF), tw);
+        qa.G[q * GS + tt] = eadd(even, emul(r, esub(odd, even)));
+      }
+    }
+  }
+}
+
+// transcript.write_many(final poly) (whir.rs:317).
+__global__ void __launch_bounds__(64) write_final_kernel(DevTx* t, uint32_t* log, JobScalars* js, const Ext* poly,
+                                                         uint32_t n) {
+  WaveTx tx;
+  tx_begin(tx, t, log, js);
+  for (uint32_t i = 0; i < n; ++i) tx.write_e(poly[i]);
+  tx_end(tx, t, js);
+}
+
+// The test's checkpoint draw (test.rs:233) after the last openings.
+__global__ void __launch_bounds__(64) checkpoint_kernel(DevTx* t, uint32_t* log, JobScalars* js) {
+  WaveTx tx;
+  tx_begin(tx, t, log, js);
+  const uint32_t v = tx.sample();
+  tx_end(tx, t, js);
+  if (threadIdx.x == 0) js->checkpoint = v;
+}
+
+}  // namespace kb

@@ -1,0 +1,262 @@
+// Data-parallel sumcheck / eq-polynomial kernels (the HBM passes of
+// crates/poly/src/eq.rs and crates/whir/src/sumcheck/{expr,combine,prover}.rs).
+// Conventions (crates/poly/src/point.rs:110-126): eq(z)[x] = prod_j (x_j ? z_j : 1 - z_j)
+// with bit j of x bound to z_j; an eq table over m variables is stored split as
+// eq0 (low m0 bits) and eq1 (high bits): eq[x] = eq0[x & (2^m0-1)] * eq1[x >> m0].
+#pragma once
+#include "field.h"
+
+namespace kb {
+
+constexpr int SC_THREADS = 256;
+
+__device__ __forceinline__ Ext ld_ext(const Ext* p) {
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  return Ext{{v.x, v.y, v.z, v.w}};
+}
+__device__ __forceinline__ void st_ext(Ext* p, const Ext& e) {
+  *reinterpret_cast<uint4*>(p) = make_uint4(e.c[0], e.c[1], e.c[2], e.c[3]);
+}
+
+// Sum `n` Ext values held one per thread (n = blockDim) into lane 0 of the block.
+template <int NT>
+__device__ __forceinline__ Ext block_reduce_ext(Ext v, Ext* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = eadd(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// eq table of m variables into `out` by doubling (whole block cooperates).
+__device__ void build_eq_block(Ext* out, const Ext* z, int m, Ext scale) {
+  if (threadIdx.x == 0) out[0] = scale;
+  __syncthreads();
+  for (int i = 0; i < m; ++i) {
+    const uint32_t h = 1u << i;
+    const Ext zi = z[i];
+    for (uint32_t j = threadIdx.x; j < h; j += blockDim.x) {
+      const Ext lo = out[j];
+      const Ext hi = emul(lo, zi);
+      out[j + h] = hi;
+      out[j] = esub(lo, hi);
+    }
+    __syncthreads();
+  }
+}
+
+// ---- claim evaluation on the base polynomial (SplitEq::compress_hi, eq.rs:290-398) ----
+// out[i] = sum_h eq(rest)[h] * poly[h * W + i] for i < W = 2^F_LOG; the block for
+// h1 = blockIdx.x sums over h0 < 2^m0 and writes W partial values * eq1[h1].
+template <int F_LOG>
+__global__ void __launch_bounds__(SC_THREADS) compress_hi_kernel(const uint32_t* __restrict__ poly,
+                                                             const Ext* __restrict__ eq0, const Ext* __restrict__ eq1,
+                                                             uint32_t m0, Ext* __restrict__ partials) {
+  constexpr int W = 1 << F_LOG;
+  constexpr int CPT = W < 4 ? W : 4;        // columns per thread
+  constexpr int TPR = W / CPT;              // threads per row
+  constexpr int RPP = SC_THREADS / TPR;     // rows per pass
+  __shared__ Ext sh[SC_THREADS * CPT];
+  const uint32_t t = threadIdx.x;
+  const uint32_t cg = t % TPR, rl = t / TPR;
+  const uint64_t h1 = blockIdx.x;
+  const uint64_t n0 = 1ull << m0;
+  Ext acc[CPT];
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) acc[c] = ezero();
+  for (uint64_t h0 = rl; h0 < n0; h0 += RPP) {
+    const Ext e = ld_ext(eq0 + h0);
+    const uint32_t* row = poly + ((h0 + (h1 << m0)) << F_LOG) + cg * CPT;
+    uint32_t v[CPT];
+    if constexpr (CPT == 4) {
+      const uint4 q = *reinterpret_cast<const uint4*>(row);
+      v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) v[c] = row[c];
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) acc[c] = eadd(acc[c], emul_base(e, v[c]));
+  }
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) sh[t * CPT + c] = acc[c];
+  __syncthreads();
+  if (t < (uint32_t)W) {
+    const uint32_t g = t / CPT, c = t % CPT;
+    Ext s = ezero();
+    for (uint32_t r = 0; r < RPP; ++r) s = eadd(s, sh[(r * TPR + g) * CPT + c]);
+    st_ext(partials + h1 * W + t, emul(s, ld_ext(eq1 + h1)));
+  }
+}
+
+// ---- evaluation of an extension polynomial (SplitEq::eval_ext, eq.rs:179-250) ----
+__global__ void __launch_bounds__(SC_THREADS) eval_ext_kernel(const Ext* __restrict__ poly, const Ext* __restrict__ eq0,
+                                                          const Ext* __restrict__ eq1, uint32_t m0,
+                                                          Ext* __restrict__ partials) {
+  __shared__ Ext sh[SC_THREADS];
+  const uint64_t h1 = blockIdx.x;
+  const uint64_t n0 = 1ull << m0;
+  Ext acc = ezero();
+  for (uint64_t h0 = threadIdx.x; h0 < n0; h0 += SC_THREADS)
+    acc = eadd(acc, emul(ld_ext(poly + h0 + (h1 << m0)), ld_ext(eq0 + h0)));
+  const Ext s = block_reduce_ext<SC_THREADS>(acc, sh);
+  if (threadIdx.x == 0) st_ext(partials + h1, emul(s, ld_ext(eq1 + h1)));
+}
+
+// ---- expression build after the SVO rounds (ProverLayout::expression, prover.rs:184-229) ----
+// epoly[j] = sum_{i<W} eq_rs[i] * poly[j*W + i]            (compress_lo, eq.rs:491-533)
+// ew[j]    = sum_c eq0_c[j & (2^m0-1)] * eq1s_c[j >> m0]    (combine_into, eq.rs:541-621;
+//            eq1s_c already carries alpha^c * eq(svo_c, rs))
+template <int F_LOG>
+__global__ void __launch_bounds__(SC_THREADS) expr_build_kernel(const uint32_t* __restrict__ poly,
+                                                            const Ext* __restrict__ eq_rs, const Ext* __restrict__ pool,
+                                                            uint32_t n_claims, uint64_t claim_stride, uint32_t m0,
+                                                            uint64_t n, Ext* __restrict__ epoly, Ext* __restrict__ ew) {
+  constexpr int W = 1 << F_LOG;
+  __shared__ Ext s_eq[W];
+  if (threadIdx.x < W) s_eq[threadIdx.x] = eq_rs[threadIdx.x];
+  __syncthreads();
+  const uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t* row = poly + (j << F_LOG);
+  Ext acc = ezero();
+  if constexpr (W >= 4) {
+#pragma unroll
+    for (int i = 0; i < W; i += 4) {
+      const uint4 q = *reinterpret_cast<const uint4*>(row + i);
+      acc = eadd(acc, emul_base(s_eq[i], q.x));
+      acc = eadd(acc, emul_base(s_eq[i + 1], q.y));
+      acc = eadd(acc, emul_base(s_eq[i + 2], q.z));
+      acc = eadd(acc, emul_base(s_eq[i + 3], q.w));
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < W; ++i) acc = eadd(acc, emul_base(s_eq[i], row[i]));
+  }
+  st_ext(epoly + j, acc);
+  const uint64_t lo = j & ((1ull << m0) - 1), hi = j >> m0;
+  const uint64_t n0 = 1ull << m0;
+  Ext w = ezero();
+  for (uint32_t c = 0; c < n_claims; ++c) {
+    const Ext* base = pool + c * claim_stride;
+    w = eadd(w, emul(ld_ext(base + lo), ld_ext(base + n0 + hi)));
+  }
+  st_ext(ew + j, w);
+}
+
+// ---- claim combination (combine_claims, combine.rs:37-64,178-203,407-468) ----
+// w[x] += sum_c eq0_c[x_lo] * eq1s_c[x_hi]  +  sum_j powR[j][x_hi] * powL[j][x_lo]
+// eq1s_c carries alpha^c; powL[j][e] = var_j^e (e < 2^m0), powR[j][e] = scale_j * var_j^(e*2^m0).
+__global__ void __launch_bounds__(SC_THREADS) combine_kernel(Ext* __restrict__ w, uint64_t n, uint32_t m0,
+                                                         const Ext* __restrict__ eq_pool, uint32_t n_eq,
+                                                         uint64_t eq_stride, const uint32_t* __restrict__ powL,
+                                                         const Ext* __restrict__ powR, const uint32_t* n_pow_ptr,
+                                                         uint32_t m1) {
+  const uint64_t x = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
+  if (x >= n) return;
+  const uint32_t n_pow = n_pow_ptr ? *n_pow_ptr : 0u;
+  const uint64_t n0 = 1ull << m0, n1 = 1ull << m1;
+  const uint64_t lo = x & (n0 - 1), hi = x >> m0;
+  Ext acc = ld_ext(w + x);
+  for (uint32_t c = 0; c < n_eq; ++c) {
+    const Ext* base = eq_pool + c * eq_stride;
+    acc = eadd(acc, emul(ld_ext(base + lo), ld_ext(base + n0 + hi)));
+  }
+  for (uint32_t j = 0; j < n_pow; ++j) acc = eadd(acc, emul_base(ld_ext(powR + j * n1 + hi), powL[j * n0 + lo]));
+  st_ext(w + x, acc);
+}
+
+// powL[j][e] = var_j^e for e < 2^m0; powR[j][e] = scale_j * var_j^(e * 2^m0) for e < 2^m1.
+__global__ void __launch_bounds__(SC_THREADS) pow_tables_kernel(const uint32_t* __restrict__ vars,
+                                                            const Ext* __restrict__ scales, const uint32_t* count_ptr,
+                                                            uint32_t m0, uint32_t m1, uint32_t* __restrict__ powL,
+                                                            Ext* __restrict__ powR) {
+  const uint32_t count = *count_ptr;
+  const uint64_t n0 = 1ull << m0, n1 = 1ull << m1;
+  const uint64_t per = n0 + n1;
+  const uint64_t id = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
+  const uint64_t j = id / per, e = id % per;
+  if (j >= count) return;
+  const uint32_t v = vars[j];
+  if (e < n0) {
+    powL[j * n0 + e] = pow(v, e);
+  } else {
+    const uint64_t e1 = e - n0;
+    st_ext(powR + j * n1 + e1, emul_base(scales[j], pow(v, e1 << m0)));
+  }
+}
+
+// ---- sumcheck round coefficients (expr.rs:14-27) and fold (poly.rs:131-142) ----
+// c0 = sum w[2i] f[2i];  c2 = sum (2w[2i+1]-w[2i]) (2f[2i+1]-f[2i]); one (c0,c2)
+// partial per block.
+__device__ __forceinline__ void pair_coeffs(const Ext& f0, const Ext& f1, const Ext& w0, const Ext& w1, Ext& c0,
+                                            Ext& c2) {
+  c0 = eadd(c0, emul(w0, f0));
+  c2 = eadd(c2, emul(esub(edbl(w1), w0), esub(edbl(f1), f0)));
+}
+
+__global__ void __launch_bounds__(SC_THREADS) coeffs_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
+                                                        uint64_t n, Ext* __restrict__ partials) {
+  __shared__ Ext sh[SC_THREADS];
+  Ext c0 = ezero(), c2 = ezero();
+  const uint64_t npairs = n / 2;
+  for (uint64_t i = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; i < npairs; i += (uint64_t)gridDim.x * SC_THREADS)
+    pair_coeffs(ld_ext(f + 2 * i), ld_ext(f + 2 * i + 1), ld_ext(w + 2 * i), ld_ext(w + 2 * i + 1), c0, c2);
+  const Ext s0 = block_reduce_ext<SC_THREADS>(c0, sh);
+  __syncthreads();
+  const Ext s2 = block_reduce_ext<SC_THREADS>(c2, sh);
+  if (threadIdx.x == 0) {
+    st_ext(partials + 2 * blockIdx.x, s0);
+    st_ext(partials + 2 * blockIdx.x + 1, s2);
+  }
+}
+
+// Fold both arrays by r (a[i] = a[2i] + r (a[2i+1] - a[2i])) and emit the next
+// round's (c0, c2) partials computed on the folded values.
+__global__ void __launch_bounds__(SC_THREADS) fold_coeffs_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
+                                                             uint64_t n, const Ext* __restrict__ r_ptr,
+                                                             Ext* __restrict__ fo, Ext* __restrict__ wo,
+                                                             Ext* __restrict__ partials) {
+  __shared__ Ext sh[SC_THREADS];
+  const Ext r = ld_ext(r_ptr);
+  Ext c0 = ezero(), c2 = ezero();
+  const uint64_t nq = n / 4;
+  for (uint64_t i = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; i < nq; i += (uint64_t)gridDim.x * SC_THREADS) {
+    const Ext a0 = ld_ext(f + 4 * i), a1 = ld_ext(f + 4 * i + 1), a2 = ld_ext(f + 4 * i + 2), a3 = ld_ext(f + 4 * i + 3);
+    const Ext b0 = ld_ext(w + 4 * i), b1 = ld_ext(w + 4 * i + 1), b2 = ld_ext(w + 4 * i + 2), b3 = ld_ext(w + 4 * i + 3);
+    const Ext F0 = eadd(a0, emul(r, esub(a1, a0)));
+    const Ext F1 = eadd(a2, emul(r, esub(a3, a2)));
+    const Ext W0 = eadd(b0, emul(r, esub(b1, b0)));
+    const Ext W1 = eadd(b2, emul(r, esub(b3, b2)));
+    st_ext(fo + 2 * i, F0);
+    st_ext(fo + 2 * i + 1, F1);
+    st_ext(wo + 2 * i, W0);
+    st_ext(wo + 2 * i + 1, W1);
+    pair_coeffs(F0, F1, W0, W1, c0, c2);
+  }
+  const Ext s0 = block_reduce_ext<SC_THREADS>(c0, sh);
+  __syncthreads();
+  const Ext s2 = block_reduce_ext<SC_THREADS>(c2, sh);
+  if (threadIdx.x == 0) {
+    st_ext(partials + 2 * blockIdx.x, s0);
+    st_ext(partials + 2 * blockIdx.x + 1, s2);
+  }
+}
+
+__global__ void __launch_bounds__(SC_THREADS) fold_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
+                                                      uint64_t n, const Ext* __restrict__ r_ptr, Ext* __restrict__ fo,
+                                                      Ext* __restrict__ wo) {
+  const Ext r = ld_ext(r_ptr);
+  const uint64_t i = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
+  if (i >= n / 2) return;
+  const Ext a0 = ld_ext(f + 2 * i), a1 = ld_ext(f + 2 * i + 1);
+  const Ext b0 = ld_ext(w + 2 * i), b1 = ld_ext(w + 2 * i + 1);
+  st_ext(fo + i, eadd(a0, emul(r, esub(a1, a0))));
+  st_ext(wo + i, eadd(b0, emul(r, esub(b1, b0))));
+}
+
+}  // namespace kb

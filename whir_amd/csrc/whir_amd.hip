@@ -1,0 +1,662 @@
+// MI355X (gfx950) WHIR prover: one translation unit holding every kernel (they
+// share the __constant__ Poseidon2 tables) plus the host orchestration and the
+// C ABI declared in include/whir_gpu.h.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "field.h"
+#include "poseidon2.h"
+#include "tx.h"
+#include "ntt.h"
+#include "merkle.h"
+#include "sumcheck.h"
+#include "protocol.h"
+#include "host_params.h"
+#include "host_rng.h"
+#include "../../include/whir_gpu.h"
+
+using namespace kb;
+using whir_host::Plan;
+using whir_host::RoundPlan;
+
+#define HIP_TRY(x)                                                                     \
+  do {                                                                                 \
+    hipError_t _e = (x);                                                               \
+    if (_e != hipSuccess) {                                                            \
+      fprintf(stderr, "whir_amd: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(_e), \
+              __FILE__, __LINE__);                                                     \
+      return WHIR_EDEVICE;                                                             \
+    }                                                                                  \
+  } while (0)
+
+// ============================ context ============================
+
+struct whir_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t* tw = nullptr;  // tw_lo[4096] || tw_hi[4096]
+  const uint32_t* tw_lo() const { return tw; }
+  const uint32_t* tw_hi() const { return tw + 4096; }
+};
+
+static int upload_constants(uint64_t seed) {
+  Poseidon2Consts c;
+  {
+    whir_host::ChaChaRng r(seed);
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 16; ++j) c.ext16_init[i][j] = r.next_koalabear();
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 16; ++j) c.ext16_term[i][j] = r.next_koalabear();
+    for (int i = 0; i < 20; ++i) c.int16[i] = r.next_koalabear();
+  }
+  {
+    whir_host::ChaChaRng r(seed);
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 24; ++j) c.ext24_init[i][j] = r.next_koalabear();
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 24; ++j) c.ext24_term[i][j] = r.next_koalabear();
+    for (int i = 0; i < 23; ++i) c.int24[i] = r.next_koalabear();
+  }
+  HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_p2), &c, sizeof(c)));
+  // V16 (canonical sign * num / 2^sh), Montgomery form for the cooperative P16.
+  static const int sgn[16] = {-1, 1, 1, 1, 1, 1, -1, -1, -1, 1, 1, 1, -1, -1, -1, -1};
+  static const uint32_t num[16] = {2, 1, 2, 1, 3, 4, 1, 3, 4, 1, 1, 1, 1, 1, 1, 1};
+  static const int sh[16] = {0, 0, 0, 1, 0, 0, 1, 0, 0, 8, 3, 24, 8, 3, 4, 24};
+  uint32_t d16[16];
+  const uint32_t half = kb::HALF;
+  for (int i = 0; i < 16; ++i) {
+    uint32_t v = kb::mul(kb::from_canonical(num[i]), kb::pow(half, (uint64_t)sh[i]));
+    d16[i] = sgn[i] < 0 ? kb::neg(v) : v;
+  }
+  HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_diag16), d16, sizeof(d16)));
+  return WHIR_OK;
+}
+
+extern "C" int whir_ctx_create(int device, uint64_t seed, whir_ctx** out) {
+  if (!out) return WHIR_EINVAL;
+  HIP_TRY(hipSetDevice(device));
+  whir_ctx* c = new whir_ctx();
+  c->device = device;
+  int rc = upload_constants(seed);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  std::vector<uint32_t> tw(8192);
+  const uint32_t g = kb::two_adic_generator(24);
+  uint32_t x = kb::ONE;
+  for (int i = 0; i < 4096; ++i) {
+    tw[i] = x;
+    x = kb::mul(x, g);
+  }
+  const uint32_t g4096 = kb::pow(g, 4096);
+  x = kb::ONE;
+  for (int i = 0; i < 4096; ++i) {
+    tw[4096 + i] = x;
+    x = kb::mul(x, g4096);
+  }
+  HIP_TRY(hipMalloc(&c->tw, 8192 * 4));
+  HIP_TRY(hipMemcpy(c->tw, tw.data(), 8192 * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipStreamCreate(&c->stream));
+  *out = c;
+  return WHIR_OK;
+}
+
+extern "C" int whir_ctx_destroy(whir_ctx* c) {
+  if (!c) return WHIR_OK;
+  hipFree(c->tw);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return WHIR_OK;
+}
+
+static hipStream_t pick_stream(whir_ctx* c, void* s) { return s ? (hipStream_t)s : c->stream; }
+
+// ============================ DFT ============================
+
+struct NttPlan {
+  int P = 0;
+  int logl[4] = {0, 0, 0, 0};
+  int logq[4] = {0, 0, 0, 0};
+};
+
+static bool ntt_plan(int log_n, int log_c, NttPlan& pl) {
+  constexpr int LOG_TILE = 13;  // 8192 words (32 KiB) per workgroup tile
+  if (log_n < 1 || log_n > 24 || log_c > 10) return false;
+  const int logq_pref = std::max(5, log_c);
+  const int max_logl = std::max(1, LOG_TILE - logq_pref);
+  int P = (log_n + max_logl - 1) / max_logl;
+  if (P > 4) return false;
+  pl.P = P;
+  int rem = log_n;
+  for (int p = 0; p < P; ++p) {
+    const int take = (rem + (P - p) - 1) / (P - p);
+    pl.logl[p] = take;
+    rem -= take;
+  }
+  int used = 0;
+  for (int p = 0; p < P; ++p) {
+    used += pl.logl[p];
+    const int log_m = log_n - used;
+    if (p + 1 < P) {
+      pl.logq[p] = std::min(logq_pref, log_m + log_c);
+    } else {
+      const int log_l1 = P > 1 ? pl.logl[0] : 0;
+      const int log_t = std::min(std::max(0, logq_pref - log_c), log_l1);
+      pl.logq[p] = log_c + log_t;
+    }
+    if (pl.logl[p] + pl.logq[p] > LOG_TILE + 1) return false;
+  }
+  return true;
+}
+
+static int launch_ntt_pass(int logl, dim3 grid, size_t lds, hipStream_t s, const NttPass& a) {
+  switch (logl) {
+#define NTT_CASE(L) \
+  case L: hipLaunchKernelGGL(ntt_pass_kernel<L>, grid, dim3(NTT_THREADS), lds, s, a); break;
+    NTT_CASE(1) NTT_CASE(2) NTT_CASE(3) NTT_CASE(4) NTT_CASE(5) NTT_CASE(6) NTT_CASE(7) NTT_CASE(8)
+    NTT_CASE(9) NTT_CASE(10) NTT_CASE(11) NTT_CASE(12)
+#undef NTT_CASE
+    default: return WHIR_EINVAL;
+  }
+  return WHIR_OK;
+}
+
+// out = DFT of the n_in-row matrix `in` zero-padded to 2^log_n rows; `tmp` has
+// room for 2^(log_n + log_c) words (unused for single-pass sizes).
+static int ntt_enqueue(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_n, int log_c, uint32_t* out,
+                       uint32_t* tmp, hipStream_t s) {
+  NttPlan pl;
+  if (!ntt_plan(log_n, log_c, pl)) return WHIR_EINVAL;
+  int used = 0;
+  for (int p = 0; p < pl.P; ++p) {
+    NttPass a;
+    memset(&a, 0, sizeof(a));
+    a.tw_lo = c->tw_lo();
+    a.tw_hi = c->tw_hi();
+    a.n_in_rows = n_in;
+    a.log_n = log_n;
+    a.log_c = log_c;
+    a.log_l = pl.logl[p];
+    used += pl.logl[p];
+    a.log_m = log_n - used;
+    a.log_q = pl.logq[p];
+    a.first = p == 0;
+    a.last = p + 1 == pl.P;
+    a.n_pass = pl.P;
+    for (int q = 0; q < 4; ++q) a.logl[q] = pl.logl[q];
+    a.src = p == 0 ? in : tmp;
+    a.dst = a.last ? out : tmp;
+    uint64_t grid;
+    if (!a.last) {
+      grid = (1ull << (log_n - a.log_l - a.log_m)) * ((1ull << (a.log_m + log_c)) >> a.log_q);
+    } else {
+      const int log_t = a.log_q - log_c;
+      grid = 1ull << (log_n - log_t - a.log_l);
+    }
+    const size_t lds = ((size_t)1 << (a.log_l + a.log_q)) * 4 + ((size_t)1 << a.log_l) * 2;
+    int rc = launch_ntt_pass(a.log_l, dim3((unsigned)grid), lds, s, a);
+    if (rc) return rc;
+  }
+  return WHIR_OK;
+}
+
+extern "C" int whir_dft_batch(whir_ctx* c, const uint32_t* d_in, uint64_t n_in_rows, uint32_t log_h, uint32_t width,
+                              uint32_t* d_out, void* stream) {
+  if (!c || !d_in || !d_out || width == 0 || (width & (width - 1)) || n_in_rows > (1ull << log_h))
+    return WHIR_EINVAL;
+  int log_c = 0;
+  while ((1u << log_c) < width) ++log_c;
+  hipStream_t s = pick_stream(c, stream);
+  uint32_t* tmp = nullptr;
+  NttPlan pl;
+  if (!ntt_plan((int)log_h, log_c, pl)) return WHIR_EINVAL;
+  if (pl.P > 1) HIP_TRY(hipMallocAsync((void**)&tmp, ((size_t)width << log_h) * 4, s));
+  int rc = ntt_enqueue(c, d_in, n_in_rows, (int)log_h, log_c, d_out, tmp, s);
+  if (tmp) HIP_TRY(hipFreeAsync(tmp, s));
+  HIP_TRY(hipGetLastError());
+  return rc;
+}
+
+// ============================ Merkle ============================
+
+static int leaves_enqueue(const uint32_t* rows, uint64_t n_rows, uint32_t row_words, uint32_t* leaves,
+                          hipStream_t s) {
+  const dim3 grid((unsigned)((n_rows + MERKLE_THREADS - 1) / MERKLE_THREADS)), block(MERKLE_THREADS);
+  switch (row_words) {
+#define LEAF_CASE(W) \
+  case W: hipLaunchKernelGGL(merkle_leaves_kernel<W>, grid, block, 0, s, rows, n_rows, leaves); break;
+    LEAF_CASE(1) LEAF_CASE(2) LEAF_CASE(4) LEAF_CASE(8) LEAF_CASE(16) LEAF_CASE(32) LEAF_CASE(64) LEAF_CASE(128)
+#undef LEAF_CASE
+    default: return WHIR_EINVAL;
+  }
+  return WHIR_OK;
+}
+
+// Leaves + every layer into `layers` (2*2^log_h - 1 digests). Root at the end.
+static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, uint32_t* layers, hipStream_t s) {
+  const uint64_t H = 1ull << log_h;
+  int rc = leaves_enqueue(rows, H, row_words, layers, s);
+  if (rc) return rc;
+  uint32_t* cur = layers;
+  uint64_t n = H;
+  while (n > 2 * MERKLE_THREADS) {
+    const uint64_t half = n / 2;
+    hipLaunchKernelGGL(merkle_layer_kernel, dim3((unsigned)((half + MERKLE_THREADS - 1) / MERKLE_THREADS)),
+                       dim3(MERKLE_THREADS), 0, s, cur, half, cur + n * 8);
+    cur += n * 8;
+    n = half;
+  }
+  if (n > 1) hipLaunchKernelGGL(merkle_top_kernel, dim3(1), dim3(MERKLE_THREADS), 0, s, cur, n);
+  return WHIR_OK;
+}
+
+extern "C" int whir_merkle_commit(whir_ctx* c, const uint32_t* d_rows, uint32_t log_h, uint32_t row_words,
+                                  uint32_t* d_layers, uint32_t* root_host, void* stream) {
+  if (!c || !d_rows || !d_layers || log_h > 28) return WHIR_EINVAL;
+  hipStream_t s = pick_stream(c, stream);
+  int rc = merkle_enqueue(d_rows, (int)log_h, row_words, d_layers, s);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError());
+  if (root_host) {
+    const uint64_t H = 1ull << log_h;
+    HIP_TRY(hipMemcpyAsync(root_host, d_layers + (2 * H - 2) * 8, 32, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  return WHIR_OK;
+}
+
+template <int W>
+__global__ void permute_kernel(uint32_t* st, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t s[W];
+#pragma unroll
+  for (int j = 0; j < W; ++j) s[j] = st[i * W + j];
+  permute<W>(s);
+#pragma unroll
+  for (int j = 0; j < W; ++j) st[i * W + j] = s[j];
+}
+
+// Cooperative (one wavefront per state) P16, as used by the device transcript.
+__global__ void permute16_coop_kernel(uint32_t* st, uint64_t n) {
+  const uint64_t i = blockIdx.x;
+  if (i >= n) return;
+  uint32_t s = st[i * 16 + (threadIdx.x & 15)];
+  permute16_coop(s);
+  if (threadIdx.x < 16) st[i * 16 + threadIdx.x] = s;
+}
+
+extern "C" int whir_poseidon2_permute(whir_ctx* c, int width, uint32_t* d_states, uint64_t n, void* stream) {
+  if (!c || !d_states) return WHIR_EINVAL;
+  hipStream_t s = pick_stream(c, stream);
+  const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+  if (width == 16) hipLaunchKernelGGL(permute_kernel<16>, grid, block, 0, s, d_states, n);
+  else if (width == 24) hipLaunchKernelGGL(permute_kernel<24>, grid, block, 0, s, d_states, n);
+  else if (width == -16) hipLaunchKernelGGL(permute16_coop_kernel, dim3((unsigned)n), dim3(64), 0, s, d_states, n);
+  else return WHIR_EINVAL;
+  HIP_TRY(hipGetLastError());
+  return WHIR_OK;
+}
+
+// ============================ prover ============================
+
+struct whir_prover {
+  whir_ctx* ctx = nullptr;
+  Plan plan;
+  void* arena = nullptr;
+  size_t arena_bytes = 0;
+  // carved buffers
+  DevTx* tx;
+  JobScalars* js;
+  uint32_t* log;
+  uint32_t* input;
+  uint32_t* cw0;
+  uint32_t* tree0;
+  uint32_t* cw[2];
+  uint32_t* tree[2];
+  uint32_t* ntt_tmp;
+  Ext* ep[2];
+  Ext* ew[2];
+  ClaimSlot* slots;
+  Ext* svo_pool;
+  Ext* ood_pool;
+  Ext* partials;
+  uint32_t* stir_idx;
+  uint32_t* stir_count;
+  QueryArrays qa;
+  uint32_t* powL;
+  Ext* powR;
+  uint64_t svo_stride, ood_stride;
+  int m0_svo, m1_svo;
+  int n_svo_claims;
+  uint32_t* host_log = nullptr;  // pinned
+  JobScalars* host_js = nullptr;
+  DevTx* host_tx = nullptr;
+};
+
+static inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+extern "C" int whir_prover_create(whir_ctx* c, const whir_params* prm, whir_prover** out) {
+  if (!c || !prm || !out) return WHIR_EINVAL;
+  Plan pl = whir_host::make_plan(prm->k, prm->folding, prm->rate, prm->initial_reduction, prm->soundness,
+                                 prm->security_level, prm->n_points);
+  if (!pl.ok) return WHIR_EINVAL;
+  const int k = pl.k, f = pl.f;
+  if (k > 28 || f > MAX_F || pl.k + pl.rate - f > 24) return WHIR_EINVAL;
+  const int n_svo = pl.n_points + pl.n_ood0;
+  if (n_svo < 1 || n_svo > MAX_SVO_CLAIMS) return WHIR_EINVAL;
+  int max_ood = 0, max_stir = pl.final_n_stir;
+  uint64_t max_cw_words = 0;
+  for (const RoundPlan& r : pl.rounds) {
+    max_ood = std::max(max_ood, r.n_ood);
+    max_stir = std::max(max_stir, r.n_stir);
+    if (r.log_rows > 24) return WHIR_EINVAL;
+    max_cw_words = std::max<uint64_t>(max_cw_words, (4ull << f) << r.log_rows);
+  }
+  if (max_stir > MAX_STIR || max_ood > 16) return WHIR_EINVAL;
+
+  whir_prover* P = new whir_prover();
+  P->ctx = c;
+  P->plan = pl;
+  P->n_svo_claims = n_svo;
+  const int m = k - f;
+  P->m0_svo = m / 2;
+  P->m1_svo = m - m / 2;
+  P->svo_stride = (1ull << P->m0_svo) + (1ull << P->m1_svo);
+  P->ood_stride = P->svo_stride;  // round polynomials have at most k - f variables
+  const uint64_t H0 = 1ull << (k + pl.rate - f);
+  const uint64_t cw0_words = H0 << f;
+  uint64_t max_rows = 1;
+  for (const RoundPlan& r : pl.rounds) max_rows = std::max<uint64_t>(max_rows, 1ull << r.log_rows);
+  const uint64_t n_ext = 1ull << m;
+  int m_pow = std::max(0, k - 2 * f);
+  const uint64_t pow_l = 1ull << (m_pow / 2), pow_r = 1ull << (m_pow - m_pow / 2);
+  const uint64_t part_ext = std::max<uint64_t>({(1ull << P->m1_svo) << f, 1ull << P->m1_svo, 4096});
+
+  struct Item { void** ptr; size_t bytes; };
+  std::vector<Item> items = {
+      {(void**)&P->tx, sizeof(DevTx)},
+      {(void**)&P->js, sizeof(JobScalars)},
+      {(void**)&P->log, pl.log_words * 4},
+      {(void**)&P->input, (1ull << k) * 4},
+      {(void**)&P->cw0, cw0_words * 4},
+      {(void**)&P->tree0, (2 * H0 - 1) * 32},
+      {(void**)&P->cw[0], max_cw_words * 4},
+      {(void**)&P->cw[1], max_cw_words * 4},
+      {(void**)&P->tree[0], (2 * max_rows - 1) * 32},
+      {(void**)&P->tree[1], (2 * max_rows - 1) * 32},
+      {(void**)&P->ntt_tmp, std::max(cw0_words, max_cw_words) * 4},
+      {(void**)&P->ep[0], n_ext * 16},
+      {(void**)&P->ep[1], n_ext * 16},
+      {(void**)&P->ew[0], n_ext * 16},
+      {(void**)&P->ew[1], n_ext * 16},
+      {(void**)&P->slots, sizeof(ClaimSlot) * (n_svo + 16)},
+      {(void**)&P->svo_pool, P->svo_stride * n_svo * 16},
+      {(void**)&P->ood_pool, P->ood_stride * std::max(1, max_ood) * 16},
+      {(void**)&P->partials, part_ext * 16},
+      {(void**)&P->stir_idx, MAX_STIR * 4},
+      {(void**)&P->stir_count, 4},
+      {(void**)&P->qa.index, MAX_STIR * 4},
+      {(void**)&P->qa.ws, MAX_STIR * 4},
+      {(void**)&P->qa.ts, MAX_STIR * 4},
+      {(void**)&P->qa.var, MAX_STIR * 4},
+      {(void**)&P->qa.scale, MAX_STIR * 16},
+      {(void**)&P->qa.G, (size_t)MAX_STIR * (1u << MAX_F) * 16},
+      {(void**)&P->powL, MAX_STIR * pow_l * 4},
+      {(void**)&P->powR, MAX_STIR * pow_r * 16},
+  };
+  size_t total = 0;
+  for (auto& it : items) total += align_up(it.bytes);
+  if (hipMalloc(&P->arena, total) != hipSuccess) {
+    delete P;
+    return WHIR_EOOM;
+  }
+  P->arena_bytes = total;
+  char* p = (char*)P->arena;
+  for (auto& it : items) {
+    *it.ptr = p;
+    p += align_up(it.bytes);
+  }
+  if (hipHostMalloc((void**)&P->host_log, pl.log_words * 4, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&P->host_js, sizeof(JobScalars), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&P->host_tx, sizeof(DevTx), hipHostMallocDefault) != hipSuccess) {
+    whir_prover_destroy(P);
+    return WHIR_EOOM;
+  }
+  *out = P;
+  return WHIR_OK;
+}
+
+extern "C" int whir_prover_destroy(whir_prover* P) {
+  if (!P) return WHIR_OK;
+  if (P->arena) hipFree(P->arena);
+  if (P->host_log) hipHostFree(P->host_log);
+  if (P->host_js) hipHostFree(P->host_js);
+  if (P->host_tx) hipHostFree(P->host_tx);
+  delete P;
+  return WHIR_OK;
+}
+
+extern "C" size_t whir_prover_proof_capacity(const whir_prover* P) { return P ? P->plan.log_words * 4 : 0; }
+extern "C" uint32_t* whir_prover_input(whir_prover* P) { return P ? P->input : nullptr; }
+
+static unsigned blocks_for(uint64_t n, unsigned per = SC_THREADS) { return (unsigned)((n + per - 1) / per); }
+
+template <int FL>
+static void launch_compress_hi(unsigned grid, hipStream_t s, const uint32_t* poly, const Ext* eq0, const Ext* eq1,
+                               uint32_t m0, Ext* partials) {
+  hipLaunchKernelGGL(compress_hi_kernel<FL>, dim3(grid), dim3(SC_THREADS), 0, s, poly, eq0, eq1, m0, partials);
+}
+template <int FL>
+static void launch_expr(unsigned grid, hipStream_t s, const uint32_t* poly, const Ext* eq_rs, const Ext* pool,
+                        uint32_t n_claims, uint64_t stride, uint32_t m0, uint64_t n, Ext* ep, Ext* ew) {
+  hipLaunchKernelGGL(expr_build_kernel<FL>, dim3(grid), dim3(SC_THREADS), 0, s, poly, eq_rs, pool, n_claims, stride,
+                     m0, n, ep, ew);
+}
+
+#define FSWITCH(f, CALL)          \
+  switch (f) {                    \
+    case 1: { constexpr int FL = 1; CALL; } break; \
+    case 2: { constexpr int FL = 2; CALL; } break; \
+    case 3: { constexpr int FL = 3; CALL; } break; \
+    case 4: { constexpr int FL = 4; CALL; } break; \
+    case 5: { constexpr int FL = 5; CALL; } break; \
+    default: return WHIR_EINVAL;  \
+  }
+
+// The whole proof as one kernel sequence on stream s (no host synchronisation).
+static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
+  whir_ctx* c = P->ctx;
+  const Plan& pl = P->plan;
+  const int k = pl.k, f = pl.f;
+  const uint32_t W = 1u << f;
+  int rc;
+  hipLaunchKernelGGL(tx_init_kernel, dim3(1), dim3(64), 0, s, P->tx, P->js, (uint32_t)pl.log_words);
+
+  // ---- commit: RS encode + Merkle (whir.rs:19-36) ----
+  const int log_h0 = k + pl.rate - f;
+  if ((rc = ntt_enqueue(c, poly, 1ull << (k - f), log_h0, f, P->cw0, P->ntt_tmp, s))) return rc;
+  if ((rc = merkle_enqueue(P->cw0, log_h0, W, P->tree0, s))) return rc;
+  const uint64_t H0 = 1ull << log_h0;
+  hipLaunchKernelGGL(write_root_kernel, dim3(1), dim3(64), 0, s, P->tx, P->log, P->js, P->tree0 + (2 * H0 - 2) * 8);
+
+  // ---- evaluation claims + OOD virtual claims (prover.rs:64-90, 231-277) ----
+  const uint32_t m0 = P->m0_svo, m1 = P->m1_svo;
+  for (int ci = 0; ci < P->n_svo_claims; ++ci) {
+    Ext* eq0 = P->svo_pool + ci * P->svo_stride;
+    Ext* eq1 = eq0 + (1ull << m0);
+    hipLaunchKernelGGL(claim_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots + ci,
+                       eq0, eq1, (uint32_t)k, (uint32_t)f, m0, m1);
+    FSWITCH(f, launch_compress_hi<FL>(1u << m1, s, poly, eq0, eq1, m0, P->partials));
+    hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots + ci,
+                       P->partials, 1u << m1, (uint32_t)f);
+  }
+  // ---- SVO rounds + expression (prover.rs:279-353, 184-229) ----
+  hipLaunchKernelGGL(svo_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots,
+                     (uint32_t)P->n_svo_claims, (uint32_t)f, P->svo_pool, P->svo_stride, m0, m1);
+  uint64_t n = 1ull << (k - f);
+  FSWITCH(f, launch_expr<FL>(blocks_for(n), s, poly, P->js->eq_rs, P->svo_pool, (uint32_t)P->n_svo_claims,
+                             P->svo_stride, m0, n, P->ep[0], P->ew[0]));
+  int cur = 0;
+  uint32_t rs_idx = (uint32_t)f;
+
+  // ---- WHIR rounds (whir.rs:205-312) ----
+  for (int r = 0; r < pl.n_rounds; ++r) {
+    const RoundPlan& rp = pl.rounds[r];
+    uint32_t* cw = P->cw[r & 1];
+    uint32_t* tree = P->tree[r & 1];
+    // commit_ext (whir.rs:38-61): 2^(kc-f) rows of 2^f extension elements
+    if ((rc = ntt_enqueue(c, (const uint32_t*)P->ep[cur], 1ull << (rp.kc - f), rp.log_rows, f + 2, cw, P->ntt_tmp, s)))
+      return rc;
+    if ((rc = merkle_enqueue(cw, rp.log_rows, 4 * W, tree, s))) return rc;
+    const uint64_t Hr = 1ull << rp.log_rows;
+    hipLaunchKernelGGL(write_root_kernel, dim3(1), dim3(64), 0, s, P->tx, P->log, P->js, tree + (2 * Hr - 2) * 8);
+    // OOD claims (whir.rs:224-235)
+    const uint32_t om0 = rp.kc / 2, om1 = rp.kc - rp.kc / 2;
+    for (int o = 0; o < rp.n_ood; ++o) {
+      Ext* eq0 = P->ood_pool + o * P->ood_stride;
+      Ext* eq1 = eq0 + (1ull << om0);
+      ClaimSlot* slot = P->slots + P->n_svo_claims + o;
+      hipLaunchKernelGGL(claim_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, slot, eq0,
+                         eq1, (uint32_t)rp.kc, 0u, om0, om1);
+      hipLaunchKernelGGL(eval_ext_kernel, dim3(1u << om1), dim3(SC_THREADS), 0, s, P->ep[cur], eq0, eq1, om0,
+                         P->partials);
+      hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, slot,
+                         P->partials, 1u << om1, 0u);
+    }
+    // STIR queries (whir.rs:237-293)
+    hipLaunchKernelGGL(stir_sample_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js,
+                       (uint32_t)rp.n_stir, (uint32_t)rp.kfd, P->stir_idx, P->stir_count);
+    if (r == 0) {
+      hipLaunchKernelGGL(stir_query_kernel, dim3(rp.n_stir), dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log,
+                         P->stir_idx, P->stir_count, P->cw0, W, P->tree0, (uint32_t)rp.kfd, 1, (uint32_t)f, P->qa);
+    } else {
+      hipLaunchKernelGGL(stir_query_kernel, dim3(rp.n_stir), dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log,
+                         P->stir_idx, P->stir_count, P->cw[(r - 1) & 1], 4 * W, P->tree[(r - 1) & 1],
+                         (uint32_t)rp.kfd, 2, (uint32_t)f, P->qa);
+    }
+    // fold / fold_with_code (prover.rs:377-561)
+    hipLaunchKernelGGL(fold_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->ood_pool,
+                       P->ood_stride, (uint32_t)rp.n_ood, om0, om1, P->qa, P->stir_count);
+    if (r > 0)
+      hipLaunchKernelGGL(pow_tables_kernel, dim3(blocks_for((uint64_t)rp.n_stir * ((1ull << om0) + (1ull << om1)))),
+                         dim3(SC_THREADS), 0, s, P->qa.var, P->qa.scale, P->stir_count, om0, om1, P->powL, P->powR);
+    hipLaunchKernelGGL(combine_kernel, dim3(blocks_for(n)), dim3(SC_THREADS), 0, s, P->ew[cur], n, om0, P->ood_pool,
+                       (uint32_t)rp.n_ood, P->ood_stride, P->powL, P->powR, r > 0 ? P->stir_count : nullptr, om1);
+    for (int t = 0; t < f; ++t) {
+      unsigned g;
+      if (t == 0) {
+        g = std::min<unsigned>(1024, blocks_for(n / 2));
+        hipLaunchKernelGGL(coeffs_kernel, dim3(g), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n, P->partials);
+      } else {
+        g = std::min<unsigned>(1024, blocks_for(n / 4));
+        hipLaunchKernelGGL(fold_coeffs_kernel, dim3(g), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
+                           P->js->rs + (rs_idx - 1), P->ep[cur ^ 1], P->ew[cur ^ 1], P->partials);
+        n /= 2;
+        cur ^= 1;
+      }
+      hipLaunchKernelGGL(round_finalize_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js,
+                         P->partials, g, rs_idx, r == 0 ? 1 : 0, P->qa, P->stir_count, (uint32_t)f,
+                         (uint32_t)rp.kfd, (uint32_t)t, c->tw_lo(), c->tw_hi());
+      ++rs_idx;
+    }
+    hipLaunchKernelGGL(fold_kernel, dim3(blocks_for(n / 2)), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
+                       P->js->rs + (rs_idx - 1), P->ep[cur ^ 1], P->ew[cur ^ 1]);
+    n /= 2;
+    cur ^= 1;
+    if (r == 0) {
+      // combine_pows_scaled(weights, ws, scales) at k - 2f variables (prover.rs:556)
+      const int mm = rp.kc - f;
+      const uint32_t pm0 = mm / 2, pm1 = mm - mm / 2;
+      hipLaunchKernelGGL(pow_tables_kernel, dim3(blocks_for((uint64_t)rp.n_stir * ((1ull << pm0) + (1ull << pm1)))),
+                         dim3(SC_THREADS), 0, s, P->qa.ws, P->qa.scale, P->stir_count, pm0, pm1, P->powL, P->powR);
+      hipLaunchKernelGGL(combine_kernel, dim3(blocks_for(n)), dim3(SC_THREADS), 0, s, P->ew[cur], n, pm0,
+                         (const Ext*)nullptr, 0u, (uint64_t)0, P->powL, P->powR, P->stir_count, pm1);
+    }
+  }
+  // ---- final polynomial and queries (whir.rs:314-355) ----
+  hipLaunchKernelGGL(write_final_kernel, dim3(1), dim3(64), 0, s, P->tx, P->log, P->js, P->ep[cur], (uint32_t)n);
+  hipLaunchKernelGGL(stir_sample_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js,
+                     (uint32_t)pl.final_n_stir, (uint32_t)pl.final_kfd, P->stir_idx, P->stir_count);
+  if (pl.n_rounds > 0) {
+    const int lr = pl.n_rounds - 1;
+    hipLaunchKernelGGL(stir_query_kernel, dim3(pl.final_n_stir), dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log,
+                       P->stir_idx, P->stir_count, P->cw[lr & 1], 4 * W, P->tree[lr & 1], (uint32_t)pl.final_kfd, 0,
+                       (uint32_t)f, P->qa);
+  } else {
+    hipLaunchKernelGGL(stir_query_kernel, dim3(pl.final_n_stir), dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log,
+                       P->stir_idx, P->stir_count, P->cw0, W, P->tree0, (uint32_t)pl.final_kfd, 0, (uint32_t)f,
+                       P->qa);
+  }
+  hipLaunchKernelGGL(checkpoint_kernel, dim3(1), dim3(64), 0, s, P->tx, P->log, P->js);
+  HIP_TRY(hipGetLastError());
+  return WHIR_OK;
+}
+
+extern "C" int whir_prover_enqueue(whir_prover* P, const uint32_t* d_poly, void* stream) {
+  if (!P || !d_poly) return WHIR_EINVAL;
+  hipStream_t s = pick_stream(P->ctx, stream);
+  int rc = prover_enqueue(P, d_poly, s);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(P->host_tx, P->tx, sizeof(DevTx), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(P->host_js, P->js, sizeof(JobScalars), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(P->host_log, P->log, P->plan.log_words * 4, hipMemcpyDeviceToHost, s));
+  return WHIR_OK;
+}
+
+extern "C" int whir_prover_result(whir_prover* P, uint8_t* proof, size_t cap, size_t* len, uint32_t* checkpoint,
+                                  uint32_t* root8) {
+  if (!P) return WHIR_EINVAL;
+  HIP_TRY(hipDeviceSynchronize());
+  const DevTx& t = *P->host_tx;
+  if (t.overflow || t.log_words > t.log_cap) return WHIR_ETRANSCRIPT;
+  const size_t bytes = (size_t)t.log_words * 4;
+  if (len) *len = bytes;
+  if (checkpoint) *checkpoint = P->host_js->checkpoint;
+  if (root8) memcpy(root8, P->host_log, 32);  // the first transcript write is the root
+  if (proof) {
+    if (bytes > cap) return WHIR_ETRANSCRIPT;
+    memcpy(proof, P->host_log, bytes);
+  }
+  return WHIR_OK;
+}
+
+extern "C" int whir_prove_host(whir_ctx* c, const whir_params* prm, const uint32_t* poly, uint8_t* proof, size_t cap,
+                               size_t* len, uint32_t* checkpoint, uint32_t* root8) {
+  whir_prover* P = nullptr;
+  int rc = whir_prover_create(c, prm, &P);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(P->input, poly, (1ull << P->plan.k) * 4, hipMemcpyHostToDevice));
+  rc = whir_prover_enqueue(P, P->input, nullptr);
+  if (!rc) rc = whir_prover_result(P, proof, cap, len, checkpoint, root8);
+  whir_prover_destroy(P);
+  return rc;
+}
+
+// ============================ memory helpers ============================
+
+extern "C" int whir_device_alloc(size_t bytes, void** out) {
+  if (!out) return WHIR_EINVAL;
+  return hipMalloc(out, bytes) == hipSuccess ? WHIR_OK : WHIR_EOOM;
+}
+extern "C" int whir_device_free(void* p) {
+  HIP_TRY(hipFree(p));
+  return WHIR_OK;
+}
+extern "C" int whir_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return WHIR_OK;
+}
+extern "C" int whir_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return WHIR_OK;
+}
+extern "C" int whir_device_synchronize(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return WHIR_OK;
+}
