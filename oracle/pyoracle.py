@@ -43,6 +43,7 @@ def lib():
         for name in ("oracle_fmul", "oracle_fadd", "oracle_fsub"):
             getattr(L, name).argtypes = [ctypes.c_uint32, ctypes.c_uint32]
         L.oracle_finv.argtypes = [ctypes.c_uint32]
+        L.oracle_emul.argtypes = [u32p, u32p, u32p]
         L.oracle_to_monty.argtypes = [ctypes.c_uint32]
         L.oracle_from_monty.argtypes = [ctypes.c_uint32]
         L.oracle_two_adic_generator.argtypes = [ctypes.c_uint]

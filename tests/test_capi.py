@@ -1,0 +1,50 @@
+"""CPU checks of the drop-in boundary: the HIP library loads, exports every
+entry point include/whir_gpu.h declares, and the Python mirror binds each one
+(no device calls are made here)."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    with open(os.path.join(ROOT, "include", "whir_gpu.h")) as fh:
+        text = fh.read()
+    names = re.findall(r"^\s*(?:int|size_t|uint32_t\s*\*)\s+(whir_[a-z0-9_]+)\s*\(", text, re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    for must in ("whir_ctx_create", "whir_dft_batch", "whir_merkle_commit", "whir_prover_enqueue",
+                 "whir_prover_result", "whir_prove_host"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    import whir_amd
+
+    lib = ctypes.CDLL(whir_amd.library_path)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_python_binding_covers_header():
+    from whir_amd._lib import SIGNATURES
+
+    bound = {n for n, _, _ in SIGNATURES}
+    assert set(declared_symbols()) <= bound
+
+
+def test_missing_library_fails_loudly(tmp_path, monkeypatch):
+    import whir_amd._lib as L
+
+    monkeypatch.setattr(L, "library_path", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(L, "_lib", None)
+    try:
+        L.lib()
+    except RuntimeError as e:
+        assert "no CPU fallback" in str(e)
+    else:
+        raise AssertionError("expected a RuntimeError")

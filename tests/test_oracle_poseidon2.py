@@ -1,0 +1,218 @@
+"""Pins the oracle's Poseidon2 to the structure Plonky3 uses for KoalaBear
+(crates/whir/src/test.rs:185-199):
+
+* MDS-light equals the explicit block matrix [2*M4 on the diagonal, M4 off it],
+  M4 = [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]] (Plonky3 MDSMat4);
+* the internal layer equals 1*1^T + diag(V) with V the documented fractions;
+* the full permutation equals an independent Python composition of those
+  layers (x^3 S-box, 4 + R_P + 4 rounds, round constants from the oracle's
+  constant stream), and a Python inverse recovers the input;
+* sponge24 / compress16 follow PaddingFreeSponge / TruncatedPermutation.
+The constant values (ChaCha12 stream, seed 1000) and V are recalled from
+Plonky3 and remain parity unpinned against the crate itself."""
+import random
+
+import numpy as np
+import pytest
+
+from kbpy import P, R, RINV
+
+M4 = [[2, 3, 1, 1], [1, 2, 3, 1], [1, 1, 2, 3], [3, 1, 1, 2]]
+
+V16 = [(-1, 2, 0), (1, 1, 0), (1, 2, 0), (1, 1, 1), (1, 3, 0), (1, 4, 0), (-1, 1, 1), (-1, 3, 0), (-1, 4, 0),
+       (1, 1, 8), (1, 1, 3), (1, 1, 24), (-1, 1, 8), (-1, 1, 3), (-1, 1, 4), (-1, 1, 24)]
+V24 = [(-1, 2, 0), (1, 1, 0), (1, 2, 0), (1, 1, 1), (1, 3, 0), (1, 4, 0), (-1, 1, 1), (-1, 3, 0), (-1, 4, 0),
+       (1, 1, 8), (1, 1, 2), (1, 1, 3), (1, 1, 4), (1, 1, 5), (1, 1, 6), (1, 1, 24), (-1, 1, 8), (-1, 1, 3),
+       (-1, 1, 4), (-1, 1, 5), (-1, 1, 6), (-1, 1, 7), (-1, 1, 9), (-1, 1, 24)]
+
+
+def vdiag(width):
+    v = V16 if width == 16 else V24
+    return [(s * n * pow(pow(2, sh, P), -1, P)) % P for s, n, sh in v]
+
+
+def mds_matrix(width):
+    M = [[0] * width for _ in range(width)]
+    for bi in range(width // 4):
+        for bj in range(width // 4):
+            f = 2 if bi == bj else 1
+            for i in range(4):
+                for j in range(4):
+                    M[4 * bi + i][4 * bj + j] = f * M4[i][j]
+    return M
+
+
+def internal_matrix(width):
+    d = vdiag(width)
+    return [[(1 + (d[i] if i == j else 0)) % P for j in range(width)] for i in range(width)]
+
+
+def matvec(M, x):
+    return [sum(a * b for a, b in zip(row, x)) % P for row in M]
+
+
+def matinv(M):
+    n = len(M)
+    A = [list(r) + [1 if i == j else 0 for j in range(n)] for i, r in enumerate(M)]
+    for c in range(n):
+        piv = next(r for r in range(c, n) if A[r][c] % P)
+        A[c], A[piv] = A[piv], A[c]
+        inv = pow(A[c][c], -1, P)
+        A[c] = [v * inv % P for v in A[c]]
+        for r in range(n):
+            if r != c and A[r][c]:
+                f = A[r][c]
+                A[r] = [(a - f * b) % P for a, b in zip(A[r], A[c])]
+    return [row[n:] for row in A]
+
+
+def consts_canonical(oracle, width):
+    ei, et, ci, d = oracle.poseidon2_constants(width, 1000)
+    conv = lambda a: [[int(x) * RINV % P for x in row] for row in a]
+    return conv(ei), conv(et), [int(x) * RINV % P for x in ci], [int(x) * RINV % P for x in d]
+
+
+def permute_py(oracle, width, state):
+    ei, et, ci, _ = consts_canonical(oracle, width)
+    ME, MI = mds_matrix(width), internal_matrix(width)
+    s = matvec(ME, state)
+    for r in range(4):
+        s = matvec(ME, [pow((x + c) % P, 3, P) for x, c in zip(s, ei[r])])
+    for r in range(len(ci)):
+        s[0] = pow((s[0] + ci[r]) % P, 3, P)
+        s = matvec(MI, s)
+    for r in range(4):
+        s = matvec(ME, [pow((x + c) % P, 3, P) for x, c in zip(s, et[r])])
+    return s
+
+
+def inverse_py(oracle, width, state):
+    ei, et, ci, _ = consts_canonical(oracle, width)
+    MEi, MIi = matinv(mds_matrix(width)), matinv(internal_matrix(width))
+    cbrt = pow(3, -1, P - 1)
+    s = list(state)
+    for r in reversed(range(4)):
+        s = [(pow(x, cbrt, P) - c) % P for x, c in zip(matvec(MEi, s), et[r])]
+    for r in reversed(range(len(ci))):
+        s = matvec(MIi, s)
+        s[0] = (pow(s[0], cbrt, P) - ci[r]) % P
+    for r in reversed(range(4)):
+        s = [(pow(x, cbrt, P) - c) % P for x, c in zip(matvec(MEi, s), ei[r])]
+    return matvec(MEi, s)
+
+
+def to_m_list(xs):
+    return np.array([x * R % P for x in xs], np.uint32)
+
+
+def from_m_list(arr):
+    return [int(x) * RINV % P for x in arr]
+
+
+@pytest.mark.parametrize("width", [16, 24])
+def test_linear_layers_are_the_documented_matrices(oracle, width):
+    rnd = random.Random(width)
+    ME, MI = mds_matrix(width), internal_matrix(width)
+    assert [int(x) * RINV % P for x in oracle.poseidon2_constants(width)[3]] == vdiag(width)
+    for _ in range(20):
+        x = [rnd.randrange(P) for _ in range(width)]
+        # both layers are F_p-linear with integer-free coefficients in V, so
+        # they commute with the Montgomery scaling
+        assert from_m_list(oracle.mds_light(width, to_m_list(x))) == matvec(ME, x)
+        assert from_m_list(oracle.internal_layer(width, to_m_list(x))) == matvec(MI, x)
+
+
+@pytest.mark.parametrize("width", [16, 24])
+def test_layers_invertible_and_sbox_bijective(width):
+    assert len(matinv(mds_matrix(width))) == width
+    assert len(matinv(internal_matrix(width))) == width
+    from math import gcd
+
+    assert gcd(3, P - 1) == 1
+
+
+@pytest.mark.parametrize("width", [16, 24])
+def test_permutation_structure_and_inverse(oracle, width):
+    rnd = random.Random(100 + width)
+    for _ in range(3):
+        x = [rnd.randrange(P) for _ in range(width)]
+        got = from_m_list(oracle.permute(width, to_m_list(x)))
+        assert got == permute_py(oracle, width, x)
+        assert inverse_py(oracle, width, got) == x
+
+
+def test_round_counts_and_constant_order(oracle):
+    # 4 + 4 external rounds of W constants, then R_P internal constants, all from
+    # one fresh StdRng(1000) stream per width
+    for width, rp in ((16, 20), (24, 23)):
+        ei, et, ci, _ = oracle.poseidon2_constants(width, 1000)
+        assert ci.size == rp
+        words = []
+        import test_oracle_rng as tr
+
+        stream = tr.stdrng_words(1000, 2 * (8 * width + rp) + 64)
+        for w in stream:
+            v = w >> 1
+            if v < P:
+                words.append(v)
+        flat = list(ei.reshape(-1)) + list(et.reshape(-1)) + list(ci)
+        assert [int(x) for x in flat] == words[: len(flat)]
+
+
+def test_sponge_and_compress(oracle):
+    rnd = np.random.default_rng(5)
+    for n in (0, 1, 8, 15, 16, 17, 32, 64, 100):
+        vals = rnd.integers(0, P, n, dtype=np.uint64).astype(np.uint32)
+        st = np.zeros(24, np.uint32)
+        pos = 0
+        while True:
+            take = min(16, n - pos)
+            if take == 0:
+                break
+            st[:take] = vals[pos:pos + take]
+            pos += take
+            st = oracle.permute(24, st)
+            if take < 16:
+                break
+        assert np.array_equal(oracle.sponge24(vals), st[:8])
+    l = rnd.integers(0, P, 8, dtype=np.uint64).astype(np.uint32)
+    r = rnd.integers(0, P, 8, dtype=np.uint64).astype(np.uint32)
+    assert np.array_equal(oracle.compress16(l, r), oracle.permute(16, np.concatenate([l, r]))[:8])
+
+
+class DuplexPy:
+    """DuplexChallenger<F, P16, 16, 8> restated over the oracle's permutation."""
+
+    def __init__(self, oracle):
+        self.o = oracle
+        self.state = np.zeros(16, np.uint32)
+        self.inp, self.out = [], []
+
+    def duplex(self):
+        for i, v in enumerate(self.inp):
+            self.state[i] = v
+        self.inp = []
+        self.state = self.o.permute(16, self.state)
+        self.out = list(self.state[:8])
+
+    def observe(self, v):
+        self.out = []
+        self.inp.append(v)
+        if len(self.inp) == 8:
+            self.duplex()
+
+    def sample(self):
+        if self.inp or not self.out:
+            self.duplex()
+        return self.out.pop()
+
+
+@pytest.mark.parametrize("n_obs,n_samples", [(0, 1), (0, 20), (1, 3), (7, 9), (8, 1), (8, 9), (9, 4), (16, 17), (23, 40)])
+def test_challenger_semantics(oracle, n_obs, n_samples):
+    rnd = np.random.default_rng(n_obs * 100 + n_samples)
+    obs = rnd.integers(0, P, n_obs, dtype=np.uint64).astype(np.uint32)
+    d = DuplexPy(oracle)
+    for v in obs:
+        d.observe(v)
+    want = [d.sample() for _ in range(n_samples)]
+    assert [int(x) for x in oracle.challenger_probe(obs, n_samples)] == [int(x) for x in want]
