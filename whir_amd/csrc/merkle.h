@@ -8,6 +8,7 @@
 #pragma once
 #include "field.h"
 #include "poseidon2.h"
+#include "tx.h"
 
 namespace kb {
 
@@ -64,28 +65,32 @@ __global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_kernel(const uint
   po[1] = make_uint4(o[4], o[5], o[6], o[7]);
 }
 
-// All layers from a layer of n <= 2*MERKLE_THREADS nodes up to the root, in one
-// workgroup; writes each layer at its place in the concatenated layout.
-__global__ void __launch_bounds__(MERKLE_THREADS) merkle_top_kernel(uint32_t* layer, uint64_t n) {
-  while (n > 1) {
-    const uint64_t half = n / 2;
-    uint32_t* next = layer + n * 8;
-    for (uint64_t j = threadIdx.x; j < half; j += MERKLE_THREADS) {
-      uint32_t l[8], r[8], o[8];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        l[i] = layer[j * 8 + i];
-        r[i] = layer[(j + half) * 8 + i];
-      }
-      compress(l, r, o);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) next[j * 8 + i] = o[i];
+// Upper layers with cooperative compressions (16 lanes per P16, tx.h), which
+// cut the per-layer latency of the narrow top of the tree. Under split-half
+// pairing the nodes of a layer that are congruent to w (mod G) are exactly the
+// subtree below node w of the G-node layer, so workgroup w builds every layer
+// from n0/2 nodes down to G nodes for its residue class without a grid-wide
+// barrier; launch with G = 1 and one workgroup for the final layers.
+constexpr int COOP_THREADS = 1024;  // 64 compressions in flight per workgroup
+
+__global__ void __launch_bounds__(COOP_THREADS) merkle_coop_subtrees_kernel(uint32_t* layer, uint64_t n0, uint32_t G) {
+  const uint32_t w = blockIdx.x;
+  const uint32_t slot = threadIdx.x >> 4, li = threadIdx.x & 15;
+  uint32_t* cur = layer;
+  uint64_t n = n0;
+  while (n > G) {
+    const uint64_t half = n >> 1;
+    uint32_t* next = cur + n * 8;
+    const uint64_t cnt = half / G;
+    for (uint64_t t = slot; t < cnt; t += COOP_THREADS / 16) {
+      const uint64_t j = w + t * G;
+      uint32_t v = li < 8 ? cur[j * 8 + li] : cur[(j + half) * 8 + (li - 8)];
+      permute16_coop(v);
+      if (li < 8) next[j * 8 + li] = v;
     }
-    __threadfence_block();
     __syncthreads();
-    layer = next;
+    cur = next;
     n = half;
   }
 }
-
 }  // namespace kb

@@ -93,9 +93,10 @@ __global__ void tx_init_kernel(DevTx* t, JobScalars* js, uint32_t cap) {
 __global__ void __launch_bounds__(64) write_root_kernel(DevTx* t, uint32_t* log, JobScalars* js, const uint32_t* root) {
   WaveTx tx;
   tx_begin(tx, t, log, js);
-  uint32_t d[8];
-  for (int j = 0; j < 8; ++j) d[j] = root[j];
-  tx.write_digest(d);
+  // write(digest): 8 hint words, then observe the 8 elements
+  const uint32_t v = root[threadIdx.x & 7];
+  for (int j = 0; j < 8; ++j) tx.hint_word((uint32_t)__shfl((int)v, j, 8));
+  for (int j = 0; j < 8; ++j) tx.observe((uint32_t)__shfl((int)v, j, 8));
   tx_end(tx, t, js);
 }
 

@@ -7,8 +7,8 @@
 //   sample()      : duplex if input pending or output empty; pop from the end;
 //   duplex        : state[0..len) = input; P16(state); output = state[0..8);
 //   sample_bits(b): canonical(sample()) & (2^b - 1).
-// One wavefront runs the transcript: every lane holds the (uniform) buffers and
-// lane i owns state word i & 15, so P16 runs 16-wide with cross-lane shuffles.
+// One wavefront runs the transcript: lane i owns state word i & 15, so P16 runs
+// 16-wide with DPP cross-lane moves.
 #pragma once
 #include "field.h"
 #include "poseidon2.h"
@@ -27,71 +27,79 @@ struct DevTx {
 
 __constant__ uint32_t c_diag16[16];  // Montgomery form of V16 (cooperative P16)
 
-__device__ __forceinline__ uint32_t shfl16(uint32_t v, int src) { return __shfl((int)v, src, 16); }
-__device__ __forceinline__ uint32_t shfl16_xor(uint32_t v, int m) { return __shfl_xor((int)v, m, 16); }
+// Cross-lane moves inside each 16-lane row via DPP (VALU latency, no LDS):
+//   quad_perm(p0,p1,p2,p3): lane i of each quad reads lane p_i of that quad;
+//   row_ror:n             : lane i of each 16-lane row reads lane (i - n) mod 16.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+constexpr int DPP_QUAD_NEXT = 1 | (2 << 2) | (3 << 4) | (0 << 6);  // lane a <- a+1 (mod 4)
+constexpr int DPP_QUAD_XOR1 = 1 | (0 << 2) | (3 << 4) | (2 << 6);
+constexpr int DPP_QUAD_XOR2 = 2 | (3 << 2) | (0 << 4) | (1 << 6);
+constexpr int DPP_ROR(int n) { return 0x120 + n; }
 
 // Cooperative Poseidon2-16: lane i (mod 16) holds state word i.
-__device__ __forceinline__ void mds_light16_coop(uint32_t& x, int i) {
-  const int a = i & 3;
-  const uint32_t xn = shfl16(x, (i & 12) | ((a + 1) & 3));
-  uint32_t t = add(x, shfl16_xor(x, 1));
-  const uint32_t s4 = add(t, shfl16_xor(t, 2));
-  const uint32_t y = add(add(s4, x), dbl(xn));  // s4 + x_a + 2 x_{a+1}
-  uint32_t u = add(y, shfl16_xor(y, 4));
-  const uint32_t cs = add(u, shfl16_xor(u, 8));
+// MDS-light per lane: y_a = S4 + x_a + 2 x_{a+1} within the quad (= M4 row a),
+// then add the sum of y over the lanes with the same index mod 4.
+__device__ __forceinline__ void mds_light16_coop(uint32_t& x) {
+  const uint32_t xn = dpp<DPP_QUAD_NEXT>(x);
+  const uint32_t t = add(x, dpp<DPP_QUAD_XOR1>(x));
+  const uint32_t s4 = add(t, dpp<DPP_QUAD_XOR2>(t));
+  const uint32_t y = add(add(s4, x), dbl(xn));
+  const uint32_t u = add(y, dpp<DPP_ROR(4)>(y));
+  const uint32_t cs = add(u, dpp<DPP_ROR(8)>(u));
   x = add(y, cs);
+}
+
+__device__ __forceinline__ uint32_t row_sum16(uint32_t v) {
+  v = add(v, dpp<DPP_ROR(8)>(v));
+  v = add(v, dpp<DPP_ROR(4)>(v));
+  v = add(v, dpp<DPP_ROR(2)>(v));
+  v = add(v, dpp<DPP_ROR(1)>(v));
+  return v;
 }
 
 __device__ __forceinline__ void permute16_coop(uint32_t& s) {
   const int i = threadIdx.x & 15;
-  mds_light16_coop(s, i);
+  mds_light16_coop(s);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     s = cube(add(s, c_p2.ext16_init[r][i]));
-    mds_light16_coop(s, i);
+    mds_light16_coop(s);
   }
   const uint32_t d = c_diag16[i];
 #pragma unroll
   for (int r = 0; r < 20; ++r) {
     const uint32_t t = cube(add(s, c_p2.int16[r]));
     s = i == 0 ? t : s;
-    uint32_t sum = add(s, shfl16_xor(s, 1));
-    sum = add(sum, shfl16_xor(sum, 2));
-    sum = add(sum, shfl16_xor(sum, 4));
-    sum = add(sum, shfl16_xor(sum, 8));
+    const uint32_t sum = row_sum16(s);
     s = add(mul(s, d), sum);
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     s = cube(add(s, c_p2.ext16_term[r][i]));
-    mds_light16_coop(s, i);
+    mds_light16_coop(s);
   }
 }
 
-// Select element `idx` (uniform, < 8) of a register array without dynamic indexing.
-__device__ __forceinline__ uint32_t sel8(const uint32_t (&a)[8], uint32_t idx) {
-  uint32_t v = a[0];
-#pragma unroll
-  for (int j = 1; j < 8; ++j) v = idx == (uint32_t)j ? a[j] : v;
-  return v;
-}
-
-// Must be used by all 64 lanes of exactly one wavefront, in uniform control flow.
+// One wavefront runs the transcript in uniform control flow. The sponge state
+// and both rate buffers are lane-distributed so nothing is indexed dynamically:
+//   s   : lane i holds state[i & 15];
+//   pin : lane i (< 8) holds the pending input word i (written into the state at
+//         the next duplex);
+//   the output buffer is state[0..8) right after a duplex (the state does not
+//   change until the next duplex), so pop() reads lane out_len - 1 of s.
 struct WaveTx {
-  uint32_t s;
-  uint32_t in_buf[8], out_buf[8];
+  uint32_t s, pin;
   uint32_t in_len, out_len;
   uint32_t log_words, log_cap, overflow;
   uint32_t* log;
 
-  __device__ void load(const DevTx* t, uint32_t* log_ptr) {
+  __device__ __forceinline__ void load(const DevTx* t, uint32_t* log_ptr) {
     const int lane = threadIdx.x & 63;
     s = t->state[lane & 15];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      in_buf[j] = t->in_buf[j];
-      out_buf[j] = t->out_buf[j];
-    }
+    pin = t->in_buf[lane & 7];
     in_len = t->in_len;
     out_len = t->out_len;
     log_words = t->log_words;
@@ -99,15 +107,14 @@ struct WaveTx {
     overflow = t->overflow;
     log = log_ptr;
   }
-  __device__ void store(DevTx* t) {
+  __device__ __forceinline__ void store(DevTx* t) {
     const int lane = threadIdx.x & 63;
     if (lane < 16) t->state[lane] = s;
+    if (lane < 8) {
+      t->in_buf[lane] = pin;
+      t->out_buf[lane] = s;
+    }
     if (lane == 0) {
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        t->in_buf[j] = in_buf[j];
-        t->out_buf[j] = out_buf[j];
-      }
       t->in_len = in_len;
       t->out_len = out_len;
       t->log_words = log_words;
@@ -115,29 +122,25 @@ struct WaveTx {
       t->overflow = overflow;
     }
   }
-  __device__ void duplex() {
-    const int i = threadIdx.x & 15;
-    const uint32_t inv = sel8(in_buf, (uint32_t)(i & 7));
-    if ((uint32_t)i < in_len) s = inv;
+  __device__ __forceinline__ void duplex() {
+    const uint32_t i = threadIdx.x & 15;
+    if (i < in_len) s = pin;
     in_len = 0;
     permute16_coop(s);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) out_buf[j] = shfl16(s, j);
     out_len = 8;
   }
-  __device__ void observe(uint32_t v) {
+  __device__ __forceinline__ void observe(uint32_t v) {
     out_len = 0;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) in_buf[j] = in_len == (uint32_t)j ? v : in_buf[j];
+    if ((threadIdx.x & 15) == in_len) pin = v;
     in_len += 1;
     if (in_len == 8) duplex();
   }
-  __device__ uint32_t sample() {
+  __device__ __forceinline__ uint32_t sample() {
     if (in_len != 0 || out_len == 0) duplex();
     out_len -= 1;
-    return sel8(out_buf, out_len);
+    return (uint32_t)__builtin_amdgcn_readlane((int)s, (int)out_len);
   }
-  __device__ Ext sample_ext() {
+  __device__ __forceinline__ Ext sample_ext() {
     Ext e;
     e.c[0] = sample();
     e.c[1] = sample();
@@ -145,27 +148,29 @@ struct WaveTx {
     e.c[3] = sample();
     return e;
   }
-  __device__ uint32_t sample_bits(uint32_t bits) { return canonical(sample()) & ((1u << bits) - 1u); }
-  __device__ void hint_word(uint32_t v) {
+  __device__ __forceinline__ uint32_t sample_bits(uint32_t bits) {
+    return canonical(sample()) & ((1u << bits) - 1u);
+  }
+  __device__ __forceinline__ void hint_word(uint32_t v) {
     if ((threadIdx.x & 63) == 0) {
       if (log_words < log_cap) log[log_words] = v;
     }
     if (log_words >= log_cap) overflow = 1;
     log_words += 1;
   }
-  __device__ void write_f(uint32_t v) {
+  __device__ __forceinline__ void write_f(uint32_t v) {
     hint_word(v);
     observe(v);
   }
-  __device__ void write_e(const Ext& e) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) hint_word(e.c[c]);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) observe(e.c[c]);
-  }
-  __device__ void write_digest(const uint32_t* d) {
-    for (int j = 0; j < 8; ++j) hint_word(d[j]);
-    for (int j = 0; j < 8; ++j) observe(d[j]);
+  __device__ __forceinline__ void write_e(const Ext& e) {
+    hint_word(e.c[0]);
+    hint_word(e.c[1]);
+    hint_word(e.c[2]);
+    hint_word(e.c[3]);
+    observe(e.c[0]);
+    observe(e.c[1]);
+    observe(e.c[2]);
+    observe(e.c[3]);
   }
 };
 

@@ -238,20 +238,36 @@ static int leaves_enqueue(const uint32_t* rows, uint64_t n_rows, uint32_t row_wo
 }
 
 // Leaves + every layer into `layers` (2*2^log_h - 1 digests). Root at the end.
+// Wide layers use one compression per lane (throughput-bound); from
+// MERKLE_COOP_FROM nodes down, cooperative subtrees (latency-bound part).
+constexpr uint64_t MERKLE_COOP_FROM = 1ull << 15;
+constexpr uint32_t MERKLE_COOP_GROUPS = 256;
+
 static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, uint32_t* layers, hipStream_t s) {
   const uint64_t H = 1ull << log_h;
   int rc = leaves_enqueue(rows, H, row_words, layers, s);
   if (rc) return rc;
   uint32_t* cur = layers;
   uint64_t n = H;
-  while (n > 2 * MERKLE_THREADS) {
+  while (n > MERKLE_COOP_FROM) {
     const uint64_t half = n / 2;
     hipLaunchKernelGGL(merkle_layer_kernel, dim3((unsigned)((half + MERKLE_THREADS - 1) / MERKLE_THREADS)),
                        dim3(MERKLE_THREADS), 0, s, cur, half, cur + n * 8);
     cur += n * 8;
     n = half;
   }
-  if (n > 1) hipLaunchKernelGGL(merkle_top_kernel, dim3(1), dim3(MERKLE_THREADS), 0, s, cur, n);
+  if (n > 1) {
+    uint32_t G = (uint32_t)std::min<uint64_t>(MERKLE_COOP_GROUPS, n / 64);
+    if (G >= 2) {
+      hipLaunchKernelGGL(merkle_coop_subtrees_kernel, dim3(G), dim3(COOP_THREADS), 0, s, cur, n, G);
+      // advance to the G-node layer
+      while (n > G) {
+        cur += n * 8;
+        n /= 2;
+      }
+    }
+    hipLaunchKernelGGL(merkle_coop_subtrees_kernel, dim3(1), dim3(COOP_THREADS), 0, s, cur, n, 1u);
+  }
   return WHIR_OK;
 }
 
