@@ -1,16 +1,26 @@
 """Benchmark of the MI355X WHIR prover hot path.
 
-Workload (the north-star configuration, one GPU): WHIR commit + prove of one
-multilinear polynomial with 2^24 KoalaBear coefficients (seeded, uniform in
-[0, p)), rate 1/2 (log_inv_rate 1), folding factor 4, initial reduction 3,
-JohnsonBound soundness at 123 bits, one evaluation claim at a point drawn from
-the transcript (the reference's run_whir_poseidon flow,
+Workload (one GPU): WHIR commit + prove of one multilinear polynomial with
+2^24 KoalaBear coefficients (seeded, uniform in [0, p)), rate 1/2
+(log_inv_rate 1), folding factor 4, initial reduction 3, JohnsonBound
+soundness at 123 bits, one evaluation claim at a point drawn from the
+transcript (the reference's run_whir_poseidon flow,
 crates/whir/src/test.rs:174-254). A step is one full proof: RS encode +
 Poseidon2 Merkle commit, claim evaluation, SVO sumcheck, 4 WHIR rounds, final
-polynomial and STIR openings, proof bytes copied to the host. The input
-polynomial is resident in HBM before the timed region.
+polynomial and STIR openings, proof bytes copied back to the host. The input
+polynomial is resident in HBM before the timed region starts.
 
-Prints one JSON line (see README of the measurement contract in DESIGN.md).
+The JSON line also carries
+  * roofline: the dominant kernel, the base-tree Poseidon2-24 leaf hash
+    (one permutation per lane, VALU-bound). Its launches are timed live with HIP
+    events on the prover's stream inside the timed region; achieved = VALU lane
+    instructions issued per second (instructions per lane from the kernel's own
+    gfx950 code object, whir_amd/kernel_stats.json) against the MI355X VALU
+    issue peak (256 CUs x 4 SIMD x 16 lanes x 2.4 GHz). `traffic` comes from
+    the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present.
+  * cpu_baseline: the CPU oracle (a multithreaded C++ restatement of the
+    reference prover, "port") proving the same polynomial on the host cores;
+    its proof bytes are compared with the GPU's (bit_exact).
 """
 import argparse
 import json
@@ -23,24 +33,55 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-K, FOLDING, RATE, IR, SOUNDNESS, SECURITY, N_POINTS = 24, 4, 1, 3, 1, 123, 1
+FOLDING, RATE, IR, SOUNDNESS, SECURITY, N_POINTS = 4, 1, 3, 1, 123, 1
 METRIC = "commit+prove ms @2^24 vars; batched proofs/s & p99 vs plain GPU, bit-exact"
+VALU_PEAK_T = 256 * 4 * 16 * 2.4e9 / 1e12  # T lane-instructions / s
+HBM_PEAK_GBS = 8000.0
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "merkle_leaves16_k24_pmc.txt")
 
 
-def rand_poly_device_seed(seed, k):
-    """Seeded synthetic coefficients, uniform in [0, p) as Montgomery words
-    (Poly::<F>::rand(&mut rng(seed), k) is reproduced by the oracle; the bench
-    uses a fast numpy stream of the same distribution)."""
+def synthetic_poly(seed, k):
+    """Seeded coefficients, uniform in [0, p), as Montgomery words."""
     rng = np.random.default_rng(seed)
     return rng.integers(0, 0x7F000001, size=1 << k, dtype=np.uint64).astype(np.uint32)
+
+
+def read_traffic():
+    try:
+        with open(PMC_SUMMARY) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(poly, k, gpu_proof, threads):
+    from oracle import pyoracle as O
+
+    O.set_threads(threads)
+    t0 = time.perf_counter()
+    proof, cp, root = O.whir_prove(k, FOLDING, RATE, IR, SOUNDNESS, SECURITY, N_POINTS, poly)
+    dt = time.perf_counter() - t0
+    rc, _ = O.whir_verify(k, FOLDING, RATE, IR, SOUNDNESS, SECURITY, N_POINTS, gpu_proof)
+    return {
+        "value": round(dt * 1e3, 1),
+        "unit": "ms",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"one full commit+prove at k={k} by the CPU oracle (C++/OpenMP restatement, not Plonky3)",
+        "bit_exact": proof == gpu_proof,
+        "oracle_verifier_accepts_gpu_proof": rc == 0,
+    }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--k", type=int, default=K)
+    ap.add_argument("--k", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
@@ -55,13 +96,14 @@ def main():
     ctx = whir_amd.Context(0, 1000)
     params = whir_params(args.k, FOLDING, RATE, IR, SOUNDNESS, SECURITY, N_POINTS)
     prover = whir_amd.Prover(params, ctx)
-    poly = rand_poly_device_seed(1, args.k)
+    poly = synthetic_poly(1, args.k)
     prover.upload(poly)  # resident in HBM before timing
 
     for _ in range(args.warmup):
         prover.enqueue()
         proof, cp, root = prover.result()
 
+    prover.set_timers(True)
     whir_amd.lib().whir_device_synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -70,6 +112,39 @@ def main():
     whir_amd.lib().whir_device_synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
+
+    leaf_ms, leaf_n, perms = prover.timer(prover.TIMER_LEAVES_BASE)
+    dft_ms, dft_n, dft_words = prover.timer(prover.TIMER_DFT_BASE)
+    with open(os.path.join(ROOT, "whir_amd", "kernel_stats.json")) as fh:
+        kstats = json.load(fh)
+    instr = kstats.get("leaves16_valu_per_lane")
+    leaf_s = leaf_ms / leaf_n / 1e3
+    achieved = instr * perms / leaf_s / 1e12 if instr else None
+    roofline = {
+        "kernel": "merkle_leaves_kernel<16> (Poseidon2-24 leaf hash of the base codeword)",
+        "bound": "valu",
+        "achieved": round(achieved, 2) if achieved else None,
+        "peak": round(VALU_PEAK_T, 2),
+        "unit": "T VALU lane-instr/s",
+        "frac": round(achieved / VALU_PEAK_T, 3) if achieved else None,
+        "traffic": read_traffic(),
+        "algorithmic_bytes": perms * (64 + 32),
+        "avg_launch_us": round(leaf_s * 1e6, 1),
+        "perms_per_launch": perms,
+        "perms_per_s": round(perms / leaf_s, 0),
+        "valu_instr_per_perm": instr,
+    }
+    dft_s = dft_ms / dft_n / 1e3
+    roofline_dft = {
+        "kernel": "base RS encode (ntt_pass_kernel passes, coset LDE of 16 columns)",
+        "bound": "hbm",
+        "achieved": round(dft_words * 4 / dft_s / 1e9, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(dft_words * 4 / dft_s / 1e9 / HBM_PEAK_GBS, 3),
+        "traffic": None,
+        "avg_launch_us": round(dft_s * 1e6, 1),
+    }
 
     out = {
         "metric": METRIC,
@@ -88,8 +163,14 @@ def main():
             "workload": f"whir commit+prove, 2^{args.k} KoalaBear coeffs, rate 1/2, folding 4, ir 3, JB-123, 1 ext claim",
             "k": args.k,
             "proof_bytes": len(proof),
+            "proofs_per_s": round(1e3 / ms, 2),
         },
+        "roofline": roofline,
+        "roofline_dft": roofline_dft,
     }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(poly, args.k, proof, threads)
     print(json.dumps(out), flush=True)
 
 

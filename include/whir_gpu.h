@@ -97,6 +97,17 @@ int whir_prover_enqueue(whir_prover* prover, const uint32_t* d_poly, void* strea
 int whir_prover_result(whir_prover* prover, uint8_t* proof, size_t cap, size_t* len, uint32_t* checkpoint,
                        uint32_t* root8);
 
+/* Optional device timers: HIP events recorded on the prover's stream around one
+ * kernel class of each proof (the reference's tracing spans "dft-base" and
+ * "leaf_hash", crates/whir/src/whir.rs:33, crates/merkle/src/poseidon_packed.rs:285).
+ * whir_prover_timer reports the accumulated milliseconds, the number of timed
+ * proofs and the work per launch (Poseidon2-24 permutations for the leaf hash,
+ * input + output words for the DFT). Enabling resets the accumulators. */
+enum { WHIR_TIMER_LEAVES_BASE = 0, WHIR_TIMER_DFT_BASE = 1, WHIR_TIMER_COUNT = 2 };
+int whir_prover_set_timers(whir_prover* prover, int enable);
+int whir_prover_timer(const whir_prover* prover, int which, double* total_ms, uint64_t* launches,
+                      uint64_t* work_units);
+
 /* Host-to-host convenience: upload, prove, download. */
 int whir_prove_host(whir_ctx* ctx, const whir_params* params, const uint32_t* poly, uint8_t* proof, size_t cap,
                     size_t* len, uint32_t* checkpoint, uint32_t* root8);

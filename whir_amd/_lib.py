@@ -52,6 +52,9 @@ SIGNATURES = [
     ("whir_prover_enqueue", ctypes.c_int, [_vp, _vp, _vp]),
     ("whir_prover_result", ctypes.c_int,
      [_vp, _u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
+    ("whir_prover_set_timers", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("whir_prover_timer", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     ("whir_prove_host", ctypes.c_int,
      [_vp, ctypes.POINTER(whir_params), _u32p, _u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
     ("whir_device_alloc", ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),

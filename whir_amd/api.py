@@ -147,6 +147,17 @@ class Prover:
                                        cp.ctypes.data_as(_u32p), root.ctypes.data_as(_u32p)), "whir_prover_result")
         return bytes(buf[: ln.value]), int(cp[0]), root
 
+    TIMER_LEAVES_BASE, TIMER_DFT_BASE = 0, 1
+
+    def set_timers(self, enable=True):
+        check(lib().whir_prover_set_timers(self.ptr, int(enable)), "whir_prover_set_timers")
+
+    def timer(self, which):
+        ms, n, work = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().whir_prover_timer(self.ptr, which, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(work)),
+              "whir_prover_timer")
+        return ms.value, n.value, work.value
+
     def close(self):
         if self.ptr:
             lib().whir_prover_destroy(self.ptr)

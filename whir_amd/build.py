@@ -3,14 +3,24 @@
 All kernels live in one translation unit (csrc/whir_amd.hip) because they share
 the __constant__ Poseidon2 tables. Packed-FP32 instruction selection is
 disabled for every HIP compile.
+
+The compile keeps its intermediate files in a temporary directory so that the
+gfx950 assembly of the hashing kernels can be read back: their VALU
+instruction counts (straight-line code, so per lane per launch) go to
+whir_amd/kernel_stats.json, which bench.py uses for the VALU roofline.
 """
+import json
 import os
+import re
+import shutil
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libwhir_amd.so")
+STATS = os.path.join(HERE, "kernel_stats.json")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("WHIR_ARCH", "gfx950")
 
@@ -24,15 +34,46 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 
+# kernels whose per-lane VALU instruction count bench.py needs
+COUNTED = {
+    "leaves16": "_ZN2kb20merkle_leaves_kernelILi16EEEvPKjmPj",
+    "leaves64": "_ZN2kb20merkle_leaves_kernelILi64EEEvPKjmPj",
+    "layer": "_ZN2kb19merkle_layer_kernelEPKjmPj",
+}
+
+
+def count_valu(asm, symbol):
+    i = asm.find(symbol + ":")
+    if i < 0:
+        return None
+    j = asm.find(".Lfunc_end", i)
+    n = 0
+    for line in asm[i:j].splitlines():
+        t = line.strip()
+        if t.startswith("v_"):
+            n += 1
+    return n
+
 
 def build(verbose=False, extra_flags=()):
     src = os.path.join(CSRC, "whir_amd.hip")
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *HIP_FLAGS, *extra_flags, "-I", CSRC, src, "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.check_call(cmd)
-    os.replace(tmp, LIB)
+    with tempfile.TemporaryDirectory(prefix="whir_build_") as tmp:
+        out = os.path.join(tmp, "libwhir_amd.so")
+        cmd = [HIPCC, *HIP_FLAGS, *extra_flags, "-save-temps", "-I", CSRC, src, "-o", out]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd, cwd=tmp)
+        asm_files = [f for f in os.listdir(tmp) if f.endswith(".s") and ARCH in f]
+        stats = {"arch": ARCH}
+        if asm_files:
+            with open(os.path.join(tmp, asm_files[0])) as fh:
+                asm = fh.read()
+            for key, sym in COUNTED.items():
+                stats[key + "_valu_per_lane"] = count_valu(asm, sym)
+        shutil.copyfile(out, LIB + ".tmp")
+        os.replace(LIB + ".tmp", LIB)
+    with open(STATS, "w") as fh:
+        json.dump(stats, fh, indent=1)
     return LIB
 
 

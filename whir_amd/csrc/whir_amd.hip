@@ -243,10 +243,13 @@ static int leaves_enqueue(const uint32_t* rows, uint64_t n_rows, uint32_t row_wo
 constexpr uint64_t MERKLE_COOP_FROM = 1ull << 15;
 constexpr uint32_t MERKLE_COOP_GROUPS = 256;
 
-static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, uint32_t* layers, hipStream_t s) {
+static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, uint32_t* layers, hipStream_t s,
+                          hipEvent_t* leaf_ev = nullptr) {
   const uint64_t H = 1ull << log_h;
+  if (leaf_ev) HIP_TRY(hipEventRecord(leaf_ev[0], s));
   int rc = leaves_enqueue(rows, H, row_words, layers, s);
   if (rc) return rc;
+  if (leaf_ev) HIP_TRY(hipEventRecord(leaf_ev[1], s));
   uint32_t* cur = layers;
   uint64_t n = H;
   while (n > MERKLE_COOP_FROM) {
@@ -353,6 +356,12 @@ struct whir_prover {
   uint32_t* host_log = nullptr;  // pinned
   JobScalars* host_js = nullptr;
   DevTx* host_tx = nullptr;
+  // optional HIP-event timers around kernel classes (whir_prover_set_timers)
+  bool timers = false;
+  hipEvent_t ev[WHIR_TIMER_COUNT][2] = {};
+  double timer_ms[WHIR_TIMER_COUNT] = {};
+  uint64_t timer_launches[WHIR_TIMER_COUNT] = {};
+  bool timer_pending = false;
 };
 
 static inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -454,7 +463,38 @@ extern "C" int whir_prover_destroy(whir_prover* P) {
   if (P->host_log) hipHostFree(P->host_log);
   if (P->host_js) hipHostFree(P->host_js);
   if (P->host_tx) hipHostFree(P->host_tx);
+  for (int i = 0; i < WHIR_TIMER_COUNT; ++i)
+    for (int j = 0; j < 2; ++j)
+      if (P->ev[i][j]) (void)hipEventDestroy(P->ev[i][j]);
   delete P;
+  return WHIR_OK;
+}
+
+extern "C" int whir_prover_set_timers(whir_prover* P, int enable) {
+  if (!P) return WHIR_EINVAL;
+  if (enable && !P->ev[0][0])
+    for (int i = 0; i < WHIR_TIMER_COUNT; ++i)
+      for (int j = 0; j < 2; ++j) HIP_TRY(hipEventCreate(&P->ev[i][j]));
+  P->timers = enable != 0;
+  for (int i = 0; i < WHIR_TIMER_COUNT; ++i) {
+    P->timer_ms[i] = 0.0;
+    P->timer_launches[i] = 0;
+  }
+  return WHIR_OK;
+}
+
+extern "C" int whir_prover_timer(const whir_prover* P, int which, double* total_ms, uint64_t* launches,
+                                 uint64_t* work_units) {
+  if (!P || which < 0 || which >= WHIR_TIMER_COUNT) return WHIR_EINVAL;
+  if (total_ms) *total_ms = P->timer_ms[which];
+  if (launches) *launches = P->timer_launches[which];
+  if (work_units) {
+    const Plan& pl = P->plan;
+    const uint64_t rows = 1ull << (pl.k + pl.rate - pl.f);
+    // leaf hash: Poseidon2-24 permutations per launch; DFT: input + output words
+    *work_units = which == WHIR_TIMER_LEAVES_BASE ? rows * (((1ull << pl.f) + 15) / 16)
+                                                  : (1ull << pl.k) + (rows << pl.f);
+  }
   return WHIR_OK;
 }
 
@@ -496,8 +536,12 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
 
   // ---- commit: RS encode + Merkle (whir.rs:19-36) ----
   const int log_h0 = k + pl.rate - f;
+  if (P->timers) HIP_TRY(hipEventRecord(P->ev[WHIR_TIMER_DFT_BASE][0], s));
   if ((rc = ntt_enqueue(c, poly, 1ull << (k - f), log_h0, f, P->cw0, P->ntt_tmp, s))) return rc;
-  if ((rc = merkle_enqueue(P->cw0, log_h0, W, P->tree0, s))) return rc;
+  if (P->timers) HIP_TRY(hipEventRecord(P->ev[WHIR_TIMER_DFT_BASE][1], s));
+  if ((rc = merkle_enqueue(P->cw0, log_h0, W, P->tree0, s, P->timers ? P->ev[WHIR_TIMER_LEAVES_BASE] : nullptr)))
+    return rc;
+  P->timer_pending = P->timers;
   const uint64_t H0 = 1ull << log_h0;
   hipLaunchKernelGGL(write_root_kernel, dim3(1), dim3(64), 0, s, P->tx, P->log, P->js, P->tree0 + (2 * H0 - 2) * 8);
 
@@ -629,6 +673,15 @@ extern "C" int whir_prover_result(whir_prover* P, uint8_t* proof, size_t cap, si
                                   uint32_t* root8) {
   if (!P) return WHIR_EINVAL;
   HIP_TRY(hipDeviceSynchronize());
+  if (P->timer_pending) {
+    for (int i = 0; i < WHIR_TIMER_COUNT; ++i) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, P->ev[i][0], P->ev[i][1]));
+      P->timer_ms[i] += ms;
+      P->timer_launches[i] += 1;
+    }
+    P->timer_pending = false;
+  }
   const DevTx& t = *P->host_tx;
   if (t.overflow || t.log_words > t.log_cap) return WHIR_ETRANSCRIPT;
   const size_t bytes = (size_t)t.log_words * 4;
