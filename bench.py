@@ -84,7 +84,19 @@ def main():
     ap.add_argument("--k", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this size (for HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration_mib:
+        import whir_amd
+
+        n = args.copy_calibration_mib << 20
+        a, b = whir_amd.api.DeviceBuffer(n), whir_amd.api.DeviceBuffer(n)
+        for _ in range(3):
+            whir_amd.lib().whir_memcpy_d2d(b.ptr, a.ptr, n, None)
+        whir_amd.lib().whir_device_synchronize()
+        print(json.dumps({"copy_calibration_bytes": n, "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the proof does not shard across GPUs here; see DESIGN.md).", file=sys.stderr)

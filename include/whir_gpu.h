@@ -117,6 +117,7 @@ int whir_device_alloc(size_t bytes, void** out);
 int whir_device_free(void* ptr);
 int whir_memcpy_h2d(void* dst, const void* src, size_t bytes);
 int whir_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int whir_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
 int whir_device_synchronize(void);
 
 #ifdef __cplusplus

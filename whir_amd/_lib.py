@@ -61,6 +61,7 @@ SIGNATURES = [
     ("whir_device_free", ctypes.c_int, [_vp]),
     ("whir_memcpy_h2d", ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     ("whir_memcpy_d2h", ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    ("whir_memcpy_d2d", ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp]),
     ("whir_device_synchronize", ctypes.c_int, []),
 ]
 

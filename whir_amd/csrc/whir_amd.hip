@@ -725,6 +725,10 @@ extern "C" int whir_memcpy_d2h(void* dst, const void* src, size_t bytes) {
   HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   return WHIR_OK;
 }
+extern "C" int whir_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return WHIR_OK;
+}
 extern "C" int whir_device_synchronize(void) {
   HIP_TRY(hipDeviceSynchronize());
   return WHIR_OK;
