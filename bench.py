@@ -21,6 +21,17 @@ The JSON line also carries
   * cpu_baseline: the CPU oracle (a multithreaded C++ restatement of the
     reference prover, "port") proving the same polynomial on the host cores;
     its proof bytes are compared with the GPU's (bit_exact).
+  * batched: 64 independent jobs, variable counts uniform in 16..22 (sizes
+    log-uniform), rates 1/2 and 1/4 mixed, folding 4, seeded; proven by the
+    proof server (hipGraph-captured proofs on several streams, device-resident
+    workspaces) against the same 64 jobs run one by one on the plain path
+    (eager launches, one stream, workspaces cached per shape). Proof bytes of
+    every job are compared between the two.
+  * serving: Poisson arrivals of jobs with 16..24 variables, rate 1/2 or 1/4,
+    1..4 evaluation claims, seeded; offered at three loads (fractions of the
+    plain path's measured capacity on this job mix); proofs/s and p50/p99
+    latency (arrival to proof on the host) for the server and for the plain
+    per-job FIFO path; every job's bytes compared between the two.
 """
 import argparse
 import json
@@ -76,6 +87,196 @@ def cpu_baseline(poly, k, gpu_proof, threads):
     }
 
 
+class PlainPath:
+    """The plain GPU path run job by job: one stream, eager launches, a cached
+    workspace per proof shape, the host waits for each proof."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.provers = {}
+
+    def prove(self, prm, d_poly):
+        import whir_amd
+
+        key = tuple(getattr(prm, f) for f, _ in prm._fields_)
+        p = self.provers.get(key)
+        if p is None:
+            p = self.provers[key] = whir_amd.Prover(prm, self.ctx)
+        p.enqueue(d_poly)
+        return p.result()[0]
+
+    def close(self):
+        for p in self.provers.values():
+            p.close()
+        self.provers = {}
+
+
+def _percentile(xs, q):
+    return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
+
+
+def _upload_jobs(specs, poly_of):
+    import whir_amd
+
+    bufs = []
+    for i, s in enumerate(specs):
+        poly = poly_of(i, s)
+        d = whir_amd.api.DeviceBuffer(poly.nbytes)
+        d.upload(poly)
+        bufs.append(d)
+    return bufs
+
+
+def _drain(srv, specs, bufs):
+    """Submit every job at once and wait for all; returns the proofs in job order."""
+    ids = [srv.submit(p, b.ptr) for p, b in zip(specs, bufs)]
+    pending = set(ids)
+    while pending:
+        pending.difference_update(srv.poll())
+    return [srv.take(j, 1 << 22)[0] for j in ids]
+
+
+def batched_leg(ctx, slots, reps=3, n_jobs=64, seed=4):
+    """64 independent proofs of mixed sizes: server (overlapped, graph-captured)
+    versus the plain path job by job."""
+    import whir_amd
+
+    rng = np.random.default_rng(seed)
+    ks = rng.integers(16, 23, n_jobs)
+    rates = rng.choice([1, 2], n_jobs)
+    specs = [whir_amd.params(int(k), FOLDING, int(r), IR, SOUNDNESS, SECURITY, 1) for k, r in zip(ks, rates)]
+    bufs = _upload_jobs(specs, lambda i, s: synthetic_poly(1000 + i, s.k))
+    plain = PlainPath(ctx)
+    srv = whir_amd.Server(slots, 64 << 30, ctx=ctx)
+    ref = [plain.prove(p, b.ptr) for p, b in zip(specs, bufs)]  # warm-up: workspaces
+    _drain(srv, specs, bufs)  # warm-up: workspaces + graph capture
+    t_plain, t_srv, same = [], [], True
+    for _ in range(reps):
+        whir_amd.lib().whir_device_synchronize()
+        t0 = time.perf_counter()
+        got = [plain.prove(p, b.ptr) for p, b in zip(specs, bufs)]
+        t_plain.append(time.perf_counter() - t0)
+        same &= got == ref
+        t0 = time.perf_counter()
+        got = _drain(srv, specs, bufs)
+        t_srv.append(time.perf_counter() - t0)
+        same &= got == ref
+    srv.close()
+    plain.close()
+    tp, ts = min(t_plain), min(t_srv)
+    return {
+        "workload": f"{n_jobs} jobs, k uniform in [16,22] (seed {seed}), rates 1/2+1/4 mixed, folding 4, 1 claim",
+        "jobs_by_k": {int(k): int((ks == k).sum()) for k in sorted(set(ks.tolist()))},
+        "server_slots": slots,
+        "plain_proofs_per_s": round(n_jobs / tp, 1),
+        "batched_proofs_per_s": round(n_jobs / ts, 1),
+        "speedup": round(tp / ts, 3),
+        "plain_ms": round(tp * 1e3, 2),
+        "batched_ms": round(ts * 1e3, 2),
+        "reps": reps,
+        "bit_identical": bool(same),
+    }
+
+
+def _serving_specs(n, seed):
+    import whir_amd
+
+    rng = np.random.default_rng(seed)
+    ks = rng.integers(16, 25, n)
+    rates = rng.choice([1, 2], n)
+    npts = rng.integers(1, 5, n)
+    return [whir_amd.params(int(k), FOLDING, int(r), IR, SOUNDNESS, SECURITY, int(m))
+            for k, r, m in zip(ks, rates, npts)]
+
+
+def _run_plain_arrivals(plain, specs, bufs, arrivals):
+    done = []
+    proofs = []
+    t0 = time.perf_counter()
+    for p, b, a in zip(specs, bufs, arrivals):
+        while time.perf_counter() - t0 < a:
+            pass
+        proofs.append(plain.prove(p, b.ptr))
+        done.append(time.perf_counter() - t0)
+    return np.asarray(done), proofs
+
+
+def _run_server_arrivals(srv, specs, bufs, arrivals):
+    n = len(specs)
+    done = np.zeros(n)
+    idx = {}
+    i, finished = 0, 0
+    t0 = time.perf_counter()
+    while finished < n:
+        now = time.perf_counter() - t0
+        while i < n and arrivals[i] <= now:
+            idx[srv.submit(specs[i], bufs[i].ptr)] = i
+            i += 1
+        for jid in srv.poll():
+            done[idx[jid]] = time.perf_counter() - t0
+            finished += 1
+    proofs = [None] * n
+    for jid, j in idx.items():
+        proofs[j] = srv.take(jid, 1 << 22)[0]
+    return done, proofs
+
+
+def serving_leg(ctx, slots, n_jobs=240, seed=5, loads=(0.5, 1.0, 1.5)):
+    """Poisson arrivals at three offered loads; server versus plain FIFO."""
+    import whir_amd
+
+    specs = _serving_specs(n_jobs, seed)
+    # three seeded polynomials per variable count, shared by the jobs of that size
+    pool = {}
+
+    def poly_of(i, s):
+        key = (s.k, i % 3)
+        if key not in pool:
+            pool[key] = synthetic_poly(2000 + 10 * s.k + i % 3, s.k)
+        return pool[key]
+
+    bufs = _upload_jobs(specs, poly_of)
+    pool.clear()
+    plain = PlainPath(ctx)
+    srv = whir_amd.Server(slots, 64 << 30, ctx=ctx)
+    ref = [plain.prove(p, b.ptr) for p, b in zip(specs, bufs)]  # warm-up
+    _drain(srv, specs, bufs)
+    t0 = time.perf_counter()
+    for p, b in zip(specs, bufs):
+        plain.prove(p, b.ptr)
+    mean_service = (time.perf_counter() - t0) / n_jobs
+    plain_capacity = 1.0 / mean_service
+    rng = np.random.default_rng(seed + 1)
+    out = {
+        "workload": f"{n_jobs} jobs, k uniform in [16,24], rate 1/2 or 1/4, 1..4 claims (seed {seed}), Poisson arrivals",
+        "server_slots": slots,
+        "plain_capacity_proofs_per_s": round(plain_capacity, 1),
+        "loads": [],
+    }
+    same = True
+    for load in loads:
+        lam = load * plain_capacity
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, n_jobs))
+        arrivals -= arrivals[0]
+        row = {"offered_load": load, "offered_proofs_per_s": round(lam, 1)}
+        for name, fn, obj in (("plain", _run_plain_arrivals, plain), ("server", _run_server_arrivals, srv)):
+            done, proofs = fn(obj, specs, bufs, arrivals)
+            lat = (done - arrivals) * 1e3
+            row[name] = {
+                "proofs_per_s": round(n_jobs / done.max(), 1),
+                "p50_ms": round(_percentile(lat, 50), 3),
+                "p99_ms": round(_percentile(lat, 99), 3),
+            }
+            same &= proofs == ref
+        out["loads"].append(row)
+    out["bit_identical"] = bool(same)
+    srv.close()
+    plain.close()
+    for b in bufs:
+        b.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +285,10 @@ def main():
     ap.add_argument("--k", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--slots", type=int, default=4, help="proof-server streams (batched and serving legs)")
+    ap.add_argument("--serve-jobs", type=int, default=240)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (for HBM counter calibration)")
     args = ap.parse_args()
@@ -180,6 +385,11 @@ def main():
         "roofline": roofline,
         "roofline_dft": roofline_dft,
     }
+    prover.close()
+    if not args.no_batched:
+        out["batched"] = batched_leg(ctx, args.slots)
+    if not args.no_serving:
+        out["serving"] = serving_leg(ctx, args.slots, n_jobs=args.serve_jobs)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(poly, args.k, proof, threads)

@@ -31,7 +31,8 @@ typedef enum {
   WHIR_EIO = 3,
   WHIR_EINVAL = 4,      /* shape or parameter outside what the reference accepts */
   WHIR_EOOM = 5,        /* device allocation failed */
-  WHIR_EDEVICE = 6      /* HIP runtime error */
+  WHIR_EDEVICE = 6,     /* HIP runtime error */
+  WHIR_EBUSY = 7        /* the queried proof is still running */
 } whir_status;
 
 /* SecurityAssumption (crates/whir/src/params.rs:6-21). */
@@ -97,6 +98,16 @@ int whir_prover_enqueue(whir_prover* prover, const uint32_t* d_poly, void* strea
 int whir_prover_result(whir_prover* prover, uint8_t* proof, size_t cap, size_t* len, uint32_t* checkpoint,
                        uint32_t* root8);
 
+/* Capture the whole proof (kernels + result copies) as a hipGraph on the first
+ * enqueue and replay it afterwards; the graph proves the prover's own input
+ * buffer (other d_poly are first copied into it on the stream). */
+int whir_prover_set_graph(whir_prover* prover, int enable);
+/* WHIR_OK once the last enqueued proof has finished, WHIR_EBUSY while running. */
+int whir_prover_query(whir_prover* prover);
+/* Device bytes held by this prover / needed by a prover of this shape. */
+size_t whir_prover_device_bytes(const whir_prover* prover);
+size_t whir_prover_bytes_for(const whir_params* params);
+
 /* Optional device timers: HIP events recorded on the prover's stream around one
  * kernel class of each proof (the reference's tracing spans "dft-base" and
  * "leaf_hash", crates/whir/src/whir.rs:33, crates/merkle/src/poseidon_packed.rs:285).
@@ -111,6 +122,25 @@ int whir_prover_timer(const whir_prover* prover, int which, double* total_ms, ui
 /* Host-to-host convenience: upload, prove, download. */
 int whir_prove_host(whir_ctx* ctx, const whir_params* params, const uint32_t* poly, uint8_t* proof, size_t cap,
                     size_t* len, uint32_t* checkpoint, uint32_t* root8);
+
+/* ---- proof server: continuous batching of independent proofs ----
+ * No counterpart in the reference (its tests prove one polynomial at a time);
+ * this is the serving form of the same prover. Jobs run on `max_slots` streams
+ * and overlap on the device; workspaces stay device-resident per proof shape
+ * (graph-captured once); admission is FIFO and gated on hbm_budget_bytes.
+ * A job's proof bytes are the same as when it is proven alone. */
+typedef struct whir_server whir_server;
+int whir_server_create(whir_ctx* ctx, int max_slots, uint64_t hbm_budget_bytes, whir_server** out);
+int whir_server_destroy(whir_server* server);
+/* Queues a proof of the 2^k words at d_poly (device memory the caller keeps
+ * valid until the job has finished). */
+int whir_server_submit(whir_server* server, const whir_params* params, const uint32_t* d_poly, int64_t* job_id);
+/* Collects finished jobs (up to `max` ids) and admits queued ones; never blocks. */
+int whir_server_poll(whir_server* server, int64_t* done_ids, int max, int* n_done);
+/* Proof of a finished job, which is then forgotten; returns the job's status. */
+int whir_server_take(whir_server* server, int64_t job_id, uint8_t* proof, size_t cap, size_t* len,
+                     uint32_t* checkpoint);
+int whir_server_stats(const whir_server* server, int* running, int* queued, uint64_t* hbm_bytes);
 
 /* ---- device memory helpers (plain HIP allocations) ---- */
 int whir_device_alloc(size_t bytes, void** out);

@@ -19,7 +19,10 @@ from .api import (  # noqa: F401
     UNIQUE_DECODING,
     Context,
     Prover,
+    Server,
     Whir,
+    params,
+    prover_bytes_for,
     dft_batch,
     merkle_commit,
     poseidon2_permute,

@@ -14,6 +14,7 @@ STATUS = {
     4: "WHIR_EINVAL",
     5: "WHIR_EOOM",
     6: "WHIR_EDEVICE",
+    7: "WHIR_EBUSY",
 }
 
 
@@ -52,6 +53,19 @@ SIGNATURES = [
     ("whir_prover_enqueue", ctypes.c_int, [_vp, _vp, _vp]),
     ("whir_prover_result", ctypes.c_int,
      [_vp, _u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
+    ("whir_prover_set_graph", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("whir_prover_query", ctypes.c_int, [_vp]),
+    ("whir_prover_device_bytes", ctypes.c_size_t, [_vp]),
+    ("whir_prover_bytes_for", ctypes.c_size_t, [ctypes.POINTER(whir_params)]),
+    ("whir_server_create", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    ("whir_server_destroy", ctypes.c_int, [_vp]),
+    ("whir_server_submit", ctypes.c_int, [_vp, ctypes.POINTER(whir_params), _vp, ctypes.POINTER(ctypes.c_int64)]),
+    ("whir_server_poll", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int)]),
+    ("whir_server_take", ctypes.c_int, [_vp, ctypes.c_int64, _u8p, ctypes.c_size_t,
+                                         ctypes.POINTER(ctypes.c_size_t), _u32p]),
+    ("whir_server_stats", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_uint64)]),
     ("whir_prover_set_timers", ctypes.c_int, [_vp, ctypes.c_int]),
     ("whir_prover_timer", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),

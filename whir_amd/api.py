@@ -149,6 +149,15 @@ class Prover:
 
     TIMER_LEAVES_BASE, TIMER_DFT_BASE = 0, 1
 
+    def set_graph(self, enable=True):
+        check(lib().whir_prover_set_graph(self.ptr, int(enable)), "whir_prover_set_graph")
+
+    def done(self):
+        rc = lib().whir_prover_query(self.ptr)
+        if rc not in (0, 7):
+            check(rc, "whir_prover_query")
+        return rc == 0
+
     def set_timers(self, enable=True):
         check(lib().whir_prover_set_timers(self.ptr, int(enable)), "whir_prover_set_timers")
 
@@ -198,3 +207,58 @@ class Whir:
         pr.upload(poly)
         pr.enqueue()
         return pr.result()
+
+
+def params(k, folding=4, rate=1, initial_reduction=3, soundness=JOHNSON_BOUND, security_level=123, n_points=1):
+    return whir_params(k, folding, rate, initial_reduction, soundness, security_level, n_points)
+
+
+def prover_bytes_for(p):
+    return lib().whir_prover_bytes_for(ctypes.byref(p))
+
+
+class Server:
+    """whir_server: continuous batching of independent proofs (see include/whir_gpu.h)."""
+
+    def __init__(self, max_slots=4, hbm_budget_bytes=64 << 30, ctx=None):
+        self.ctx = ctx or default_context()
+        p = ctypes.c_void_p()
+        check(lib().whir_server_create(self.ctx.ptr, max_slots, hbm_budget_bytes, ctypes.byref(p)),
+              "whir_server_create")
+        self.ptr = p.value
+        self._ids = (ctypes.c_int64 * 256)()
+
+    def submit(self, p, d_poly):
+        jid = ctypes.c_int64()
+        check(lib().whir_server_submit(self.ptr, ctypes.byref(p), d_poly, ctypes.byref(jid)), "whir_server_submit")
+        return jid.value
+
+    def poll(self):
+        n = ctypes.c_int()
+        check(lib().whir_server_poll(self.ptr, self._ids, 256, ctypes.byref(n)), "whir_server_poll")
+        return [self._ids[i] for i in range(n.value)]
+
+    def take(self, job_id, capacity):
+        buf = np.zeros(capacity, np.uint8)
+        ln = ctypes.c_size_t()
+        cp = np.zeros(1, np.uint32)
+        check(lib().whir_server_take(self.ptr, job_id, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
+                                     cp.ctypes.data_as(_u32p)), "whir_server_take")
+        return bytes(buf[: ln.value]), int(cp[0])
+
+    def stats(self):
+        r, q, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
+        check(lib().whir_server_stats(self.ptr, ctypes.byref(r), ctypes.byref(q), ctypes.byref(b)),
+              "whir_server_stats")
+        return r.value, q.value, b.value
+
+    def close(self):
+        if self.ptr:
+            lib().whir_server_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -362,12 +362,24 @@ struct whir_prover {
   double timer_ms[WHIR_TIMER_COUNT] = {};
   uint64_t timer_launches[WHIR_TIMER_COUNT] = {};
   bool timer_pending = false;
+  // stream of the last enqueue; optional hipGraph of the whole proof
+  hipStream_t last_stream = nullptr;
+  bool graph_mode = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipStream_t graph_stream = nullptr;
+  hipEvent_t done = nullptr;
 };
 
 static inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
-extern "C" int whir_prover_create(whir_ctx* c, const whir_params* prm, whir_prover** out) {
-  if (!c || !prm || !out) return WHIR_EINVAL;
+struct ArenaItem {
+  void** ptr;
+  size_t bytes;
+};
+
+// Fills P's plan-derived fields and the list of device buffers of one proof.
+static int prover_layout(whir_prover* P, const whir_params* prm, std::vector<ArenaItem>& items) {
   Plan pl = whir_host::make_plan(prm->k, prm->folding, prm->rate, prm->initial_reduction, prm->soundness,
                                  prm->security_level, prm->n_points);
   if (!pl.ok) return WHIR_EINVAL;
@@ -384,9 +396,6 @@ extern "C" int whir_prover_create(whir_ctx* c, const whir_params* prm, whir_prov
     max_cw_words = std::max<uint64_t>(max_cw_words, (4ull << f) << r.log_rows);
   }
   if (max_stir > MAX_STIR || max_ood > 16) return WHIR_EINVAL;
-
-  whir_prover* P = new whir_prover();
-  P->ctx = c;
   P->plan = pl;
   P->n_svo_claims = n_svo;
   const int m = k - f;
@@ -399,12 +408,10 @@ extern "C" int whir_prover_create(whir_ctx* c, const whir_params* prm, whir_prov
   uint64_t max_rows = 1;
   for (const RoundPlan& r : pl.rounds) max_rows = std::max<uint64_t>(max_rows, 1ull << r.log_rows);
   const uint64_t n_ext = 1ull << m;
-  int m_pow = std::max(0, k - 2 * f);
+  const int m_pow = std::max(0, k - 2 * f);
   const uint64_t pow_l = 1ull << (m_pow / 2), pow_r = 1ull << (m_pow - m_pow / 2);
   const uint64_t part_ext = std::max<uint64_t>({(1ull << P->m1_svo) << f, 1ull << P->m1_svo, 4096});
-
-  struct Item { void** ptr; size_t bytes; };
-  std::vector<Item> items = {
+  items = {
       {(void**)&P->tx, sizeof(DevTx)},
       {(void**)&P->js, sizeof(JobScalars)},
       {(void**)&P->log, pl.log_words * 4},
@@ -435,9 +442,36 @@ extern "C" int whir_prover_create(whir_ctx* c, const whir_params* prm, whir_prov
       {(void**)&P->powL, MAX_STIR * pow_l * 4},
       {(void**)&P->powR, MAX_STIR * pow_r * 16},
   };
+  return WHIR_OK;
+}
+
+static size_t arena_size(const std::vector<ArenaItem>& items) {
   size_t total = 0;
   for (auto& it : items) total += align_up(it.bytes);
+  return total;
+}
+
+// Device bytes one proof of this shape needs (0 if the shape is invalid).
+extern "C" size_t whir_prover_bytes_for(const whir_params* prm) {
+  whir_prover tmp;
+  std::vector<ArenaItem> items;
+  if (!prm || prover_layout(&tmp, prm, items)) return 0;
+  return arena_size(items);
+}
+
+extern "C" int whir_prover_create(whir_ctx* c, const whir_params* prm, whir_prover** out) {
+  if (!c || !prm || !out) return WHIR_EINVAL;
+  whir_prover* P = new whir_prover();
+  P->ctx = c;
+  std::vector<ArenaItem> items;
+  int rc = prover_layout(P, prm, items);
+  if (rc) {
+    delete P;
+    return rc;
+  }
+  const size_t total = arena_size(items);
   if (hipMalloc(&P->arena, total) != hipSuccess) {
+    (void)hipGetLastError();
     delete P;
     return WHIR_EOOM;
   }
@@ -447,6 +481,7 @@ extern "C" int whir_prover_create(whir_ctx* c, const whir_params* prm, whir_prov
     *it.ptr = p;
     p += align_up(it.bytes);
   }
+  const Plan& pl = P->plan;
   if (hipHostMalloc((void**)&P->host_log, pl.log_words * 4, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&P->host_js, sizeof(JobScalars), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&P->host_tx, sizeof(DevTx), hipHostMallocDefault) != hipSuccess) {
@@ -466,6 +501,9 @@ extern "C" int whir_prover_destroy(whir_prover* P) {
   for (int i = 0; i < WHIR_TIMER_COUNT; ++i)
     for (int j = 0; j < 2; ++j)
       if (P->ev[i][j]) (void)hipEventDestroy(P->ev[i][j]);
+  if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
+  if (P->graph) (void)hipGraphDestroy(P->graph);
+  if (P->done) (void)hipEventDestroy(P->done);
   delete P;
   return WHIR_OK;
 }
@@ -658,9 +696,7 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
   return WHIR_OK;
 }
 
-extern "C" int whir_prover_enqueue(whir_prover* P, const uint32_t* d_poly, void* stream) {
-  if (!P || !d_poly) return WHIR_EINVAL;
-  hipStream_t s = pick_stream(P->ctx, stream);
+static int enqueue_with_copies(whir_prover* P, const uint32_t* d_poly, hipStream_t s) {
   int rc = prover_enqueue(P, d_poly, s);
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(P->host_tx, P->tx, sizeof(DevTx), hipMemcpyDeviceToHost, s));
@@ -669,10 +705,56 @@ extern "C" int whir_prover_enqueue(whir_prover* P, const uint32_t* d_poly, void*
   return WHIR_OK;
 }
 
+extern "C" int whir_prover_set_graph(whir_prover* P, int enable) {
+  if (!P) return WHIR_EINVAL;
+  P->graph_mode = enable != 0;
+  return WHIR_OK;
+}
+
+extern "C" int whir_prover_enqueue(whir_prover* P, const uint32_t* d_poly, void* stream) {
+  if (!P || !d_poly) return WHIR_EINVAL;
+  hipStream_t s = pick_stream(P->ctx, stream);
+  P->last_stream = s;
+  if (!P->done) HIP_TRY(hipEventCreateWithFlags(&P->done, hipEventDisableTiming));
+  if (!P->graph_mode) {
+    int rc = enqueue_with_copies(P, d_poly, s);
+    if (rc) return rc;
+  } else {
+    // The graph always proves the prover's own input buffer.
+    if (P->timers) return WHIR_EINVAL;
+    if (d_poly != P->input)
+      HIP_TRY(hipMemcpyAsync(P->input, d_poly, (1ull << P->plan.k) * 4, hipMemcpyDeviceToDevice, s));
+    if (!P->gexec) {
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue_with_copies(P, P->input, s);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return WHIR_EDEVICE;
+      P->graph = g;
+      HIP_TRY(hipGraphInstantiate(&P->gexec, g, nullptr, nullptr, 0));
+      P->graph_stream = s;
+    }
+    HIP_TRY(hipGraphLaunch(P->gexec, s));
+  }
+  HIP_TRY(hipEventRecord(P->done, s));
+  return WHIR_OK;
+}
+
+// WHIR_OK when the last enqueued proof has finished, 7 while it is running.
+extern "C" int whir_prover_query(whir_prover* P) {
+  if (!P || !P->done) return WHIR_EINVAL;
+  hipError_t e = hipEventQuery(P->done);
+  if (e == hipSuccess) return WHIR_OK;
+  if (e == hipErrorNotReady) return 7;
+  return WHIR_EDEVICE;
+}
+
 extern "C" int whir_prover_result(whir_prover* P, uint8_t* proof, size_t cap, size_t* len, uint32_t* checkpoint,
                                   uint32_t* root8) {
   if (!P) return WHIR_EINVAL;
-  HIP_TRY(hipDeviceSynchronize());
+  if (P->done) HIP_TRY(hipEventSynchronize(P->done));
+  else HIP_TRY(hipDeviceSynchronize());
   if (P->timer_pending) {
     for (int i = 0; i < WHIR_TIMER_COUNT; ++i) {
       float ms = 0.f;
@@ -694,6 +776,8 @@ extern "C" int whir_prover_result(whir_prover* P, uint8_t* proof, size_t cap, si
   }
   return WHIR_OK;
 }
+
+extern "C" size_t whir_prover_device_bytes(const whir_prover* P) { return P ? P->arena_bytes : 0; }
 
 extern "C" int whir_prove_host(whir_ctx* c, const whir_params* prm, const uint32_t* poly, uint8_t* proof, size_t cap,
                                size_t* len, uint32_t* checkpoint, uint32_t* root8) {
@@ -733,3 +817,5 @@ extern "C" int whir_device_synchronize(void) {
   HIP_TRY(hipDeviceSynchronize());
   return WHIR_OK;
 }
+
+#include "server.h"
