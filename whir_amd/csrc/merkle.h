@@ -41,7 +41,8 @@ __global__ void __launch_bounds__(MERKLE_THREADS) merkle_leaves_kernel(const uin
 #pragma unroll
       for (int j = 0; j < FULL; ++j) s[j] = row[c * 16 + j];
     }
-    permute<24>(s);
+    if (c + 1 < CHUNKS) permute<24, 16, 24>(s);  // the capacity carries over
+    else permute<24, 0, 8>(s);
   }
   uint4* out = reinterpret_cast<uint4*>(leaves + i * 8);
   out[0] = make_uint4(s[0], s[1], s[2], s[3]);

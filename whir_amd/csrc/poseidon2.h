@@ -1,14 +1,33 @@
 // Poseidon2 over KoalaBear, widths 16 and 24, S-box x^3: one permutation per
-// lane with the whole state in VGPRs. Round constants sit in __constant__
-// memory (uniform -> scalar loads); the internal diagonal V is applied with
-// shifts/adds (multiplication by +-2^-n uses p = 127 * 2^24 + 1) instead of
-// Montgomery multiplies.
+// lane with the whole state in VGPRs.
 //
 // Structure (Plonky3 p3-poseidon2 as instantiated by the reference,
 // crates/whir/src/test.rs:185-199): MDS-light; 4 external rounds (+RC, x^3,
 // MDS-light); R_P internal rounds (+RC on s0, x^3 on s0, 1*1^T + diag(V));
-// 4 external rounds.
+// 4 external rounds. The results are the reference's words, bit for bit; the
+// arithmetic inside is organised for gfx950's VALU:
+//
+// * Lazy reduction. Every linear layer is accumulated exactly in 64 bits with
+//   v_mad_u64_u32 (32x32+64 in one full-rate instruction) and reduced once per
+//   output by a Montgomery REDC (T -> T * 2^-32 mod p: one mul_lo and one mad),
+//   never through chains of modular additions. Outputs stay unreduced where
+//   the next consumer's bounds allow (see the bounds on each function).
+// * Scaled state. REDC multiplies by R^-1 (R = 2^32), so between layers the
+//   state holds x = w * a (mod p) for the reference words w and a known scale
+//   a that depends only on the round position. Linear layers with integer
+//   coefficients and the S-box commute with this (x^3 -> scale a^3), so the
+//   only corrections are: round constants folded into the preceding linear
+//   layer's 64-bit sum already multiplied by the scale there; one extra
+//   Montgomery multiply of s0 per internal round (by kappa = a^-2 R) to keep all
+//   lanes on one scale; and one multiply per output word at the end.
+//   Poseidon2Lazy holds these per-round constants, made on the host from the
+//   plain round constants by lazy_consts_build().
+// Bounds (p = 2^31 - 2^24 + 1, R = 2^32, REDC(T) < T / R + p, valid for
+// T < p R): MDS-light sums of inputs < 2p stay below 99p, so their REDC is
+// < p + 2^7 ("near-canonical", safe as both factors of a square); an internal
+// layer sum + d_i x_i with d_i < p and x_i < 2p stays below 2p^2 + 48p < p R.
 #pragma once
+#include <cstring>
 #include "field.h"
 
 namespace kb {
@@ -26,145 +45,261 @@ __constant__ Poseidon2Consts c_p2;
 
 __device__ __forceinline__ uint32_t cube(uint32_t x) { return mul(sqr(x), x); }
 
-// x * 2^-n for 1 <= n <= 24: since p = 1 (mod 2^24), k = (-x) mod 2^n makes
-// x + k*p divisible by 2^n, and (x + k*p) / 2^n = ((x + k) >> n) + 127*k*2^(24-n).
-template <int N>
-__device__ __forceinline__ uint32_t mul_2exp_neg(uint32_t x) {
-  const uint32_t mask = (1u << N) - 1u;
-  uint32_t k = (0u - x) & mask;
-  uint32_t r = ((x + k) >> N) + ((k * 127u) << (24 - N));
-  uint32_t t = r - P;
-  return r < t ? r : t;
-}
-
-// Plonky3 MDSMat4: [[2,3,1,1],[1,2,3,1],[1,1,2,3],[3,1,1,2]].
-__device__ __forceinline__ void mat4(uint32_t& x0, uint32_t& x1, uint32_t& x2, uint32_t& x3) {
-  uint32_t t01 = add(x0, x1);
-  uint32_t t23 = add(x2, x3);
-  uint32_t t0123 = add(t01, t23);
-  uint32_t t01123 = add(t0123, x1);
-  uint32_t t01233 = add(t0123, x3);
-  uint32_t y3 = add(t01233, dbl(x0));
-  uint32_t y1 = add(t01123, dbl(x2));
-  uint32_t y0 = add(t01123, t01);
-  uint32_t y2 = add(t01233, t23);
-  x0 = y0;
-  x1 = y1;
-  x2 = y2;
-  x3 = y3;
+// Per-round constants of the lazily reduced permutation of width W (values in
+// [0, p)).
+//   lin_add[0]     : ext round 0 RC, folded into the first MDS-light
+//   lin_add[1..3]  : ext rounds 1..3 RC, folded into ext rounds 0..2's MDS
+//   lin_add[4][0]  : internal round 0 RC (lane 0 only), into ext round 3's MDS
+//   int_add[r]     : internal round r RC (r >= 1), into round r-1's layer
+//   lin_add[5]     : terminal round 0 RC, into the last internal layer
+//   lin_add[6..8]  : terminal rounds 1..3 RC, into terminal rounds 0..2's MDS
+//   kappa[r]       : s0 scale correction of internal round r
+//   fix            : a_final^-1 * R, turns the final state back into words
+// diag(i) is V_i as an integer in [0, p) (Plonky3 KoalaBear INTERNAL_V16 /
+// INTERNAL_V24: sign * num / 2^sh).
+struct DiagTerm {
+  int sign;
+  uint32_t num;
+  int sh;
+};
+constexpr DiagTerm kV16[16] = {{-1, 2, 0}, {1, 1, 0}, {1, 2, 0}, {1, 1, 1}, {1, 3, 0},  {1, 4, 0},
+                               {-1, 1, 1}, {-1, 3, 0}, {-1, 4, 0}, {1, 1, 8}, {1, 1, 3}, {1, 1, 24},
+                               {-1, 1, 8}, {-1, 1, 3}, {-1, 1, 4}, {-1, 1, 24}};
+constexpr DiagTerm kV24[24] = {{-1, 2, 0}, {1, 1, 0},  {1, 2, 0},  {1, 1, 1},  {1, 3, 0},  {1, 4, 0},
+                               {-1, 1, 1}, {-1, 3, 0}, {-1, 4, 0}, {1, 1, 8},  {1, 1, 2},  {1, 1, 3},
+                               {1, 1, 4},  {1, 1, 5},  {1, 1, 6},  {1, 1, 24}, {-1, 1, 8}, {-1, 1, 3},
+                               {-1, 1, 4}, {-1, 1, 5}, {-1, 1, 6}, {-1, 1, 7}, {-1, 1, 9}, {-1, 1, 24}};
+// num * 2^-sh mod p with sign; 2^-k = p - (p - 1) / 2^k since p = 1 (mod 2^24).
+constexpr uint32_t diag_value(DiagTerm d) {
+  const uint64_t inv = d.sh == 0 ? 1 : (uint64_t)P - ((uint64_t)(P - 1) >> d.sh);
+  const uint32_t v = (uint32_t)((uint64_t)d.num * inv % P);
+  return d.sign < 0 ? (v ? P - v : 0u) : v;
 }
 
 template <int W>
-__device__ __forceinline__ void mds_light(uint32_t (&s)[W]) {
+struct Poseidon2Lazy {
+  static constexpr int RP = W == 16 ? 20 : 23;
+  uint32_t lin_add[9][W];
+  uint32_t int_add[RP];
+  uint32_t kappa[RP];
+  uint32_t fix;
+  KB_HD static constexpr uint32_t diag(int i) { return diag_value(W == 16 ? kV16[i] : kV24[i]); }
+};
+
+struct Poseidon2LazyConsts {
+  Poseidon2Lazy<16> w16;
+  Poseidon2Lazy<24> w24;
+};
+
+__constant__ Poseidon2LazyConsts c_p2l;
+
+template <int W>
+__device__ __forceinline__ const Poseidon2Lazy<W>& lazy_consts() {
+  if constexpr (W == 16) return c_p2l.w16;
+  else return c_p2l.w24;
+}
+
+KB_HD uint64_t mad64(uint32_t a, uint32_t b, uint64_t c) { return (uint64_t)a * b + c; }
+
+// a * K + c in one v_mad_u64_u32 with K an inline constant (1 or 2); written
+// as inline ALU asm so that the compiler keeps one instruction per term instead
+// of factoring the sums into 64 x 32-bit products.
+template <int K>
+KB_HD uint64_t mad_k(uint32_t a, uint64_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t d;
+  asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(d) : "v"(a), "i"(K), "v"(c) : "vcc");
+  return d;
+#else
+  return (uint64_t)a * K + c;
+#endif
+}
+// a * K (zero-extended to 64 bits) in one instruction.
+template <int K>
+KB_HD uint64_t mul_k(uint32_t a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t d;
+  asm("v_mad_u64_u32 %0, vcc, %1, %2, 0" : "=v"(d) : "v"(a), "i"(K) : "vcc");
+  return d;
+#else
+  return (uint64_t)a * K;
+#endif
+}
+// c + u for a wave-uniform u (an SGPR operand), one instruction.
+KB_HD uint64_t add_u(uint64_t c, uint32_t u) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t d;
+  asm("v_mad_u64_u32 %0, vcc, %1, 1, %2" : "=v"(d) : "s"(u), "v"(c) : "vcc");
+  return d;
+#else
+  return c + u;
+#endif
+}
+// a * b + c for a wave-uniform b (an SGPR operand), one instruction.
+KB_HD uint64_t mad_u(uint32_t a, uint32_t b, uint64_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t d;
+  asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c) : "vcc");
+  return d;
+#else
+  return (uint64_t)a * b + c;
+#endif
+}
+// T * R^-1 mod p, in [0, T / R + p) for T < p R.
+KB_HD uint32_t redc(uint64_t T) {
+  const uint32_t m = (uint32_t)T * NPINV;
+  return (uint32_t)(mad64(m, P, T) >> 32);
+}
+KB_HD uint32_t mulr(uint32_t a, uint32_t b) { return redc((uint64_t)a * b); }  // needs a * b < p R
+KB_HD uint32_t canon2(uint32_t t) {  // [0, 2p) -> [0, p)
+  const uint32_t u = t - P;
+  return t < u ? t : u;
+}
+
+enum { LAZY_ADD_NONE = 0, LAZY_ADD_FIRST = 1, LAZY_ADD_ALL = 2 };
+
+// x <- REDC(MDS-light(x) + add), x_i < 2p on entry, < p + 2^7 on exit.
+// MDS-light = (J + I) (x) M4 (J all-ones over the 4-chunks); per chunk
+// M4 x = S + (x_a + 2 x_{a+1}) with S the chunk sum (Plonky3's M4 rows).
+template <int W, int ADD>
+KB_HD void mds_lazy(uint32_t (&x)[W], const uint32_t* add) {
+  uint64_t y[W];
 #pragma unroll
-  for (int c = 0; c < W; c += 4) mat4(s[c], s[c + 1], s[c + 2], s[c + 3]);
-  uint32_t sums[4];
+  for (int c = 0; c < W; c += 4) {
+    uint64_t s = mul_k<1>(x[c]);
+    s = mad_k<1>(x[c + 1], s);
+    s = mad_k<1>(x[c + 2], s);
+    s = mad_k<1>(x[c + 3], s);
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    sums[i] = s[i];
+    for (int a = 0; a < 4; ++a) y[c + a] = mad_k<2>(x[c + ((a + 1) & 3)], mad_k<1>(x[c + a], s));
+  }
+  uint64_t sums[4];
 #pragma unroll
-    for (int c = 4; c < W; c += 4) sums[i] = add(sums[i], s[c + i]);
+  for (int a = 0; a < 4; ++a) {
+    sums[a] = y[a];
+#pragma unroll
+    for (int c = 4; c < W; c += 4) sums[a] += y[c + a];
   }
 #pragma unroll
-  for (int i = 0; i < W; ++i) s[i] = add(s[i], sums[i & 3]);
-}
-
-// V_i * x + sum for the KoalaBear internal diagonals
-//   V16 = [-2, 1, 2, 1/2, 3, 4, -1/2, -3, -4, 2^-8, 2^-3, 2^-24, -2^-8, -2^-3, -2^-4, -2^-24]
-//   V24 = [-2, 1, 2, 1/2, 3, 4, -1/2, -3, -4, 2^-8, 2^-2, 2^-3, 2^-4, 2^-5, 2^-6, 2^-24,
-//          -2^-8, -2^-3, -2^-4, -2^-5, -2^-6, -2^-7, -2^-9, -2^-24]
-template <int W, int I>
-__device__ __forceinline__ uint32_t diag_apply(uint32_t x, uint32_t sum) {
-  // common prefix for both widths
-  if constexpr (I == 0) return sub(sub(sum, x), x);
-  else if constexpr (I == 1) return add(sum, x);
-  else if constexpr (I == 2) return add(sum, dbl(x));
-  else if constexpr (I == 3) return add(sum, mul_2exp_neg<1>(x));
-  else if constexpr (I == 4) return add(sum, add(dbl(x), x));
-  else if constexpr (I == 5) return add(sum, dbl(dbl(x)));
-  else if constexpr (I == 6) return sub(sum, mul_2exp_neg<1>(x));
-  else if constexpr (I == 7) return sub(sum, add(dbl(x), x));
-  else if constexpr (I == 8) return sub(sum, dbl(dbl(x)));
-  else if constexpr (W == 16) {
-    if constexpr (I == 9) return add(sum, mul_2exp_neg<8>(x));
-    else if constexpr (I == 10) return add(sum, mul_2exp_neg<3>(x));
-    else if constexpr (I == 11) return add(sum, mul_2exp_neg<24>(x));
-    else if constexpr (I == 12) return sub(sum, mul_2exp_neg<8>(x));
-    else if constexpr (I == 13) return sub(sum, mul_2exp_neg<3>(x));
-    else if constexpr (I == 14) return sub(sum, mul_2exp_neg<4>(x));
-    else return sub(sum, mul_2exp_neg<24>(x));
-  } else {
-    if constexpr (I == 9) return add(sum, mul_2exp_neg<8>(x));
-    else if constexpr (I == 10) return add(sum, mul_2exp_neg<2>(x));
-    else if constexpr (I == 11) return add(sum, mul_2exp_neg<3>(x));
-    else if constexpr (I == 12) return add(sum, mul_2exp_neg<4>(x));
-    else if constexpr (I == 13) return add(sum, mul_2exp_neg<5>(x));
-    else if constexpr (I == 14) return add(sum, mul_2exp_neg<6>(x));
-    else if constexpr (I == 15) return add(sum, mul_2exp_neg<24>(x));
-    else if constexpr (I == 16) return sub(sum, mul_2exp_neg<8>(x));
-    else if constexpr (I == 17) return sub(sum, mul_2exp_neg<3>(x));
-    else if constexpr (I == 18) return sub(sum, mul_2exp_neg<4>(x));
-    else if constexpr (I == 19) return sub(sum, mul_2exp_neg<5>(x));
-    else if constexpr (I == 20) return sub(sum, mul_2exp_neg<6>(x));
-    else if constexpr (I == 21) return sub(sum, mul_2exp_neg<7>(x));
-    else if constexpr (I == 22) return sub(sum, mul_2exp_neg<9>(x));
-    else return sub(sum, mul_2exp_neg<24>(x));
+  for (int i = 0; i < W; ++i) {
+    uint64_t t = y[i] + sums[i & 3];
+    if constexpr (ADD == LAZY_ADD_ALL) t = add_u(t, add[i]);
+    if constexpr (ADD == LAZY_ADD_FIRST) {
+      if (i == 0) t = add_u(t, add[0]);
+    }
+    x[i] = redc(t);
   }
 }
 
-template <int W, int I = 0>
-__device__ __forceinline__ void diag_all(uint32_t (&s)[W], uint32_t sum) {
-  if constexpr (I < W) {
-    s[I] = diag_apply<W, I>(s[I], sum);
-    diag_all<W, I + 1>(s, sum);
-  }
-}
-
-template <int W>
-__device__ __forceinline__ void internal_layer(uint32_t (&s)[W]) {
-  uint32_t sum = s[0];
+// External round: x^3 on every lane (x near-canonical, result < 2p), then MDS.
+template <int W, int ADD>
+KB_HD void external_lazy(uint32_t (&x)[W], const uint32_t* add) {
 #pragma unroll
-  for (int i = 1; i < W; ++i) sum = add(sum, s[i]);
-  diag_all<W>(s, sum);
+  for (int i = 0; i < W; ++i) x[i] = mulr(mulr(x[i], x[i]), x[i]);
+  mds_lazy<W, ADD>(x, add);
 }
 
-template <int W>
-__device__ __forceinline__ uint32_t rc_init(int r, int i) {
-  if constexpr (W == 16) return c_p2.ext16_init[r][i];
-  else return c_p2.ext24_init[r][i];
-}
-template <int W>
-__device__ __forceinline__ uint32_t rc_term(int r, int i) {
-  if constexpr (W == 16) return c_p2.ext16_term[r][i];
-  else return c_p2.ext24_term[r][i];
-}
-template <int W>
-__device__ __forceinline__ uint32_t rc_int(int r) {
-  if constexpr (W == 16) return c_p2.int16[r];
-  else return c_p2.int24[r];
+// Internal round r: s0 <- kappa_r * s0^3 (x0 near-canonical -> < 1.87p), then
+// x_i <- REDC(sum + V_i x_i) with the next round constant folded in; lane 0
+// uses V_0 = -2 as 2 (2p - x0) so that its result is near-canonical for the
+// next S-box. The last round canonicalises every lane (its outputs feed the
+// terminal S-boxes) and folds terminal round 0's constants.
+template <int W, bool LAST>
+KB_HD void internal_lazy(uint32_t (&x)[W], int r, const Poseidon2Lazy<W>& C) {
+  x[0] = mulr(mulr(mulr(x[0], x[0]), x[0]), C.kappa[r]);
+  uint64_t sum = mul_k<1>(x[0]);
+#pragma unroll
+  for (int j = 1; j < W; ++j) sum = mad_k<1>(x[j], sum);
+  uint64_t t0 = add_u(mad_k<2>(2u * P - x[0], sum), LAST ? C.lin_add[5][0] : C.int_add[r + 1]);
+  x[0] = LAST ? canon2(redc(t0)) : redc(t0);
+#pragma unroll
+  for (int i = 1; i < W; ++i) {
+    uint64_t t = mad_u(x[i], Poseidon2Lazy<W>::diag(i), sum);
+    if constexpr (LAST) {
+      t = add_u(t, C.lin_add[5][i]);
+      x[i] = canon2(redc(t));
+    } else {
+      x[i] = redc(t);
+    }
+  }
 }
 
-template <int W>
+// Poseidon2 permutation of width W. In: s = words in [0, p). Out: s[i] for
+// OUT_LO <= i < OUT_HI are the reference's output words in [0, p); the other
+// lanes hold scaled values and must not be used.
+template <int W, int OUT_LO = 0, int OUT_HI = W>
+KB_HD void permute_lazy(uint32_t (&s)[W], const Poseidon2Lazy<W>& C) {
+  constexpr int RP = Poseidon2Lazy<W>::RP;
+  mds_lazy<W, LAZY_ADD_ALL>(s, C.lin_add[0]);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) external_lazy<W, LAZY_ADD_ALL>(s, C.lin_add[r + 1]);
+  external_lazy<W, LAZY_ADD_FIRST>(s, C.lin_add[4]);
+#pragma unroll
+  for (int r = 0; r < RP - 1; ++r) internal_lazy<W, false>(s, r, C);
+  internal_lazy<W, true>(s, RP - 1, C);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) external_lazy<W, LAZY_ADD_ALL>(s, C.lin_add[6 + r]);
+  external_lazy<W, LAZY_ADD_NONE>(s, nullptr);
+#pragma unroll
+  for (int i = OUT_LO; i < OUT_HI; ++i) s[i] = canon2(mulr(s[i], C.fix));
+}
+
+template <int W, int OUT_LO = 0, int OUT_HI = W>
 __device__ __forceinline__ void permute(uint32_t (&s)[W]) {
-  constexpr int RP = W == 16 ? 20 : 23;
-  mds_light<W>(s);
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-#pragma unroll
-    for (int i = 0; i < W; ++i) s[i] = cube(add(s[i], rc_init<W>(r, i)));
-    mds_light<W>(s);
+  permute_lazy<W, OUT_LO, OUT_HI>(s, lazy_consts<W>());
+}
+
+// Host: the constants of permute_lazy from the plain round constants
+// (Montgomery words, in the reference's order) by replaying the scale of every
+// layer: a = 1 at the input, a -> a / R at each REDC, a -> a^3 at each S-box.
+namespace lazy_host {
+inline uint32_t mm(uint64_t a, uint64_t b) { return (uint32_t)(a * b % P); }
+inline uint32_t mpow(uint32_t a, uint64_t e) {
+  uint32_t r = 1;
+  while (e) {
+    if (e & 1) r = mm(r, a);
+    a = mm(a, a);
+    e >>= 1;
   }
-#pragma unroll
+  return r;
+}
+inline uint32_t minv(uint32_t a) { return mpow(a, P - 2); }
+}  // namespace lazy_host
+
+template <int W>
+inline void lazy_consts_build(const uint32_t (&ext_init)[4][W], const uint32_t (&ext_term)[4][W],
+                              const uint32_t* internal, Poseidon2Lazy<W>& C) {
+  using namespace lazy_host;
+  constexpr int RP = Poseidon2Lazy<W>::RP;
+  const uint32_t R = (uint32_t)((1ull << 32) % P), Rinv = minv(R);
+  memset(&C, 0, sizeof(C));
+  uint32_t a = 1;
+  auto cube_scale = [&]() { a = mm(mm(a, a), a); };
+  for (int i = 0; i < W; ++i) C.lin_add[0][i] = mm(ext_init[0][i], a);
+  a = mm(a, Rinv);
+  for (int r = 0; r < 4; ++r) {
+    cube_scale();
+    if (r < 3)
+      for (int i = 0; i < W; ++i) C.lin_add[r + 1][i] = mm(ext_init[r + 1][i], a);
+    else
+      C.lin_add[4][0] = mm(internal[0], a);
+    a = mm(a, Rinv);
+  }
   for (int r = 0; r < RP; ++r) {
-    s[0] = cube(add(s[0], rc_int<W>(r)));
-    internal_layer<W>(s);
+    C.kappa[r] = mm(minv(mm(a, a)), R);
+    if (r + 1 < RP)
+      C.int_add[r + 1] = mm(internal[r + 1], a);
+    else
+      for (int i = 0; i < W; ++i) C.lin_add[5][i] = mm(ext_term[0][i], a);
+    a = mm(a, Rinv);
   }
-#pragma unroll
   for (int r = 0; r < 4; ++r) {
-#pragma unroll
-    for (int i = 0; i < W; ++i) s[i] = cube(add(s[i], rc_term<W>(r, i)));
-    mds_light<W>(s);
+    cube_scale();
+    if (r < 3)
+      for (int i = 0; i < W; ++i) C.lin_add[6 + r][i] = mm(ext_term[r + 1][i], a);
+    a = mm(a, Rinv);
   }
+  C.fix = mm(minv(a), R);
 }
 
 // TruncatedPermutation<P16, 2, 8, 16>: first 8 words of P16(l || r).
@@ -175,7 +310,7 @@ __device__ __forceinline__ void compress(const uint32_t (&l)[8], const uint32_t 
     s[i] = l[i];
     s[8 + i] = r[i];
   }
-  permute<16>(s);
+  permute<16, 0, 8>(s);
 #pragma unroll
   for (int i = 0; i < 8; ++i) out[i] = s[i];
 }

@@ -66,6 +66,12 @@ static int upload_constants(uint64_t seed) {
   static const int sgn[16] = {-1, 1, 1, 1, 1, 1, -1, -1, -1, 1, 1, 1, -1, -1, -1, -1};
   static const uint32_t num[16] = {2, 1, 2, 1, 3, 4, 1, 3, 4, 1, 1, 1, 1, 1, 1, 1};
   static const int sh[16] = {0, 0, 0, 1, 0, 0, 1, 0, 0, 8, 3, 24, 8, 3, 4, 24};
+  {
+    static Poseidon2LazyConsts lz;
+    lazy_consts_build<16>(c.ext16_init, c.ext16_term, c.int16, lz.w16);
+    lazy_consts_build<24>(c.ext24_init, c.ext24_term, c.int24, lz.w24);
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_p2l), &lz, sizeof(lz)));
+  }
   uint32_t d16[16];
   const uint32_t half = kb::HALF;
   for (int i = 0; i < 16; ++i) {
