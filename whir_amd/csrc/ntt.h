@@ -1,182 +1,258 @@
 // Column-batched DFT over KoalaBear (Reed–Solomon encoding), replacing Plonky3
 // Radix2DFTSmallBatch::dft_batch / dft_algebra_batch as called at
 // crates/whir/src/whir.rs:34,59:
-//   out[i][c] = sum_{j < n_in} in[j][c] * w^(i*j),  w = two_adic_generator(log2 N)
-// for an N-row, C-column row-major matrix whose rows >= n_in are zero (the
-// rate padding, whir.rs:32,57). An extension matrix is the same transform on
-// its 4*C base columns.
+//   out[i][c] = sum_{j < n_in} in[j][c] * w_N^(i*j),  w_N = two_adic_generator(log2 N)
+// for the N-row, C-column row-major result of zero-padding the n_in-row input
+// (the rate padding, whir.rs:32,57). An extension matrix is the same transform
+// on its 4*C base columns.
 //
-// Decomposition: N = L_1 * ... * L_P (four-step applied recursively). Pass p
-// works on blocks of M_{p-1} = L_p * M_p rows; for block b, column q (q spans
-// the M_p rows times C columns that follow each other in memory) it computes a
-// length-L_p DFT over the stride-M_p rows, then multiplies output i_p by
-// w_{M_{p-1}}^(i_p * j') (j' = row offset inside the block). Positions end up in
-// mixed-radix digit order; the last pass (M_P = 1) writes each output to its
-// natural row i_1 + L_1 i_2 + L_1 L_2 i_3 + ...
+// Coset form. With n = 2^ceil(log2 n_in) and E = N / n, output row i = t E + e
+// (t < n, e < E) is
+//   out[t E + e][c] = sum_{j < n} (in[j][c] * w_N^(e j)) * w_n^(t j),
+// i.e. a size-n DFT of the "twisted" n x (E C) matrix X[j][(e, c)] =
+// in[j][c] w_N^(e j), and the n x (E C) result is exactly the N x C output in
+// natural row-major order. The zero rows are never touched: a rate-1/2^r LDE
+// costs log2(n) butterfly stages, not log2(N), and the late WHIR rounds (a
+// 16-row input spread over 2^15 rows) become one short pass.
 //
-// Per workgroup: an L x Q tile (Q >= 32 consecutive words per row segment, so
-// every global access is a >= 128 B contiguous run) staged in LDS, radix-2 DIT
-// butterflies on bit-reversed LDS rows. Pass 1 skips loads of zero rows.
-// Twiddles: w_{2^24}^e = T_lo[e & 4095] * T_hi[e >> 12] (two 16 KB tables).
+// The size-n DFT is a four-step decomposition n = L_1 ... L_P (L_p <= 256,
+// P <= 3). Pass p works on blocks of M_{p-1} = L_p M_p rows: for block b and
+// every word offset w = m B + col (m < M_p, B = E C columns) it transforms the
+// L_p elements at rows j M_p + m (j < L_p) and multiplies output t by
+// w_{M_{p-1}}^(m t) (not in the final pass). Outputs end up in digit-reversed
+// block order; the final pass of a multi-pass transform stores row
+// t_1 + L_1 t_2 (+ L_1 L_2 t_3) directly.
+//
+// Per workgroup: QV "lines" (V consecutive columns of one (b, m)) x L rows.
+// L = R1 * R2: stage A, each thread loads R1 rows of its line straight from
+// HBM (consecutive lanes -> consecutive 16 B), runs an R1-point DFT in
+// registers and the inner twiddles w_L^(j_b t_a); one LDS exchange; stage B
+// runs R2-point DFTs in registers and stores straight to HBM. All index math is
+// 32-bit (a transform holds < 2^32 words).
 #pragma once
 #include "field.h"
 
 namespace kb {
 
-constexpr int NTT_THREADS = 256;
-constexpr int NTT_MAX_LDS_WORDS = 16384;  // 64 KiB tile
+// ---- compile-time Montgomery constants for the register DFTs ----
+constexpr uint32_t cx_redc(uint64_t x) {
+  const uint32_t m = (uint32_t)x * NPINV;
+  const uint32_t t = (uint32_t)((x + (uint64_t)m * P) >> 32);
+  return t >= P ? t - P : t;
+}
+constexpr uint32_t cx_mul(uint32_t a, uint32_t b) { return cx_redc((uint64_t)a * b); }
+constexpr uint32_t cx_pow(uint32_t a, uint64_t e) {
+  uint32_t r = ONE;
+  while (e) {
+    if (e & 1) r = cx_mul(r, a);
+    a = cx_mul(a, a);
+    e >>= 1;
+  }
+  return r;
+}
+struct Omega16 {
+  uint32_t w[16];
+};
+constexpr Omega16 make_omega16() {
+  Omega16 t{};
+  const uint32_t g3 = (uint32_t)(((uint64_t)3 << 32) % P);  // 3 in Montgomery form
+  const uint32_t w16 = cx_pow(g3, (uint64_t)(P - 1) >> 4);  // two_adic_generator(4)
+  uint32_t x = ONE;
+  for (int i = 0; i < 16; ++i) {
+    t.w[i] = x;
+    x = cx_mul(x, w16);
+  }
+  return t;
+}
+constexpr Omega16 kOmega16 = make_omega16();
 
-struct NttPass {
-  const uint32_t* src;
-  uint32_t* dst;
+template <int V>
+struct Vec {
+  uint32_t v[V];
+};
+
+template <int V>
+__device__ __forceinline__ Vec<V> vload(const uint32_t* p) {
+  Vec<V> r;
+  if constexpr (V == 4) {
+    const uint4 t = *reinterpret_cast<const uint4*>(p);
+    r.v[0] = t.x;
+    r.v[1] = t.y;
+    r.v[2] = t.z;
+    r.v[3] = t.w;
+  } else if constexpr (V == 2) {
+    const uint2 t = *reinterpret_cast<const uint2*>(p);
+    r.v[0] = t.x;
+    r.v[1] = t.y;
+  } else {
+    r.v[0] = *p;
+  }
+  return r;
+}
+template <int V>
+__device__ __forceinline__ void vstore(uint32_t* p, const Vec<V>& x) {
+  if constexpr (V == 4) *reinterpret_cast<uint4*>(p) = make_uint4(x.v[0], x.v[1], x.v[2], x.v[3]);
+  else if constexpr (V == 2) *reinterpret_cast<uint2*>(p) = make_uint2(x.v[0], x.v[1]);
+  else *p = x.v[0];
+}
+template <int V>
+__device__ __forceinline__ void vscale(Vec<V>& x, uint32_t w) {
+#pragma unroll
+  for (int c = 0; c < V; ++c) x.v[c] = mul(x.v[c], w);
+}
+
+constexpr int bitrev(int i, int bits) {
+  int r = 0;
+  for (int b = 0; b < bits; ++b) r |= ((i >> b) & 1) << (bits - 1 - b);
+  return r;
+}
+
+// x <- DFT_R(x) (natural order in and out, R = 2^LR <= 16): radix-2 DIT on the
+// bit-reversed registers; twiddles w_{2^(s+1)}^j are compile-time constants and
+// j = 0 multiplies are skipped (17 multiplies for R = 16).
+template <int LR, int V>
+__device__ __forceinline__ void dft_regs(Vec<V> (&x)[1 << LR]) {
+  constexpr int R = 1 << LR;
+  Vec<V> y[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) y[i] = x[bitrev(i, LR)];
+#pragma unroll
+  for (int s = 0; s < LR; ++s) {
+    const int half = 1 << s;
+#pragma unroll
+    for (int g = 0; g < R; g += 2 * half) {
+#pragma unroll
+      for (int j = 0; j < half; ++j) {
+        const int i0 = g + j, i1 = g + j + half;
+        const uint32_t w = kOmega16.w[j << (3 - s)];
+#pragma unroll
+        for (int c = 0; c < V; ++c) {
+          const uint32_t u = y[i0].v[c];
+          const uint32_t t = j == 0 ? y[i1].v[c] : mul(y[i1].v[c], w);
+          y[i0].v[c] = add(u, t);
+          y[i1].v[c] = sub(u, t);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < R; ++i) x[i] = y[i];
+}
+
+struct NttArgs {
+  const uint32_t* src;    // first pass: the n_in x C input; later: the n x B work array
+  uint32_t* dst;          // the n x B work array, or the output (final pass)
   const uint32_t* tw_lo;  // w24^i, i < 4096
   const uint32_t* tw_hi;  // w24^(4096 i), i < 4096
-  uint64_t n_in_rows;     // rows of src that may be non-zero (first pass)
-  uint32_t log_n, log_c;  // N = 2^log_n rows, C = 2^log_c columns
-  uint32_t log_l, log_m;  // this pass: L = 2^log_l, remaining M = 2^log_m
-  uint32_t log_q;         // tile width (words)
-  uint32_t first, last;
+  uint32_t n_in;          // first pass: input rows (rows >= n_in read as zero)
+  uint32_t log_c;         // first pass: input columns C (>= V)
+  uint32_t log_nn;        // first pass: log2 N (order of the coset twist)
+  uint32_t log_b;         // B = E C columns of the work array
+  uint32_t log_n;         // n rows of the work array
+  uint32_t log_m;         // M_p (this pass transforms L rows spaced M_p apart)
+  uint32_t log_qv;        // lines per workgroup
+  uint32_t first;         // read the input matrix and apply the coset twist
+  uint32_t twiddle_out;   // multiply output t by w_{M_{p-1}}^(m t) (not the final pass)
+  uint32_t natural;       // final pass of P >= 2: store rows in natural order
+  uint32_t log_l1, log_l2;  // L_1, L_2 (natural-order store)
   uint32_t n_pass;
-  uint32_t logl[4];  // all pass sizes (for the last pass's digit order)
 };
 
 __device__ __forceinline__ uint32_t tw24(const uint32_t* lo, const uint32_t* hi, uint32_t e) {
-  return mul(lo[e & 4095u], hi[e >> 12]);
+  return mul(lo[e & 4095u], hi[(e >> 12) & 4095u]);
 }
 
-template <int LOGL>
-__global__ void __launch_bounds__(NTT_THREADS) ntt_pass_kernel(NttPass a) {
+template <int LR1, int LR2, int V>
+__global__ void __launch_bounds__(256) ntt_kernel(NttArgs a) {
   extern __shared__ uint32_t lds[];
-  constexpr uint32_t L = 1u << LOGL;
-  const uint32_t Q = 1u << a.log_q;
-  const uint32_t C = 1u << a.log_c;
+  constexpr int R1 = 1 << LR1, R2 = 1 << LR2, LL = LR1 + LR2;
+  constexpr int LV = V == 4 ? 2 : V == 2 ? 1 : 0;
+  const uint32_t QV = 1u << a.log_qv;
   const uint32_t tid = threadIdx.x;
-  uint32_t* tile = lds;            // [L][Q]
-  uint32_t* twl = lds + (L << a.log_q);  // L/2 twiddles of order L
+  const uint32_t cq = tid & (QV - 1);
+  const uint32_t jb = tid >> a.log_qv;  // stage A: j_b; stage B: first t_a
+  const uint32_t log_lpb = a.log_m + a.log_b - LV;  // lines per block
+  const uint32_t line = (blockIdx.x << a.log_qv) + cq;
+  const uint32_t b = line >> log_lpb;
+  const uint32_t w = (line & ((1u << log_lpb) - 1)) << LV;  // m B + col
+  const uint32_t m = w >> a.log_b;
+  const uint32_t col = w & ((1u << a.log_b) - 1);
+  const uint32_t log_mprev = a.log_m + LL;
+  const uint32_t jstride = 1u << (a.log_m + a.log_b);
+  const uint32_t base = (b << (log_mprev + a.log_b)) + w;
 
-  // inner twiddles w_L^e
-  for (uint32_t e = tid; e < L / 2; e += NTT_THREADS) twl[e] = tw24(a.tw_lo, a.tw_hi, e << (24 - LOGL));
-
-  const uint64_t n_elems_tile = (uint64_t)L << a.log_q;
-  uint64_t blk_base = 0;       // first element of the block (non-last) / of the tile (last)
-  uint64_t q0 = 0;             // first q of the tile inside its block (non-last)
-  uint64_t out_row0 = 0;       // last pass: natural row of (t=0, i_P=0)
-  if (!a.last) {
-    // grid: (number of blocks) x (q tiles per block)
-    const uint64_t q_tiles = ((uint64_t)1 << (a.log_m + a.log_c)) >> a.log_q;
-    const uint64_t b = blockIdx.x / q_tiles;
-    q0 = (uint64_t)(blockIdx.x % q_tiles) << a.log_q;
-    blk_base = b << (a.log_l + a.log_m + a.log_c);
+  // ---- stage A: rows j = j_a R2 + j_b ----
+  Vec<V> x[R1];
+  if (!a.first) {
+#pragma unroll
+    for (int ja = 0; ja < R1; ++ja) x[ja] = vload<V>(a.src + base + ((uint32_t)ja * R2 + jb) * jstride);
   } else {
-    // tile = T consecutive values of the first digit i_1 (others fixed);
-    // Q = T * C words per transform row.
-    const uint32_t log_t = a.log_q - a.log_c;
-    const uint32_t log_l1 = a.n_pass > 1 ? a.logl[0] : 0;
-    const uint32_t t_tiles = (1u << log_l1) >> log_t;  // tiles over i_1
-    const uint64_t w = blockIdx.x;
-    const uint64_t i1_0 = (w % (t_tiles ? t_tiles : 1)) << log_t;
-    uint64_t rest = w / (t_tiles ? t_tiles : 1);  // digits i_2..i_{P-1}, i_2 least significant
-    // b_rest: the same digits with i_2 most significant
-    uint64_t b_rest = 0;
-    uint32_t log_s1 = 0;  // log of L_2 * ... * L_{P-1}
-    for (uint32_t d = 1; d + 1 < a.n_pass; ++d) log_s1 += a.logl[d];
-    {
-      uint64_t r = rest;
-      uint32_t shift = log_s1;
-      for (uint32_t d = 1; d + 1 < a.n_pass; ++d) {
-        const uint32_t ld = a.logl[d];
-        const uint64_t digit = r & (((uint64_t)1 << ld) - 1);
-        r >>= ld;
-        shift -= ld;
-        b_rest |= digit << shift;
+    // input row r = j M_1 + m (one block), coset e, input column c; twist w_N^(e r)
+    const uint32_t e = col >> a.log_c, c = col & ((1u << a.log_c) - 1);
+    uint32_t tw = ONE, st = ONE;
+    if (e) {
+      const uint32_t sh = 24 - a.log_nn;
+      tw = tw24(a.tw_lo, a.tw_hi, (e * ((jb << a.log_m) + m)) << sh);
+      st = tw24(a.tw_lo, a.tw_hi, (e << (LR2 + a.log_m)) << sh);
+    }
+#pragma unroll
+    for (int ja = 0; ja < R1; ++ja) {
+      const uint32_t r = ((((uint32_t)ja * R2 + jb)) << a.log_m) + m;
+      if (r < a.n_in) {
+        x[ja] = vload<V>(a.src + (r << a.log_c) + c);
+      } else {
+#pragma unroll
+        for (int q = 0; q < V; ++q) x[ja].v[q] = 0u;
+      }
+      if (e) {
+        vscale<V>(x[ja], tw);
+        tw = mul(tw, st);
       }
     }
-    const uint64_t b0 = (i1_0 << log_s1) + b_rest;  // block of t = 0; t advances by 2^log_s1
-    blk_base = b0;  // reused below as the block index
-    out_row0 = i1_0 + (rest << log_l1);
   }
+  dft_regs<LR1, V>(x);
+  if (R2 > 1 && jb) {  // inner twiddles w_L^(j_b t_a)
+    const uint32_t w1 = tw24(a.tw_lo, a.tw_hi, jb << (24 - LL));
+    uint32_t cur = w1;
+#pragma unroll
+    for (int ta = 1; ta < R1; ++ta) {
+      vscale<V>(x[ta], cur);
+      cur = mul(cur, w1);
+    }
+  }
+  Vec<V>* l4 = reinterpret_cast<Vec<V>*>(lds);
+#pragma unroll
+  for (int ta = 0; ta < R1; ++ta) l4[((uint32_t)ta * R2 + jb) * QV + cq] = x[ta];
   __syncthreads();
 
-  // ---- load (into bit-reversed rows) ----
-  for (uint64_t e = tid; e < n_elems_tile; e += NTT_THREADS) {
-    uint32_t jp, q;
-    uint64_t gidx;
-    bool zero = false;
-    if (!a.last) {
-      jp = (uint32_t)(e >> a.log_q);
-      q = (uint32_t)(e & (Q - 1));
-      const uint64_t qq = q0 + q;
-      gidx = blk_base + ((uint64_t)jp << (a.log_m + a.log_c)) + qq;
-      if (a.first) {
-        const uint64_t row = ((uint64_t)jp << a.log_m) + (qq >> a.log_c);
-        zero = row >= a.n_in_rows;
+  // ---- stage B: tasks t_a = j_b + k R2, outputs t = t_a + R1 t_b ----
+  uint32_t nat0 = 0;
+  if (a.natural) nat0 = a.n_pass == 2 ? b : (b >> a.log_l2) + ((b & ((1u << a.log_l2) - 1)) << a.log_l1);
+  const uint32_t log_nat_stride = a.n_pass == 2 ? a.log_l1 : a.log_l1 + a.log_l2;
+#pragma unroll
+  for (int k = 0; k < R1 / R2; ++k) {
+    const uint32_t ta = jb + (uint32_t)k * R2;
+    Vec<V> z[R2];
+#pragma unroll
+    for (int j = 0; j < R2; ++j) z[j] = l4[(ta * R2 + j) * QV + cq];
+    dft_regs<LR2, V>(z);
+    if (a.twiddle_out && m) {
+      const uint32_t sh = 24 - log_mprev;
+      uint32_t cur = tw24(a.tw_lo, a.tw_hi, (m * ta) << sh);
+      const uint32_t st = tw24(a.tw_lo, a.tw_hi, (m << LR1) << sh);
+#pragma unroll
+      for (int tb = 0; tb < R2; ++tb) {
+        vscale<V>(z[tb], cur);
+        cur = mul(cur, st);
       }
-    } else {
-      // contiguous walk over T blocks of L rows x C columns
-      const uint32_t col = (uint32_t)(e & (C - 1));
-      const uint64_t rowt = e >> a.log_c;         // t * L + jp
-      jp = (uint32_t)(rowt & (L - 1));
-      const uint64_t t = rowt >> LOGL;
-      q = (uint32_t)((t << a.log_c) + col);
-      uint32_t log_s1 = 0;
-      for (uint32_t d = 1; d + 1 < a.n_pass; ++d) log_s1 += a.logl[d];
-      const uint64_t b = blk_base + (t << log_s1);
-      const uint64_t row = (b << LOGL) + jp;
-      gidx = (row << a.log_c) + col;
-      if (a.first) zero = row >= a.n_in_rows;
     }
-    const uint32_t v = zero ? 0u : a.src[gidx];
-    const uint32_t jr = __brev(jp) >> (32 - LOGL);
-    tile[(jr << a.log_q) + q] = v;
-  }
-  __syncthreads();
-
-  // ---- radix-2 DIT stages ----
-  const uint32_t n_bfly = (L / 2) << a.log_q;
-#pragma unroll 1
-  for (int s = 0; s < LOGL; ++s) {
-    const uint32_t half = 1u << s;
-    for (uint32_t e = tid; e < n_bfly; e += NTT_THREADS) {
-      const uint32_t q = e & (Q - 1);
-      const uint32_t bi = e >> a.log_q;              // butterfly index in [0, L/2)
-      const uint32_t j = bi & (half - 1);
-      const uint32_t g = bi >> s;
-      const uint32_t i0 = (g << (s + 1)) + j;
-      const uint32_t i1 = i0 + half;
-      const uint32_t w = twl[j << (LOGL - 1 - s)];
-      const uint32_t u = tile[(i0 << a.log_q) + q];
-      const uint32_t v = mul(tile[(i1 << a.log_q) + q], w);
-      tile[(i0 << a.log_q) + q] = add(u, v);
-      tile[(i1 << a.log_q) + q] = sub(u, v);
-    }
-    __syncthreads();
-  }
-
-  // ---- twiddle + store ----
-  for (uint64_t e = tid; e < n_elems_tile; e += NTT_THREADS) {
-    if (!a.last) {
-      const uint32_t ip = (uint32_t)(e >> a.log_q);
-      const uint32_t q = (uint32_t)(e & (Q - 1));
-      const uint64_t qq = q0 + q;
-      uint32_t v = tile[(ip << a.log_q) + q];
-      // w_{M_{p-1}}^(ip * j'), j' = qq >> log_c; exponent in the order-2^24 group
-      const uint64_t jprime = qq >> a.log_c;
-      const uint32_t log_mprev = a.log_l + a.log_m;
-      const uint64_t ex = ((uint64_t)ip * jprime) & (((uint64_t)1 << log_mprev) - 1);
-      if (ex) v = mul(v, tw24(a.tw_lo, a.tw_hi, (uint32_t)(ex << (24 - log_mprev))));
-      a.dst[blk_base + ((uint64_t)ip << (a.log_m + a.log_c)) + qq] = v;
-    } else {
-      // walk outputs so that consecutive threads write consecutive rows:
-      // e = (ip * T + t) * C + col
-      const uint32_t col = (uint32_t)(e & (C - 1));
-      const uint64_t r = e >> a.log_c;
-      const uint32_t log_t = a.log_q - a.log_c;
-      const uint64_t t = r & (((uint64_t)1 << log_t) - 1);
-      const uint32_t ip = (uint32_t)(r >> log_t);
-      const uint32_t q = (uint32_t)((t << a.log_c) + col);
-      const uint32_t v = tile[(ip << a.log_q) + q];
-      const uint64_t row = out_row0 + t + ((uint64_t)ip << (a.log_n - LOGL));
-      a.dst[(row << a.log_c) + col] = v;
+#pragma unroll
+    for (int tb = 0; tb < R2; ++tb) {
+      const uint32_t t = ta + (uint32_t)tb * R1;
+      uint32_t* p;
+      if (a.natural) p = a.dst + (((nat0 + (t << log_nat_stride))) << a.log_b) + col;
+      else p = a.dst + base + t * jstride;
+      vstore<V>(p, z[tb]);
     }
   }
 }

@@ -124,89 +124,91 @@ static hipStream_t pick_stream(whir_ctx* c, void* s) { return s ? (hipStream_t)s
 
 // ============================ DFT ============================
 
+// Plan of one coset-form transform: N = 2^log_nn output rows of C = 2^log_c
+// columns from n_in input rows (n = 2^log_n >= n_in rows of the work array,
+// B = 2^log_b = (N / n) C columns), P passes of L_p = 2^logl[p] <= 256.
 struct NttPlan {
-  int P = 0;
-  int logl[4] = {0, 0, 0, 0};
-  int logq[4] = {0, 0, 0, 0};
+  int log_nn = 0, log_c = 0, log_n = 0, log_b = 0, lv = 0, P = 0;
+  int logl[3] = {0, 0, 0};
 };
 
-static bool ntt_plan(int log_n, int log_c, NttPlan& pl) {
-  constexpr int LOG_TILE = 13;  // 8192 words (32 KiB) per workgroup tile
-  if (log_n < 1 || log_n > 24 || log_c > 10) return false;
-  const int logq_pref = std::max(5, log_c);
-  const int max_logl = std::max(1, LOG_TILE - logq_pref);
-  int P = (log_n + max_logl - 1) / max_logl;
-  if (P > 4) return false;
-  pl.P = P;
+constexpr int NTT_MAX_LOGL = 8;
+
+static bool ntt_plan(uint64_t n_in, int log_nn, int log_c, NttPlan& pl) {
+  if (log_nn < 1 || log_nn > 24 || log_c < 0 || log_c > 10 || log_nn + log_c > 31) return false;
+  if (n_in > (1ull << log_nn)) return false;
+  int log_n = 1;
+  while ((1ull << log_n) < n_in) ++log_n;
+  if (log_n > log_nn) return false;
+  pl.log_nn = log_nn;
+  pl.log_c = log_c;
+  pl.log_n = log_n;
+  pl.log_b = log_c + (log_nn - log_n);
+  pl.lv = std::min(2, log_c);
+  pl.P = (log_n + NTT_MAX_LOGL - 1) / NTT_MAX_LOGL;
+  if (pl.P > 3) return false;
   int rem = log_n;
-  for (int p = 0; p < P; ++p) {
-    const int take = (rem + (P - p) - 1) / (P - p);
-    pl.logl[p] = take;
-    rem -= take;
-  }
-  int used = 0;
-  for (int p = 0; p < P; ++p) {
-    used += pl.logl[p];
-    const int log_m = log_n - used;
-    if (p + 1 < P) {
-      pl.logq[p] = std::min(logq_pref, log_m + log_c);
-    } else {
-      const int log_l1 = P > 1 ? pl.logl[0] : 0;
-      const int log_t = std::min(std::max(0, logq_pref - log_c), log_l1);
-      pl.logq[p] = log_c + log_t;
-    }
-    if (pl.logl[p] + pl.logq[p] > LOG_TILE + 1) return false;
+  for (int p = 0; p < pl.P; ++p) {
+    pl.logl[p] = (rem + (pl.P - p) - 1) / (pl.P - p);
+    rem -= pl.logl[p];
   }
   return true;
 }
 
-static int launch_ntt_pass(int logl, dim3 grid, size_t lds, hipStream_t s, const NttPass& a) {
+typedef void (*NttKernelFn)(NttArgs);
+
+template <int V>
+static NttKernelFn ntt_kernel_for(int logl) {
   switch (logl) {
-#define NTT_CASE(L) \
-  case L: hipLaunchKernelGGL(ntt_pass_kernel<L>, grid, dim3(NTT_THREADS), lds, s, a); break;
-    NTT_CASE(1) NTT_CASE(2) NTT_CASE(3) NTT_CASE(4) NTT_CASE(5) NTT_CASE(6) NTT_CASE(7) NTT_CASE(8)
-    NTT_CASE(9) NTT_CASE(10) NTT_CASE(11) NTT_CASE(12)
-#undef NTT_CASE
-    default: return WHIR_EINVAL;
+    case 1: return ntt_kernel<1, 0, V>;
+    case 2: return ntt_kernel<1, 1, V>;
+    case 3: return ntt_kernel<2, 1, V>;
+    case 4: return ntt_kernel<2, 2, V>;
+    case 5: return ntt_kernel<3, 2, V>;
+    case 6: return ntt_kernel<3, 3, V>;
+    case 7: return ntt_kernel<4, 3, V>;
+    case 8: return ntt_kernel<4, 4, V>;
+    default: return nullptr;
   }
-  return WHIR_OK;
 }
 
-// out = DFT of the n_in-row matrix `in` zero-padded to 2^log_n rows; `tmp` has
-// room for 2^(log_n + log_c) words (unused for single-pass sizes).
-static int ntt_enqueue(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_n, int log_c, uint32_t* out,
+// out (N x C, natural rows) = DFT of the n_in-row matrix `in` zero-padded to
+// N = 2^log_nn rows; `tmp` has room for N * C words (used when P >= 2).
+static int ntt_enqueue(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_nn, int log_c, uint32_t* out,
                        uint32_t* tmp, hipStream_t s) {
   NttPlan pl;
-  if (!ntt_plan(log_n, log_c, pl)) return WHIR_EINVAL;
+  if (!ntt_plan(n_in, log_nn, log_c, pl)) return WHIR_EINVAL;
   int used = 0;
   for (int p = 0; p < pl.P; ++p) {
-    NttPass a;
+    const int l = pl.logl[p];
+    const int lr2 = l / 2;
+    used += l;
+    NttArgs a;
     memset(&a, 0, sizeof(a));
     a.tw_lo = c->tw_lo();
     a.tw_hi = c->tw_hi();
-    a.n_in_rows = n_in;
-    a.log_n = log_n;
-    a.log_c = log_c;
-    a.log_l = pl.logl[p];
-    used += pl.logl[p];
-    a.log_m = log_n - used;
-    a.log_q = pl.logq[p];
+    a.n_in = (uint32_t)n_in;
+    a.log_c = (uint32_t)log_c;
+    a.log_nn = (uint32_t)log_nn;
+    a.log_b = (uint32_t)pl.log_b;
+    a.log_n = (uint32_t)pl.log_n;
+    a.log_m = (uint32_t)(pl.log_n - used);
+    const int log_lines = pl.log_n + pl.log_b - pl.lv - l;
+    a.log_qv = (uint32_t)std::min(8 - lr2, log_lines);
     a.first = p == 0;
-    a.last = p + 1 == pl.P;
-    a.n_pass = pl.P;
-    for (int q = 0; q < 4; ++q) a.logl[q] = pl.logl[q];
+    a.twiddle_out = p + 1 < pl.P;
+    a.natural = pl.P > 1 && p + 1 == pl.P;
+    a.log_l1 = (uint32_t)pl.logl[0];
+    a.log_l2 = (uint32_t)pl.logl[1];
+    a.n_pass = (uint32_t)pl.P;
     a.src = p == 0 ? in : tmp;
-    a.dst = a.last ? out : tmp;
-    uint64_t grid;
-    if (!a.last) {
-      grid = (1ull << (log_n - a.log_l - a.log_m)) * ((1ull << (a.log_m + log_c)) >> a.log_q);
-    } else {
-      const int log_t = a.log_q - log_c;
-      grid = 1ull << (log_n - log_t - a.log_l);
-    }
-    const size_t lds = ((size_t)1 << (a.log_l + a.log_q)) * 4 + ((size_t)1 << a.log_l) * 2;
-    int rc = launch_ntt_pass(a.log_l, dim3((unsigned)grid), lds, s, a);
-    if (rc) return rc;
+    a.dst = p + 1 == pl.P ? out : tmp;
+    const unsigned grid = 1u << (log_lines - (int)a.log_qv);
+    const unsigned threads = 1u << (lr2 + (int)a.log_qv);
+    const size_t lds = ((size_t)1 << (l + (int)a.log_qv + pl.lv)) * 4;
+    NttKernelFn fn = pl.lv == 2 ? ntt_kernel_for<4>(l) : pl.lv == 1 ? ntt_kernel_for<2>(l) : ntt_kernel_for<1>(l);
+    if (!fn) return WHIR_EINVAL;
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(threads), lds, s, a);
   }
   return WHIR_OK;
 }
@@ -220,7 +222,7 @@ extern "C" int whir_dft_batch(whir_ctx* c, const uint32_t* d_in, uint64_t n_in_r
   hipStream_t s = pick_stream(c, stream);
   uint32_t* tmp = nullptr;
   NttPlan pl;
-  if (!ntt_plan((int)log_h, log_c, pl)) return WHIR_EINVAL;
+  if (!ntt_plan(n_in_rows, (int)log_h, log_c, pl)) return WHIR_EINVAL;
   if (pl.P > 1) HIP_TRY(hipMallocAsync((void**)&tmp, ((size_t)width << log_h) * 4, s));
   int rc = ntt_enqueue(c, d_in, n_in_rows, (int)log_h, log_c, d_out, tmp, s);
   if (tmp) HIP_TRY(hipFreeAsync(tmp, s));
