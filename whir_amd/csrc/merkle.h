@@ -71,27 +71,40 @@ __global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_kernel(const uint
 // pairing the nodes of a layer that are congruent to w (mod G) are exactly the
 // subtree below node w of the G-node layer, so workgroup w builds every layer
 // from n0/2 nodes down to G nodes for its residue class without a grid-wide
-// barrier; launch with G = 1 and one workgroup for the final layers.
+// barrier: its n0/G nodes stay in LDS from layer to layer (each layer is also
+// written out for the openings). Launch with G = 1 and one workgroup for the
+// final layers.
 constexpr int COOP_THREADS = 1024;  // 64 compressions in flight per workgroup
+constexpr uint32_t COOP_MAX_NODES = 256;  // n0 / G
 
-__global__ void __launch_bounds__(COOP_THREADS) merkle_coop_subtrees_kernel(uint32_t* layer, uint64_t n0, uint32_t G) {
+__global__ void __launch_bounds__(COOP_THREADS) merkle_coop_subtrees_kernel(uint32_t* layer, uint32_t n0, uint32_t G) {
+  __shared__ uint32_t s_node[COOP_MAX_NODES * 8];
   const uint32_t w = blockIdx.x;
   const uint32_t slot = threadIdx.x >> 4, li = threadIdx.x & 15;
-  uint32_t* cur = layer;
-  uint64_t n = n0;
-  while (n > G) {
-    const uint64_t half = n >> 1;
-    uint32_t* next = cur + n * 8;
-    const uint64_t cnt = half / G;
-    for (uint64_t t = slot; t < cnt; t += COOP_THREADS / 16) {
-      const uint64_t j = w + t * G;
-      uint32_t v = li < 8 ? cur[j * 8 + li] : cur[(j + half) * 8 + (li - 8)];
-      permute16_coop(v);
-      if (li < 8) next[j * 8 + li] = v;
+  CoopP16 K;
+  K.load();
+  uint32_t kk = n0 / G;  // nodes of this residue class in the current layer
+  for (uint32_t e = threadIdx.x; e < kk * 8; e += COOP_THREADS)
+    s_node[e] = layer[(uint64_t)(w + (e >> 3) * G) * 8 + (e & 7)];
+  __syncthreads();
+  uint32_t* next = layer + (uint64_t)n0 * 8;
+  uint32_t n = n0;
+  while (kk > 1) {
+    const uint32_t h = kk >> 1;
+    // compression t reads nodes t and t + h and overwrites node t: no other
+    // compression of this layer touches either, so the update is in place
+    for (uint32_t t = slot; t < h; t += COOP_THREADS / 16) {
+      uint32_t v = li < 8 ? s_node[t * 8 + li] : s_node[(t + h) * 8 + (li - 8)];
+      permute16_coop(v, K);
+      if (li < 8) {
+        s_node[t * 8 + li] = v;
+        next[(uint64_t)(w + t * G) * 8 + li] = v;
+      }
     }
     __syncthreads();
-    cur = next;
-    n = half;
+    n >>= 1;
+    next += (uint64_t)n * 8;
+    kk = h;
   }
 }
 }  // namespace kb

@@ -32,6 +32,8 @@
 
 namespace kb {
 
+// The plain round constants (Montgomery words, the reference's draw order);
+// host side only: the kernels use the derived Poseidon2Lazy tables.
 struct Poseidon2Consts {
   uint32_t ext16_init[4][16];
   uint32_t ext16_term[4][16];
@@ -40,10 +42,6 @@ struct Poseidon2Consts {
   uint32_t ext24_term[4][24];
   uint32_t int24[23];
 };
-
-__constant__ Poseidon2Consts c_p2;
-
-__device__ __forceinline__ uint32_t cube(uint32_t x) { return mul(sqr(x), x); }
 
 // Per-round constants of the lazily reduced permutation of width W (values in
 // [0, p)).

@@ -5,6 +5,7 @@
 // eq0 (low m0 bits) and eq1 (high bits): eq[x] = eq0[x & (2^m0-1)] * eq1[x >> m0].
 #pragma once
 #include "field.h"
+#include "poseidon2.h"
 
 namespace kb {
 
@@ -151,23 +152,55 @@ __global__ void __launch_bounds__(SC_THREADS) expr_build_kernel(const uint32_t* 
 // ---- claim combination (combine_claims, combine.rs:37-64,178-203,407-468) ----
 // w[x] += sum_c eq0_c[x_lo] * eq1s_c[x_hi]  +  sum_j powR[j][x_hi] * powL[j][x_lo]
 // eq1s_c carries alpha^c; powL[j][e] = var_j^e (e < 2^m0), powR[j][e] = scale_j * var_j^(e*2^m0).
+// COMBINE_SPLIT adjacent lanes share an output and split the terms; the pow
+// terms are Montgomery products taken two per REDC (2 p^2 < p 2^32) and summed
+// unreduced in 64 bits, so a term costs ~5 VALU instructions instead of an
+// extension-by-base multiply plus modular adds.
+constexpr int COMBINE_SPLIT = 4;
+constexpr uint32_t R2MOD = (uint32_t)(((uint64_t)ONE * ONE) % P);  // 2^64 mod p
+
 __global__ void __launch_bounds__(SC_THREADS) combine_kernel(Ext* __restrict__ w, uint64_t n, uint32_t m0,
                                                          const Ext* __restrict__ eq_pool, uint32_t n_eq,
                                                          uint64_t eq_stride, const uint32_t* __restrict__ powL,
                                                          const Ext* __restrict__ powR, const uint32_t* n_pow_ptr,
                                                          uint32_t m1) {
-  const uint64_t x = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
-  if (x >= n) return;
+  constexpr uint32_t S = COMBINE_SPLIT;
+  const uint64_t gid = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
+  const uint64_t x = gid / S;
+  const uint32_t sub = (uint32_t)(gid % S);
+  const bool valid = x < n;  // every lane takes part in the reduction below
+  const uint64_t xx = valid ? x : 0;
   const uint32_t n_pow = n_pow_ptr ? *n_pow_ptr : 0u;
   const uint64_t n0 = 1ull << m0, n1 = 1ull << m1;
-  const uint64_t lo = x & (n0 - 1), hi = x >> m0;
-  Ext acc = ld_ext(w + x);
-  for (uint32_t c = 0; c < n_eq; ++c) {
-    const Ext* base = eq_pool + c * eq_stride;
-    acc = eadd(acc, emul(ld_ext(base + lo), ld_ext(base + n0 + hi)));
+  const uint64_t lo = xx & (n0 - 1), hi = xx >> m0;
+  uint64_t a[4] = {0, 0, 0, 0};
+  uint32_t j = sub;
+#pragma unroll 2
+  for (; j + S < n_pow; j += 2 * S) {
+    const uint32_t l0 = powL[(uint64_t)j * n0 + lo], l1 = powL[(uint64_t)(j + S) * n0 + lo];
+    const Ext r0 = ld_ext(powR + (uint64_t)j * n1 + hi), r1 = ld_ext(powR + (uint64_t)(j + S) * n1 + hi);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] += redc(mad64(l0, r0.c[c], (uint64_t)l1 * r1.c[c]));
   }
-  for (uint32_t j = 0; j < n_pow; ++j) acc = eadd(acc, emul_base(ld_ext(powR + j * n1 + hi), powL[j * n0 + lo]));
-  st_ext(w + x, acc);
+  if (j < n_pow) {
+    const uint32_t l0 = powL[(uint64_t)j * n0 + lo];
+    const Ext r0 = ld_ext(powR + (uint64_t)j * n1 + hi);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] += redc((uint64_t)l0 * r0.c[c]);
+  }
+  Ext part;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) part.c[c] = canon2(mulr(redc(a[c]), R2MOD));  // a mod p
+  for (uint32_t c = sub; c < n_eq; c += S) {
+    const Ext* base = eq_pool + c * eq_stride;
+    part = eadd(part, emul(ld_ext(base + lo), ld_ext(base + n0 + hi)));
+  }
+#pragma unroll
+  for (uint32_t m = 1; m < S; m <<= 1) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) part.c[c] = add(part.c[c], (uint32_t)__shfl_xor((int)part.c[c], (int)m));
+  }
+  if (valid && sub == 0) st_ext(w + x, eadd(ld_ext(w + x), part));
 }
 
 // powL[j][e] = var_j^e for e < 2^m0; powR[j][e] = scale_j * var_j^(e * 2^m0) for e < 2^m1.

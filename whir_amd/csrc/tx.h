@@ -25,8 +25,6 @@ struct DevTx {
   uint32_t pad[3];
 };
 
-__constant__ uint32_t c_diag16[16];  // Montgomery form of V16 (cooperative P16)
-
 // Cross-lane moves inside each 16-lane row via DPP (VALU latency, no LDS):
 //   quad_perm(p0,p1,p2,p3): lane i of each quad reads lane p_i of that quad;
 //   row_ror:n             : lane i of each 16-lane row reads lane (i - n) mod 16.
@@ -34,53 +32,85 @@ template <int CTRL>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
 }
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v) {
+  const uint32_t lo = dpp<CTRL>((uint32_t)v), hi = dpp<CTRL>((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
 constexpr int DPP_QUAD_NEXT = 1 | (2 << 2) | (3 << 4) | (0 << 6);  // lane a <- a+1 (mod 4)
 constexpr int DPP_QUAD_XOR1 = 1 | (0 << 2) | (3 << 4) | (2 << 6);
 constexpr int DPP_QUAD_XOR2 = 2 | (3 << 2) | (0 << 4) | (1 << 6);
 constexpr int DPP_ROR(int n) { return 0x120 + n; }
 
-// Cooperative Poseidon2-16: lane i (mod 16) holds state word i.
-// MDS-light per lane: y_a = S4 + x_a + 2 x_{a+1} within the quad (= M4 row a),
-// then add the sum of y over the lanes with the same index mod 4.
-__device__ __forceinline__ void mds_light16_coop(uint32_t& x) {
+// Cooperative Poseidon2-16: lane i (mod 16) holds state word i, with the
+// lazy reduction and scaled state of permute_lazy (poseidon2.h). The per-lane
+// constants are loaded into registers once per kernel (CoopP16::load), so a
+// permutation reads no memory.
+struct CoopP16 {
+  uint32_t la[9];   // lin_add[k][lane]
+  uint32_t ia[20];  // int_add[r] on lane 0, 0 elsewhere
+  uint32_t d;       // V_lane as an integer; 2 on lane 0 (applied to 2p - x0)
+  uint32_t kappa[20];
+  uint32_t fix;
+
+  __device__ __forceinline__ void load() {
+    const Poseidon2Lazy<16>& C = c_p2l.w16;
+    const uint32_t i = threadIdx.x & 15;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) la[k] = C.lin_add[k][i];
+#pragma unroll
+    for (int r = 1; r < 20; ++r) ia[r] = i == 0 ? C.int_add[r] : 0u;
+    ia[0] = 0;
+    uint32_t dv = 2;
+#pragma unroll
+    for (int j = 1; j < 16; ++j) dv = i == (uint32_t)j ? Poseidon2Lazy<16>::diag(j) : dv;
+    d = dv;
+#pragma unroll
+    for (int r = 0; r < 20; ++r) kappa[r] = C.kappa[r];
+    fix = C.fix;
+  }
+};
+
+// REDC(MDS-light(x) + add) across the 16 lanes of a row; x < 2p, result
+// < p + 2^7. Per chunk (quad) row a of M4 is S4 + x_a + 2 x_{a+1}; then every
+// lane adds the sum over the four chunks at its position (rotations by 4, 8).
+__device__ __forceinline__ uint32_t mds16_coop_lazy(uint32_t x, uint32_t add) {
   const uint32_t xn = dpp<DPP_QUAD_NEXT>(x);
-  const uint32_t t = add(x, dpp<DPP_QUAD_XOR1>(x));
-  const uint32_t s4 = add(t, dpp<DPP_QUAD_XOR2>(t));
-  const uint32_t y = add(add(s4, x), dbl(xn));
-  const uint32_t u = add(y, dpp<DPP_ROR(4)>(y));
-  const uint32_t cs = add(u, dpp<DPP_ROR(8)>(u));
-  x = add(y, cs);
+  const uint64_t s2 = (uint64_t)x + dpp<DPP_QUAD_XOR1>(x);
+  const uint64_t s4 = s2 + dpp64<DPP_QUAD_XOR2>(s2);
+  const uint64_t y = mad_k<2>(xn, mad_k<1>(x, s4));
+  const uint64_t u = y + dpp64<DPP_ROR(4)>(y);
+  const uint64_t cs = u + dpp64<DPP_ROR(8)>(u);
+  return redc(mad_k<1>(add, y + cs));
 }
 
-__device__ __forceinline__ uint32_t row_sum16(uint32_t v) {
-  v = add(v, dpp<DPP_ROR(8)>(v));
-  v = add(v, dpp<DPP_ROR(4)>(v));
-  v = add(v, dpp<DPP_ROR(2)>(v));
-  v = add(v, dpp<DPP_ROR(1)>(v));
+__device__ __forceinline__ uint64_t row_sum16_64(uint32_t x) {
+  uint64_t v = (uint64_t)x + dpp<DPP_ROR(8)>(x);
+  v += dpp64<DPP_ROR(4)>(v);
+  v += dpp64<DPP_ROR(2)>(v);
+  v += dpp64<DPP_ROR(1)>(v);
   return v;
 }
 
-__device__ __forceinline__ void permute16_coop(uint32_t& s) {
-  const int i = threadIdx.x & 15;
-  mds_light16_coop(s);
+// In: s = the lane's state word in [0, p). Out: the permuted word in [0, p).
+__device__ __forceinline__ void permute16_coop(uint32_t& s, const CoopP16& K) {
+  const bool lane0 = (threadIdx.x & 15) == 0;
+  uint32_t x = mds16_coop_lazy(s, K.la[0]);
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    s = cube(add(s, c_p2.ext16_init[r][i]));
-    mds_light16_coop(s);
-  }
-  const uint32_t d = c_diag16[i];
+  for (int r = 1; r < 5; ++r) x = mds16_coop_lazy(mulr(mulr(x, x), x), K.la[r]);
 #pragma unroll
   for (int r = 0; r < 20; ++r) {
-    const uint32_t t = cube(add(s, c_p2.int16[r]));
-    s = i == 0 ? t : s;
-    const uint32_t sum = row_sum16(s);
-    s = add(mul(s, d), sum);
+    const uint32_t c = mulr(mulr(mulr(x, x), x), K.kappa[r]);
+    x = lane0 ? c : x;
+    const uint64_t sum = row_sum16_64(x);
+    const uint32_t y = lane0 ? 2u * P - x : x;
+    const uint64_t t = mad_k<1>(r < 19 ? K.ia[r + 1] : K.la[5], mad64(y, K.d, sum));
+    x = r < 19 ? redc(t) : canon2(redc(t));
   }
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    s = cube(add(s, c_p2.ext16_term[r][i]));
-    mds_light16_coop(s);
-  }
+  for (int r = 6; r < 9; ++r) x = mds16_coop_lazy(mulr(mulr(x, x), x), K.la[r]);
+  x = mds16_coop_lazy(mulr(mulr(x, x), x), 0u);
+  s = canon2(mulr(x, K.fix));
 }
 
 // One wavefront runs the transcript in uniform control flow. The sponge state
@@ -95,6 +125,7 @@ struct WaveTx {
   uint32_t in_len, out_len;
   uint32_t log_words, log_cap, overflow;
   uint32_t* log;
+  CoopP16 k;
 
   __device__ __forceinline__ void load(const DevTx* t, uint32_t* log_ptr) {
     const int lane = threadIdx.x & 63;
@@ -106,6 +137,7 @@ struct WaveTx {
     log_cap = t->log_cap;
     overflow = t->overflow;
     log = log_ptr;
+    k.load();
   }
   __device__ __forceinline__ void store(DevTx* t) {
     const int lane = threadIdx.x & 63;
@@ -126,7 +158,7 @@ struct WaveTx {
     const uint32_t i = threadIdx.x & 15;
     if (i < in_len) s = pin;
     in_len = 0;
-    permute16_coop(s);
+    permute16_coop(s, k);
     out_len = 8;
   }
   __device__ __forceinline__ void observe(uint32_t v) {

@@ -61,24 +61,12 @@ static int upload_constants(uint64_t seed) {
       for (int j = 0; j < 24; ++j) c.ext24_term[i][j] = r.next_koalabear();
     for (int i = 0; i < 23; ++i) c.int24[i] = r.next_koalabear();
   }
-  HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_p2), &c, sizeof(c)));
-  // V16 (canonical sign * num / 2^sh), Montgomery form for the cooperative P16.
-  static const int sgn[16] = {-1, 1, 1, 1, 1, 1, -1, -1, -1, 1, 1, 1, -1, -1, -1, -1};
-  static const uint32_t num[16] = {2, 1, 2, 1, 3, 4, 1, 3, 4, 1, 1, 1, 1, 1, 1, 1};
-  static const int sh[16] = {0, 0, 0, 1, 0, 0, 1, 0, 0, 8, 3, 24, 8, 3, 4, 24};
   {
     static Poseidon2LazyConsts lz;
     lazy_consts_build<16>(c.ext16_init, c.ext16_term, c.int16, lz.w16);
     lazy_consts_build<24>(c.ext24_init, c.ext24_term, c.int24, lz.w24);
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_p2l), &lz, sizeof(lz)));
   }
-  uint32_t d16[16];
-  const uint32_t half = kb::HALF;
-  for (int i = 0; i < 16; ++i) {
-    uint32_t v = kb::mul(kb::from_canonical(num[i]), kb::pow(half, (uint64_t)sh[i]));
-    d16[i] = sgn[i] < 0 ? kb::neg(v) : v;
-  }
-  HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_diag16), d16, sizeof(d16)));
   return WHIR_OK;
 }
 
@@ -249,7 +237,7 @@ static int leaves_enqueue(const uint32_t* rows, uint64_t n_rows, uint32_t row_wo
 // Wide layers use one compression per lane (throughput-bound); from
 // MERKLE_COOP_FROM nodes down, cooperative subtrees (latency-bound part).
 constexpr uint64_t MERKLE_COOP_FROM = 1ull << 15;
-constexpr uint32_t MERKLE_COOP_GROUPS = 256;
+constexpr uint64_t MERKLE_COOP_STAGE1 = 128;  // nodes per workgroup in the first cooperative stage
 
 static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, uint32_t* layers, hipStream_t s,
                           hipEvent_t* leaf_ev = nullptr) {
@@ -268,16 +256,17 @@ static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, u
     n = half;
   }
   if (n > 1) {
-    uint32_t G = (uint32_t)std::min<uint64_t>(MERKLE_COOP_GROUPS, n / 64);
-    if (G >= 2) {
-      hipLaunchKernelGGL(merkle_coop_subtrees_kernel, dim3(G), dim3(COOP_THREADS), 0, s, cur, n, G);
-      // advance to the G-node layer
-      while (n > G) {
+    // first stage: residue classes of MERKLE_COOP_STAGE1 nodes; then one
+    // workgroup for the last <= COOP_MAX_NODES nodes
+    if (n > COOP_MAX_NODES) {
+      const uint32_t G = (uint32_t)(n / MERKLE_COOP_STAGE1);
+      hipLaunchKernelGGL(merkle_coop_subtrees_kernel, dim3(G), dim3(COOP_THREADS), 0, s, cur, (uint32_t)n, G);
+      while (n > G) {  // advance to the G-node layer
         cur += n * 8;
         n /= 2;
       }
     }
-    hipLaunchKernelGGL(merkle_coop_subtrees_kernel, dim3(1), dim3(COOP_THREADS), 0, s, cur, n, 1u);
+    hipLaunchKernelGGL(merkle_coop_subtrees_kernel, dim3(1), dim3(COOP_THREADS), 0, s, cur, (uint32_t)n, 1u);
   }
   return WHIR_OK;
 }
@@ -314,7 +303,9 @@ __global__ void permute16_coop_kernel(uint32_t* st, uint64_t n) {
   const uint64_t i = blockIdx.x;
   if (i >= n) return;
   uint32_t s = st[i * 16 + (threadIdx.x & 15)];
-  permute16_coop(s);
+  CoopP16 K;
+  K.load();
+  permute16_coop(s, K);
   if (threadIdx.x < 16) st[i * 16 + threadIdx.x] = s;
 }
 
@@ -652,7 +643,7 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
     if (r > 0)
       hipLaunchKernelGGL(pow_tables_kernel, dim3(blocks_for((uint64_t)rp.n_stir * ((1ull << om0) + (1ull << om1)))),
                          dim3(SC_THREADS), 0, s, P->qa.var, P->qa.scale, P->stir_count, om0, om1, P->powL, P->powR);
-    hipLaunchKernelGGL(combine_kernel, dim3(blocks_for(n)), dim3(SC_THREADS), 0, s, P->ew[cur], n, om0, P->ood_pool,
+    hipLaunchKernelGGL(combine_kernel, dim3(blocks_for(n * COMBINE_SPLIT)), dim3(SC_THREADS), 0, s, P->ew[cur], n, om0, P->ood_pool,
                        (uint32_t)rp.n_ood, P->ood_stride, P->powL, P->powR, r > 0 ? P->stir_count : nullptr, om1);
     for (int t = 0; t < f; ++t) {
       unsigned g;
@@ -681,7 +672,7 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
       const uint32_t pm0 = mm / 2, pm1 = mm - mm / 2;
       hipLaunchKernelGGL(pow_tables_kernel, dim3(blocks_for((uint64_t)rp.n_stir * ((1ull << pm0) + (1ull << pm1)))),
                          dim3(SC_THREADS), 0, s, P->qa.ws, P->qa.scale, P->stir_count, pm0, pm1, P->powL, P->powR);
-      hipLaunchKernelGGL(combine_kernel, dim3(blocks_for(n)), dim3(SC_THREADS), 0, s, P->ew[cur], n, pm0,
+      hipLaunchKernelGGL(combine_kernel, dim3(blocks_for(n * COMBINE_SPLIT)), dim3(SC_THREADS), 0, s, P->ew[cur], n, pm0,
                          (const Ext*)nullptr, 0u, (uint64_t)0, P->powL, P->powR, P->stir_count, pm1);
     }
   }
