@@ -332,6 +332,21 @@ def main():
 
     leaf_ms, leaf_n, perms = prover.timer(prover.TIMER_LEAVES_BASE)
     dft_ms, dft_n, dft_words = prover.timer(prover.TIMER_DFT_BASE)
+
+    # the same proof as one captured hipGraph per launch (no per-kernel timers)
+    prover.set_timers(False)
+    prover.set_graph(True)
+    for _ in range(2):
+        prover.enqueue()
+        gproof = prover.result()[0]
+    whir_amd.lib().whir_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        prover.enqueue()
+        gproof = prover.result()[0]
+    whir_amd.lib().whir_device_synchronize()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    prover.set_graph(False)
     with open(os.path.join(ROOT, "whir_amd", "kernel_stats.json")) as fh:
         kstats = json.load(fh)
     instr = kstats.get("leaves16_valu_per_lane")
@@ -381,6 +396,8 @@ def main():
             "k": args.k,
             "proof_bytes": len(proof),
             "proofs_per_s": round(1e3 / ms, 2),
+            "graph_ms_per_step": round(graph_ms, 4),
+            "graph_proof_identical": gproof == proof,
         },
         "roofline": roofline,
         "roofline_dft": roofline_dft,

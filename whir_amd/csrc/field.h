@@ -18,6 +18,7 @@ constexpr uint32_t ONE = 0x01fffffeu;    // 2^32 mod p (Montgomery form of 1)
 constexpr uint32_t TWO = 0x03fffffcu;    // Montgomery form of 2
 constexpr uint32_t W3 = 0x05fffffau;     // Montgomery form of 3 (extension W)
 constexpr uint32_t HALF = 0x00ffffffu;   // Montgomery form of 1/2
+constexpr uint32_t R2MOD = (uint32_t)((uint64_t)ONE * ONE % P);  // 2^64 mod p (mulr(x, R2MOD) = x R)
 
 KB_HD uint32_t reduce64(uint64_t x) {
   // Montgomery reduction, x < p * 2^32 -> x * 2^-32 mod p in [0, p).

@@ -25,7 +25,10 @@ __global__ void __launch_bounds__(MERKLE_THREADS) merkle_leaves_kernel(const uin
 #pragma unroll
   for (int j = 0; j < 24; ++j) s[j] = 0u;
   constexpr int CHUNKS = (ROW_WORDS + 15) / 16;
-#pragma unroll
+  // One permutation body for every chunk (the code of a P24 is ~35 KB):
+  // it restores both the digest lanes and the capacity lanes carried over.
+  constexpr uint32_t OUT = CHUNKS > 1 ? 0xFF00FFu : 0xFFu;
+#pragma unroll 1
   for (int c = 0; c < CHUNKS; ++c) {
     constexpr int FULL = ROW_WORDS >= 16 ? 16 : ROW_WORDS;
     if constexpr (FULL % 4 == 0) {
@@ -41,8 +44,7 @@ __global__ void __launch_bounds__(MERKLE_THREADS) merkle_leaves_kernel(const uin
 #pragma unroll
       for (int j = 0; j < FULL; ++j) s[j] = row[c * 16 + j];
     }
-    if (c + 1 < CHUNKS) permute<24, 16, 24>(s);  // the capacity carries over
-    else permute<24, 0, 8>(s);
+    permute<24, OUT>(s);
   }
   uint4* out = reinterpret_cast<uint4*>(leaves + i * 8);
   out[0] = make_uint4(s[0], s[1], s[2], s[3]);

@@ -222,10 +222,10 @@ KB_HD void internal_lazy(uint32_t (&x)[W], int r, const Poseidon2Lazy<W>& C) {
   }
 }
 
-// Poseidon2 permutation of width W. In: s = words in [0, p). Out: s[i] for
-// OUT_LO <= i < OUT_HI are the reference's output words in [0, p); the other
-// lanes hold scaled values and must not be used.
-template <int W, int OUT_LO = 0, int OUT_HI = W>
+// Poseidon2 permutation of width W. In: s = words in [0, p). Out: s[i] for the
+// lanes i in OUT (a bit mask) are the reference's output words in [0, p); the
+// other lanes hold scaled values and must not be used.
+template <int W, uint32_t OUT = (1u << W) - 1u>
 KB_HD void permute_lazy(uint32_t (&s)[W], const Poseidon2Lazy<W>& C) {
   constexpr int RP = Poseidon2Lazy<W>::RP;
   mds_lazy<W, LAZY_ADD_ALL>(s, C.lin_add[0]);
@@ -239,12 +239,13 @@ KB_HD void permute_lazy(uint32_t (&s)[W], const Poseidon2Lazy<W>& C) {
   for (int r = 0; r < 3; ++r) external_lazy<W, LAZY_ADD_ALL>(s, C.lin_add[6 + r]);
   external_lazy<W, LAZY_ADD_NONE>(s, nullptr);
 #pragma unroll
-  for (int i = OUT_LO; i < OUT_HI; ++i) s[i] = canon2(mulr(s[i], C.fix));
+  for (int i = 0; i < W; ++i)
+    if ((OUT >> i) & 1u) s[i] = canon2(mulr(s[i], C.fix));
 }
 
-template <int W, int OUT_LO = 0, int OUT_HI = W>
+template <int W, uint32_t OUT = (1u << W) - 1u>
 __device__ __forceinline__ void permute(uint32_t (&s)[W]) {
-  permute_lazy<W, OUT_LO, OUT_HI>(s, lazy_consts<W>());
+  permute_lazy<W, OUT>(s, lazy_consts<W>());
 }
 
 // Host: the constants of permute_lazy from the plain round constants
@@ -308,7 +309,7 @@ __device__ __forceinline__ void compress(const uint32_t (&l)[8], const uint32_t 
     s[i] = l[i];
     s[8 + i] = r[i];
   }
-  permute<16, 0, 8>(s);
+  permute<16, 0xFFu>(s);
 #pragma unroll
   for (int i = 0; i < 8; ++i) out[i] = s[i];
 }

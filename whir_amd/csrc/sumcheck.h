@@ -157,7 +157,6 @@ __global__ void __launch_bounds__(SC_THREADS) expr_build_kernel(const uint32_t* 
 // unreduced in 64 bits, so a term costs ~5 VALU instructions instead of an
 // extension-by-base multiply plus modular adds.
 constexpr int COMBINE_SPLIT = 4;
-constexpr uint32_t R2MOD = (uint32_t)(((uint64_t)ONE * ONE) % P);  // 2^64 mod p
 
 __global__ void __launch_bounds__(SC_THREADS) combine_kernel(Ext* __restrict__ w, uint64_t n, uint32_t m0,
                                                          const Ext* __restrict__ eq_pool, uint32_t n_eq,
