@@ -205,9 +205,15 @@ KB_HD void external_lazy(uint32_t (&x)[W], const uint32_t* add) {
 template <int W, bool LAST>
 KB_HD void internal_lazy(uint32_t (&x)[W], int r, const Poseidon2Lazy<W>& C) {
   x[0] = mulr(mulr(mulr(x[0], x[0]), x[0]), C.kappa[r]);
-  uint64_t sum = mul_k<1>(x[0]);
+  // two interleaved accumulation chains (the S-box of lane 0 overlaps the first)
+  uint64_t sa = mul_k<1>(x[1]), sb = mul_k<1>(x[2]);
 #pragma unroll
-  for (int j = 1; j < W; ++j) sum = mad_k<1>(x[j], sum);
+  for (int j = 3; j + 1 < W; j += 2) {
+    sa = mad_k<1>(x[j], sa);
+    sb = mad_k<1>(x[j + 1], sb);
+  }
+  sa = mad_k<1>(x[W - 1], sa);
+  const uint64_t sum = mad_k<1>(x[0], sa + sb);
   uint64_t t0 = add_u(mad_k<2>(2u * P - x[0], sum), LAST ? C.lin_add[5][0] : C.int_add[r + 1]);
   x[0] = LAST ? canon2(redc(t0)) : redc(t0);
 #pragma unroll

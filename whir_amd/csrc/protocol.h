@@ -89,51 +89,80 @@ __global__ void tx_init_kernel(DevTx* t, JobScalars* js, uint32_t cap) {
   }
 }
 
-// transcript.write(root) (poseidon_packed.rs:136-140)
-__global__ void __launch_bounds__(64) write_root_kernel(DevTx* t, uint32_t* log, JobScalars* js, const uint32_t* root) {
-  WaveTx tx;
-  tx_begin(tx, t, log, js);
-  // write(digest): 8 hint words, then observe the 8 elements
-  const uint32_t v = root[threadIdx.x & 7];
-  for (int j = 0; j < 8; ++j) tx.hint_word((uint32_t)__shfl((int)v, j, 8));
-  for (int j = 0; j < 8; ++j) tx.observe((uint32_t)__shfl((int)v, j, 8));
-  tx_end(tx, t, js);
-}
+// Where the next claim's draw goes (a claim kernel can finish one claim and
+// begin the next in the same launch).
+struct ClaimBegin {
+  ClaimSlot* slot;  // nullptr: nothing to begin
+  Ext* eq0;
+  Ext* eq1;
+  uint32_t k, fsv, m0, m1;
+};
+
+struct ClaimScratch {
+  Ext pt[MAX_K];
+  Ext tl[1 << EQ_LDS_LOG];
+  Ext th[1 << EQ_LDS_LOG];
+};
 
 // Draw the claim point: point = expand(k, draw()) (test.rs:225-226, prover.rs:240,
 // whir.rs:227-229), then build the eq tables of point[fsv..k) split at m0.
-__global__ void __launch_bounds__(SERIAL_THREADS) claim_begin_kernel(DevTx* t, uint32_t* log, JobScalars* js,
-                                                                     ClaimSlot* slot, Ext* eq0, Ext* eq1, uint32_t k,
-                                                                     uint32_t fsv, uint32_t m0, uint32_t m1) {
-  __shared__ Ext s_pt[MAX_K];
+__device__ void claim_begin_body(WaveTx& tx, const ClaimBegin& cb, ClaimScratch& sc) {
   if (is_wave0()) {
-    WaveTx tx;
-    tx_begin(tx, t, log, js);
     Ext z = tx.sample_ext();
-    tx_end(tx, t, js);
     if (threadIdx.x == 0) {
-      for (uint32_t i = 0; i < k; ++i) {
-        s_pt[i] = z;
-        slot->point[i] = z;
+      for (uint32_t i = 0; i < cb.k; ++i) {
+        sc.pt[i] = z;
+        cb.slot->point[i] = z;
         z = esqr(z);
       }
-      slot->k = k;
+      cb.slot->k = cb.k;
     }
   }
   __syncthreads();
-  build_eq_block(eq0, s_pt + fsv, (int)m0, eone());
-  build_eq_block(eq1, s_pt + fsv + m0, (int)m1, eone());
+  build_eq_block(cb.eq0, sc.pt + cb.fsv, (int)cb.m0, eone(), sc.tl, sc.th);
+  build_eq_block(cb.eq1, sc.pt + cb.fsv + cb.m0, (int)cb.m1, eone(), sc.tl, sc.th);
+}
+
+// transcript.write(root) (poseidon_packed.rs:136-140): 8 hint words, then
+// observe the 8 elements. Wave 0 only.
+__device__ __forceinline__ void write_root_body(WaveTx& tx, const uint32_t* root) {
+  const uint32_t v = root[threadIdx.x & 7];
+  for (int j = 0; j < 8; ++j) tx.hint_word((uint32_t)__shfl((int)v, j, 8));
+  for (int j = 0; j < 8; ++j) tx.observe((uint32_t)__shfl((int)v, j, 8));
+}
+
+// transcript.write(root) alone (a commitment with no claim drawn after it).
+__global__ void __launch_bounds__(64) write_root_kernel(DevTx* t, uint32_t* log, JobScalars* js, const uint32_t* root) {
+  WaveTx tx;
+  tx_begin(tx, t, log, js);
+  write_root_body(tx, root);
+  tx_end(tx, t, js);
+}
+
+// [transcript.write(root) if root != nullptr;] begin a claim.
+__global__ void __launch_bounds__(SERIAL_THREADS) claim_begin_kernel(DevTx* t, uint32_t* log, JobScalars* js,
+                                                                     const uint32_t* root, ClaimBegin cb) {
+  __shared__ ClaimScratch sc;
+  WaveTx tx;
+  if (is_wave0()) {
+    tx_begin(tx, t, log, js);
+    if (root) write_root_body(tx, root);
+  }
+  claim_begin_body(tx, cb, sc);
+  if (is_wave0()) tx_end(tx, t, js);
 }
 
 // Reduce the evaluation partials and write the claim's value (prover.rs:87,276;
 // whir.rs:230). fsv > 0: partials hold 2^fsv compressed values per block
 // (SvoPoint::eval, svo.rs:77-91); fsv == 0: one value per block (eval_ext).
+// Then begin the next claim when `next.slot` is set.
 __global__ void __launch_bounds__(SERIAL_THREADS) claim_end_kernel(DevTx* t, uint32_t* log, JobScalars* js,
                                                                    ClaimSlot* slot, const Ext* partials,
-                                                                   uint32_t n_part, uint32_t fsv) {
+                                                                   uint32_t n_part, uint32_t fsv, ClaimBegin next) {
   __shared__ Ext sh[SERIAL_THREADS];
   __shared__ Ext s_comp[1 << MAX_F];
   __shared__ Ext s_eq[1 << MAX_F];
+  __shared__ ClaimScratch sc;
   const uint32_t W = 1u << fsv;
   const uint32_t tid = threadIdx.x;
   const uint32_t lanes_per = SERIAL_THREADS / W;
@@ -172,13 +201,14 @@ __global__ void __launch_bounds__(SERIAL_THREADS) claim_end_kernel(DevTx* t, uin
       for (uint32_t i = tid; i < W; i += 64) part = eadd(part, emul(s_comp[i], s_eq[i]));
     eval = wave_sum_ext(part);  // meaningful in wave 0
   }
+  WaveTx tx;
   if (is_wave0()) {
     if (tid == 0) slot->eval = eval;
-    WaveTx tx;
     tx_begin(tx, t, log, js);
     tx.write_e(eval);
-    tx_end(tx, t, js);
   }
+  if (next.slot) claim_begin_body(tx, next, sc);
+  if (is_wave0()) tx_end(tx, t, js);
 }
 
 // ProverLayout::new_prover (prover.rs:279-353) with Svo::calculate_accumulators

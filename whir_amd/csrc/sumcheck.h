@@ -18,7 +18,6 @@ __device__ __forceinline__ Ext ld_ext(const Ext* p) {
 __device__ __forceinline__ void st_ext(Ext* p, const Ext& e) {
   *reinterpret_cast<uint4*>(p) = make_uint4(e.c[0], e.c[1], e.c[2], e.c[3]);
 }
-
 // Sum `n` Ext values held one per thread (n = blockDim) into lane 0 of the block.
 template <int NT>
 __device__ __forceinline__ Ext block_reduce_ext(Ext v, Ext* sh) {
@@ -33,21 +32,39 @@ __device__ __forceinline__ Ext block_reduce_ext(Ext v, Ext* sh) {
   return sh[0];
 }
 
-// eq table of m variables into `out` by doubling (whole block cooperates).
-__device__ void build_eq_block(Ext* out, const Ext* z, int m, Ext scale) {
-  if (threadIdx.x == 0) out[0] = scale;
+// eq table of m <= 2 EQ_LDS_LOG variables into `out` (global, 2^m entries,
+// times `scale`), whole block cooperating: a low table over the first
+// a = min(m, EQ_LDS_LOG) variables and a high table over the rest are doubled
+// in LDS (sL, sH: 2^EQ_LDS_LOG entries each), then one parallel pass writes
+// out[x] = sL[x mod 2^a] * sH[x >> a].
+constexpr int EQ_LDS_LOG = 8;
+
+__device__ void build_eq_block(Ext* out, const Ext* z, int m, Ext scale, Ext* sL, Ext* sH) {
+  const int a = m < EQ_LDS_LOG ? m : EQ_LDS_LOG, b = m - a;
+  if (threadIdx.x == 0) {
+    sL[0] = scale;
+    sH[0] = eone();
+  }
   __syncthreads();
-  for (int i = 0; i < m; ++i) {
+  const int steps = a > b ? a : b;
+  for (int i = 0; i < steps; ++i) {
     const uint32_t h = 1u << i;
-    const Ext zi = z[i];
-    for (uint32_t j = threadIdx.x; j < h; j += blockDim.x) {
-      const Ext lo = out[j];
-      const Ext hi = emul(lo, zi);
-      out[j + h] = hi;
-      out[j] = esub(lo, hi);
+    for (uint32_t j = threadIdx.x; j < 2 * h; j += blockDim.x) {
+      const bool high = j >= h;
+      const uint32_t jj = high ? j - h : j;
+      if (high ? i < b : i < a) {
+        Ext* t = high ? sH : sL;
+        const Ext lo = t[jj];
+        const Ext hi = emul(lo, z[high ? a + i : i]);
+        t[jj + h] = hi;
+        t[jj] = esub(lo, hi);
+      }
     }
     __syncthreads();
   }
+  const uint32_t n = 1u << m, ma = (1u << a) - 1;
+  for (uint32_t x = threadIdx.x; x < n; x += blockDim.x) st_ext(out + x, emul(sL[x & ma], sH[x >> a]));
+  __syncthreads();
 }
 
 // ---- claim evaluation on the base polynomial (SplitEq::compress_hi, eq.rs:290-398) ----

@@ -580,18 +580,35 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
     return rc;
   P->timer_pending = P->timers;
   const uint64_t H0 = 1ull << log_h0;
-  hipLaunchKernelGGL(write_root_kernel, dim3(1), dim3(64), 0, s, P->tx, P->log, P->js, P->tree0 + (2 * H0 - 2) * 8);
+  const uint32_t* root0 = P->tree0 + (2 * H0 - 2) * 8;
 
   // ---- evaluation claims + OOD virtual claims (prover.rs:64-90, 231-277) ----
+  // transcript.write(root) runs in the first claim's kernel; each claim's end
+  // runs in the same kernel as the next claim's draw.
   const uint32_t m0 = P->m0_svo, m1 = P->m1_svo;
+  auto svo_claim = [&](int ci) {
+    ClaimBegin cb{};
+    if (ci < P->n_svo_claims) {
+      cb.slot = P->slots + ci;
+      cb.eq0 = P->svo_pool + ci * P->svo_stride;
+      cb.eq1 = cb.eq0 + (1ull << m0);
+      cb.k = (uint32_t)k;
+      cb.fsv = (uint32_t)f;
+      cb.m0 = m0;
+      cb.m1 = m1;
+    }
+    return cb;
+  };
+  if (P->n_svo_claims > 0)
+    hipLaunchKernelGGL(claim_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, root0,
+                       svo_claim(0));
+  else
+    hipLaunchKernelGGL(write_root_kernel, dim3(1), dim3(64), 0, s, P->tx, P->log, P->js, root0);
   for (int ci = 0; ci < P->n_svo_claims; ++ci) {
-    Ext* eq0 = P->svo_pool + ci * P->svo_stride;
-    Ext* eq1 = eq0 + (1ull << m0);
-    hipLaunchKernelGGL(claim_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots + ci,
-                       eq0, eq1, (uint32_t)k, (uint32_t)f, m0, m1);
-    FSWITCH(f, launch_compress_hi<FL>(1u << m1, s, poly, eq0, eq1, m0, P->partials));
+    const ClaimBegin cb = svo_claim(ci);
+    FSWITCH(f, launch_compress_hi<FL>(1u << m1, s, poly, cb.eq0, cb.eq1, m0, P->partials));
     hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots + ci,
-                       P->partials, 1u << m1, (uint32_t)f);
+                       P->partials, 1u << m1, (uint32_t)f, svo_claim(ci + 1));
   }
   // ---- SVO rounds + expression (prover.rs:279-353, 184-229) ----
   hipLaunchKernelGGL(svo_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots,
@@ -612,19 +629,33 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
       return rc;
     if ((rc = merkle_enqueue(cw, rp.log_rows, 4 * W, tree, s))) return rc;
     const uint64_t Hr = 1ull << rp.log_rows;
-    hipLaunchKernelGGL(write_root_kernel, dim3(1), dim3(64), 0, s, P->tx, P->log, P->js, tree + (2 * Hr - 2) * 8);
-    // OOD claims (whir.rs:224-235)
+    const uint32_t* root = tree + (2 * Hr - 2) * 8;
+    // OOD claims (whir.rs:224-235), the first one drawn in the root's kernel
     const uint32_t om0 = rp.kc / 2, om1 = rp.kc - rp.kc / 2;
+    auto ood_claim = [&](int o) {
+      ClaimBegin cb{};
+      if (o < rp.n_ood) {
+        cb.slot = P->slots + P->n_svo_claims + o;
+        cb.eq0 = P->ood_pool + o * P->ood_stride;
+        cb.eq1 = cb.eq0 + (1ull << om0);
+        cb.k = (uint32_t)rp.kc;
+        cb.fsv = 0;
+        cb.m0 = om0;
+        cb.m1 = om1;
+      }
+      return cb;
+    };
+    if (rp.n_ood > 0)
+      hipLaunchKernelGGL(claim_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, root,
+                         ood_claim(0));
+    else
+      hipLaunchKernelGGL(write_root_kernel, dim3(1), dim3(64), 0, s, P->tx, P->log, P->js, root);
     for (int o = 0; o < rp.n_ood; ++o) {
-      Ext* eq0 = P->ood_pool + o * P->ood_stride;
-      Ext* eq1 = eq0 + (1ull << om0);
-      ClaimSlot* slot = P->slots + P->n_svo_claims + o;
-      hipLaunchKernelGGL(claim_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, slot, eq0,
-                         eq1, (uint32_t)rp.kc, 0u, om0, om1);
-      hipLaunchKernelGGL(eval_ext_kernel, dim3(1u << om1), dim3(SC_THREADS), 0, s, P->ep[cur], eq0, eq1, om0,
+      const ClaimBegin cb = ood_claim(o);
+      hipLaunchKernelGGL(eval_ext_kernel, dim3(1u << om1), dim3(SC_THREADS), 0, s, P->ep[cur], cb.eq0, cb.eq1, om0,
                          P->partials);
-      hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, slot,
-                         P->partials, 1u << om1, 0u);
+      hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, cb.slot,
+                         P->partials, 1u << om1, 0u, ood_claim(o + 1));
     }
     // STIR queries (whir.rs:237-293)
     hipLaunchKernelGGL(stir_sample_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js,
