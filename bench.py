@@ -57,12 +57,13 @@ def synthetic_poly(seed, k):
     return rng.integers(0, 0x7F000001, size=1 << k, dtype=np.uint64).astype(np.uint32)
 
 
-def read_traffic():
+def read_traffic(key="traffic_bytes_per_launch"):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary."""
     try:
         with open(PMC_SUMMARY) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                if line.startswith(key + ":"):
+                    return float(line.split(":", 1)[1].split()[0])
     except OSError:
         pass
     return None
@@ -367,14 +368,17 @@ def main():
         "valu_instr_per_perm": instr,
     }
     dft_s = dft_ms / dft_n / 1e3
+    dft_traffic = read_traffic("ntt_base_traffic_bytes")
     roofline_dft = {
-        "kernel": "base RS encode (ntt_pass_kernel passes, coset LDE of 16 columns)",
+        "kernel": "base RS encode (3 ntt_kernel passes, coset-form LDE 2^20 -> 2^21 rows x 16 columns)",
         "bound": "hbm",
         "achieved": round(dft_words * 4 / dft_s / 1e9, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(dft_words * 4 / dft_s / 1e9 / HBM_PEAK_GBS, 3),
-        "traffic": None,
+        "traffic": dft_traffic,
+        "traffic_GBps": round(dft_traffic / dft_s / 1e9, 1) if dft_traffic else None,
+        "algorithmic_bytes": dft_words * 4,
         "avg_launch_us": round(dft_s * 1e6, 1),
     }
 

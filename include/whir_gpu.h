@@ -60,6 +60,12 @@ typedef struct whir_prover whir_prover; /* device workspace for one proof shape 
 int whir_ctx_create(int device, uint64_t poseidon_seed, whir_ctx** out);
 int whir_ctx_destroy(whir_ctx* ctx);
 
+/* NTT tile shape of this context (no counterpart in the reference, whose DFT
+ * has no device tiling): passes of at most 2^max_log_pass rows (1..8; raised
+ * to what three passes need), workgroups of up to 2^log_threads threads (6..8).
+ * Results do not depend on it. Defaults 8, 8. */
+int whir_ctx_set_ntt_tuning(whir_ctx* ctx, int max_log_pass, int log_threads);
+
 /* ---- component entry points ---- */
 
 /* Replaces TwoAdicSubgroupDft::dft_batch (whir.rs:34): each of the `width`

@@ -38,6 +38,9 @@ using whir_host::RoundPlan;
 struct whir_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // NTT tile shape: largest pass 2^ntt_max_logl rows, 2^ntt_log_threads threads per workgroup
+  int ntt_max_logl = 8;
+  int ntt_log_threads = 8;
   uint32_t* tw = nullptr;  // tw_lo[4096] || tw_hi[4096]
   const uint32_t* tw_lo() const { return tw; }
   const uint32_t* tw_hi() const { return tw + 4096; }
@@ -108,6 +111,13 @@ extern "C" int whir_ctx_destroy(whir_ctx* c) {
   return WHIR_OK;
 }
 
+extern "C" int whir_ctx_set_ntt_tuning(whir_ctx* c, int max_log_pass, int log_threads) {
+  if (!c || max_log_pass < 1 || max_log_pass > 8 || log_threads < 6 || log_threads > 8) return WHIR_EINVAL;
+  c->ntt_max_logl = max_log_pass;
+  c->ntt_log_threads = log_threads;
+  return WHIR_OK;
+}
+
 static hipStream_t pick_stream(whir_ctx* c, void* s) { return s ? (hipStream_t)s : c->stream; }
 
 // ============================ DFT ============================
@@ -120,9 +130,7 @@ struct NttPlan {
   int logl[3] = {0, 0, 0};
 };
 
-constexpr int NTT_MAX_LOGL = 8;
-
-static bool ntt_plan(uint64_t n_in, int log_nn, int log_c, NttPlan& pl) {
+static bool ntt_plan(uint64_t n_in, int log_nn, int log_c, NttPlan& pl, int max_logl) {
   if (log_nn < 1 || log_nn > 24 || log_c < 0 || log_c > 10 || log_nn + log_c > 31) return false;
   if (n_in > (1ull << log_nn)) return false;
   int log_n = 1;
@@ -133,7 +141,8 @@ static bool ntt_plan(uint64_t n_in, int log_nn, int log_c, NttPlan& pl) {
   pl.log_n = log_n;
   pl.log_b = log_c + (log_nn - log_n);
   pl.lv = std::min(2, log_c);
-  pl.P = (log_n + NTT_MAX_LOGL - 1) / NTT_MAX_LOGL;
+  max_logl = std::max(max_logl, (log_n + 2) / 3);  // at most three passes
+  pl.P = (log_n + max_logl - 1) / max_logl;
   if (pl.P > 3) return false;
   int rem = log_n;
   for (int p = 0; p < pl.P; ++p) {
@@ -165,7 +174,7 @@ static NttKernelFn ntt_kernel_for(int logl) {
 static int ntt_enqueue(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_nn, int log_c, uint32_t* out,
                        uint32_t* tmp, hipStream_t s) {
   NttPlan pl;
-  if (!ntt_plan(n_in, log_nn, log_c, pl)) return WHIR_EINVAL;
+  if (!ntt_plan(n_in, log_nn, log_c, pl, c->ntt_max_logl)) return WHIR_EINVAL;
   int used = 0;
   for (int p = 0; p < pl.P; ++p) {
     const int l = pl.logl[p];
@@ -182,7 +191,7 @@ static int ntt_enqueue(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_n
     a.log_n = (uint32_t)pl.log_n;
     a.log_m = (uint32_t)(pl.log_n - used);
     const int log_lines = pl.log_n + pl.log_b - pl.lv - l;
-    a.log_qv = (uint32_t)std::min(8 - lr2, log_lines);
+    a.log_qv = (uint32_t)std::max(0, std::min(c->ntt_log_threads - lr2, log_lines));
     a.first = p == 0;
     a.twiddle_out = p + 1 < pl.P;
     a.natural = pl.P > 1 && p + 1 == pl.P;
@@ -210,7 +219,7 @@ extern "C" int whir_dft_batch(whir_ctx* c, const uint32_t* d_in, uint64_t n_in_r
   hipStream_t s = pick_stream(c, stream);
   uint32_t* tmp = nullptr;
   NttPlan pl;
-  if (!ntt_plan(n_in_rows, (int)log_h, log_c, pl)) return WHIR_EINVAL;
+  if (!ntt_plan(n_in_rows, (int)log_h, log_c, pl, c->ntt_max_logl)) return WHIR_EINVAL;
   if (pl.P > 1) HIP_TRY(hipMallocAsync((void**)&tmp, ((size_t)width << log_h) * 4, s));
   int rc = ntt_enqueue(c, d_in, n_in_rows, (int)log_h, log_c, d_out, tmp, s);
   if (tmp) HIP_TRY(hipFreeAsync(tmp, s));
