@@ -87,6 +87,59 @@ int whir_merkle_commit(whir_ctx* ctx, const uint32_t* d_rows, uint32_t log_h, ui
 /* Poseidon2 permutation of n independent states (width 16 or 24), in place. */
 int whir_poseidon2_permute(whir_ctx* ctx, int width, uint32_t* d_states, uint64_t n, void* stream);
 
+/* Replaces MerkleTree::query / MerkleTreeExt::query (crates/merkle/src/lib.rs:30-37,
+ * poseidon_packed.rs:145-233) for n leaf indices: per index, the row (row_words
+ * words of d_rows) then the log_h sibling digests bottom-up (sibling of the
+ * node at level i = (index mod 2^(log_h-i)) ^ 2^(log_h-i-1)), i.e. the hint
+ * words the reference writes. d_out: n * (row_words + 8 log_h) words. Indices
+ * >= 2^log_h leave their slot untouched. */
+int whir_merkle_open(whir_ctx* ctx, const uint32_t* d_layers, uint32_t log_h, const uint32_t* d_rows,
+                     uint32_t row_words, const uint32_t* d_idx, uint32_t n, uint32_t* d_out, void* stream);
+
+/* ---- sumcheck / eq functions (extension elements are 4 Montgomery words,
+ * coefficient-major; points are k extension elements, bit j <-> z_j) ---- */
+
+/* Replaces Point::eq / eq_packed (crates/poly/src/point.rs:110-126):
+ * d_out[x] = scale * prod_j (x_j ? z_j : 1 - z_j), 2^k entries, k <= 32. */
+int whir_eq_table(whir_ctx* ctx, const uint32_t* d_z, uint32_t k, const uint32_t scale[4], uint32_t* d_out,
+                  void* stream);
+
+/* Replaces SplitEq::eval_ext (crates/poly/src/eq.rs:179-250) / Poly::eval_ext
+ * (poly.rs:89-95): d_out (1 ext) = <poly, eq(z)> for a 2^k extension polynomial. */
+int whir_eval_ext(whir_ctx* ctx, const uint32_t* d_poly, uint32_t k, const uint32_t* d_z, uint32_t* d_out,
+                  void* stream);
+
+/* Replaces SplitEq::compress_hi (crates/poly/src/eq.rs:290-398): binds the top m
+ * of the k variables of a base polynomial to z (m coordinates), d_out has
+ * 2^(k-m) <= 32 extension values; m = k is SplitEq::eval_base (eq.rs:107-177). */
+int whir_compress_hi(whir_ctx* ctx, const uint32_t* d_poly, uint32_t k, const uint32_t* d_z, uint32_t m,
+                     uint32_t* d_out, void* stream);
+
+/* Replaces SplitEq::compress_lo (crates/poly/src/eq.rs:400-539): binds the low
+ * m <= 5 variables of a base polynomial; d_out has 2^(k-m) extension values. */
+int whir_compress_lo(whir_ctx* ctx, const uint32_t* d_poly, uint32_t k, const uint32_t* d_z, uint32_t m,
+                     uint32_t* d_out, void* stream);
+
+/* Replaces sumcheck coeffs (crates/whir/src/sumcheck/expr.rs:14-27): for two 2^k
+ * extension arrays, d_c0c2 = (sum w[2i] f[2i], sum (2w[2i+1]-w[2i])(2f[2i+1]-f[2i])). */
+int whir_sumcheck_coeffs(whir_ctx* ctx, const uint32_t* d_f, const uint32_t* d_w, uint32_t k, uint32_t* d_c0c2,
+                         void* stream);
+
+/* Replaces Poly::fix_lo_var_mut (crates/poly/src/poly.rs:131-142):
+ * d_out[i] = a[2i] + r (a[2i+1] - a[2i]) for a 2^k extension array. */
+int whir_fix_lo_var(whir_ctx* ctx, const uint32_t* d_poly, uint32_t k, const uint32_t r[4], uint32_t* d_out,
+                    void* stream);
+
+/* Replaces combine_eqs (crates/whir/src/sumcheck/combine.rs:178-203):
+ * d_w[x] += sum_j alpha^j eq(points_j)[x] over n points of k coordinates. */
+int whir_combine_eqs(whir_ctx* ctx, uint32_t* d_w, uint32_t k, const uint32_t* d_points, uint32_t n,
+                     const uint32_t alpha[4], void* stream);
+
+/* Replaces combine_pows (crates/whir/src/sumcheck/combine.rs:407-438):
+ * d_w[i] += sum_j alpha^(shift+j) vars_j^i over n base-field vars. */
+int whir_combine_pows(whir_ctx* ctx, uint32_t* d_w, uint32_t k, const uint32_t* d_vars, uint32_t n,
+                      const uint32_t alpha[4], uint64_t shift, void* stream);
+
 /* ---- whole prover (Whir::commit + claims + Whir::open) ---- */
 
 int whir_prover_create(whir_ctx* ctx, const whir_params* params, whir_prover** out);

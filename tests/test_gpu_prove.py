@@ -72,3 +72,16 @@ def test_same_result_alone_and_repeated(gpu_ctx, oracle):
     assert outs[0] == outs[2] and outs[0] != outs[1]
     want = oracle.whir_prove(k, f, rate, ir, s, sec, npts, oracle.rand_poly(4, k))[0]
     assert outs[1] == want
+
+
+def test_large_job_eight_claims(gpu_ctx, oracle):
+    """The large single-job configuration: 2^26 coefficients, rate 1/4 (a 1 GiB
+    base codeword), folding 4, eight evaluation claims — bytes equal to the
+    oracle's and accepted by the oracle verifier."""
+    cfg = (26, 4, 2, 3, 1, 123, 8)
+    gproof, gcp, groot, oproof, ocp, oroot = _prove_both(oracle, cfg, seed=3)
+    assert np.array_equal(groot, oroot)
+    d = _first_diff(gproof, oproof)
+    assert d == -1, f"proof differs at byte {d}"
+    rc, vcp = oracle.whir_verify(*cfg, gproof)
+    assert rc == 0 and vcp == gcp

@@ -47,6 +47,17 @@ SIGNATURES = [
     ("whir_dft_batch", ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp]),
     ("whir_merkle_commit", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp, _u32p, _vp]),
     ("whir_poseidon2_permute", ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint64, _vp]),
+    ("whir_merkle_open", ctypes.c_int,
+     [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp]),
+    ("whir_eq_table", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, _vp, _vp]),
+    ("whir_eval_ext", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp, _vp]),
+    ("whir_compress_hi", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp]),
+    ("whir_compress_lo", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp]),
+    ("whir_sumcheck_coeffs", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    ("whir_fix_lo_var", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, _vp, _vp]),
+    ("whir_combine_eqs", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _u32p, _vp]),
+    ("whir_combine_pows", ctypes.c_int,
+     [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _u32p, ctypes.c_uint64, _vp]),
     ("whir_prover_create", ctypes.c_int, [_vp, ctypes.POINTER(whir_params), ctypes.POINTER(_vp)]),
     ("whir_prover_destroy", ctypes.c_int, [_vp]),
     ("whir_prover_proof_capacity", ctypes.c_size_t, [_vp]),

@@ -118,6 +118,127 @@ def poseidon2_permute(states, width, cooperative=False, ctx=None):
     return d.download(np.uint32, s.size).reshape(s.shape)
 
 
+def _dev(arr):
+    a = _u32(arr)
+    d = DeviceBuffer(a.nbytes)
+    d.upload(a)
+    return d, a
+
+
+def _ext4(v):
+    a = _u32(v).reshape(4)
+    return a, a.ctypes.data_as(_u32p)
+
+
+def _run(rc, name, out, count):
+    check(rc, name)
+    check(lib().whir_device_synchronize(), "sync")
+    return out.download(np.uint32, count)
+
+
+def merkle_open(layers, rows, log_h, indices, ctx=None):
+    """MerkleTree::query for each index: row words then log_h sibling digests
+    (rows: 2^log_h x row_words; layers: output of merkle_commit)."""
+    ctx = ctx or default_context()
+    r = _u32(rows)
+    row_words = r.shape[1]
+    dl, _ = _dev(layers)
+    dr, _ = _dev(r)
+    di, idx = _dev(np.asarray(indices, np.uint32))
+    per = row_words + 8 * log_h
+    do = DeviceBuffer(max(4, len(idx) * per * 4))
+    out = _run(lib().whir_merkle_open(ctx.ptr, dl.ptr, log_h, dr.ptr, row_words, di.ptr, len(idx), do.ptr, None),
+               "whir_merkle_open", do, len(idx) * per)
+    return out.reshape(len(idx), per)
+
+
+def eq_table(z, scale, ctx=None):
+    """Point::eq: 2^k x 4 words for a point z (k x 4 words)."""
+    ctx = ctx or default_context()
+    dz, zz = _dev(np.asarray(z).reshape(-1, 4))
+    k = zz.shape[0]
+    sc, scp = _ext4(scale)
+    do = DeviceBuffer((4 << k) * 4)
+    return _run(lib().whir_eq_table(ctx.ptr, dz.ptr, k, scp, do.ptr, None), "whir_eq_table", do,
+                4 << k).reshape(-1, 4)
+
+
+def eval_ext(poly, z, ctx=None):
+    """SplitEq::eval_ext of a 2^k extension polynomial (2^k x 4 words)."""
+    ctx = ctx or default_context()
+    dp, p = _dev(np.asarray(poly).reshape(-1, 4))
+    dz, zz = _dev(np.asarray(z).reshape(-1, 4))
+    do = DeviceBuffer(16)
+    return _run(lib().whir_eval_ext(ctx.ptr, dp.ptr, zz.shape[0], dz.ptr, do.ptr, None), "whir_eval_ext", do, 4)
+
+
+def compress_hi(poly, k, z, ctx=None):
+    """SplitEq::compress_hi: bind the top len(z) variables of a base polynomial."""
+    ctx = ctx or default_context()
+    dp, _ = _dev(poly)
+    dz, zz = _dev(np.asarray(z).reshape(-1, 4))
+    m = zz.shape[0]
+    do = DeviceBuffer((4 << (k - m)) * 4)
+    return _run(lib().whir_compress_hi(ctx.ptr, dp.ptr, k, dz.ptr, m, do.ptr, None), "whir_compress_hi", do,
+                4 << (k - m)).reshape(-1, 4)
+
+
+def compress_lo(poly, k, z, ctx=None):
+    """SplitEq::compress_lo: bind the low len(z) variables of a base polynomial."""
+    ctx = ctx or default_context()
+    dp, _ = _dev(poly)
+    dz, zz = _dev(np.asarray(z).reshape(-1, 4))
+    m = zz.shape[0]
+    do = DeviceBuffer((4 << (k - m)) * 4)
+    return _run(lib().whir_compress_lo(ctx.ptr, dp.ptr, k, dz.ptr, m, do.ptr, None), "whir_compress_lo", do,
+                4 << (k - m)).reshape(-1, 4)
+
+
+def sumcheck_coeffs(f, w, ctx=None):
+    """Sumcheck round coefficients (c0, c2) of two 2^k extension arrays."""
+    ctx = ctx or default_context()
+    df, ff = _dev(np.asarray(f).reshape(-1, 4))
+    dw, _ = _dev(np.asarray(w).reshape(-1, 4))
+    k = int(ff.shape[0]).bit_length() - 1
+    do = DeviceBuffer(32)
+    return _run(lib().whir_sumcheck_coeffs(ctx.ptr, df.ptr, dw.ptr, k, do.ptr, None), "whir_sumcheck_coeffs", do,
+                8).reshape(2, 4)
+
+
+def fix_lo_var(poly, r, ctx=None):
+    """Poly::fix_lo_var_mut of a 2^k extension array."""
+    ctx = ctx or default_context()
+    dp, p = _dev(np.asarray(poly).reshape(-1, 4))
+    k = int(p.shape[0]).bit_length() - 1
+    rr, rp = _ext4(r)
+    do = DeviceBuffer((4 << (k - 1)) * 4)
+    return _run(lib().whir_fix_lo_var(ctx.ptr, dp.ptr, k, rp, do.ptr, None), "whir_fix_lo_var", do,
+                4 << (k - 1)).reshape(-1, 4)
+
+
+def combine_eqs(w, points, alpha, ctx=None):
+    """w += sum_j alpha^j eq(points_j); w is 2^k x 4 words, points n x k x 4."""
+    ctx = ctx or default_context()
+    dw, ww = _dev(np.asarray(w).reshape(-1, 4))
+    k = int(ww.shape[0]).bit_length() - 1
+    pts = np.asarray(points, np.uint32).reshape(-1, k, 4)
+    dpt, _ = _dev(pts)
+    aa, ap = _ext4(alpha)
+    return _run(lib().whir_combine_eqs(ctx.ptr, dw.ptr, k, dpt.ptr, pts.shape[0], ap, None), "whir_combine_eqs",
+                dw, ww.size).reshape(-1, 4)
+
+
+def combine_pows(w, variables, alpha, shift, ctx=None):
+    """w[i] += sum_j alpha^(shift+j) vars_j^i; w is 2^k x 4 words."""
+    ctx = ctx or default_context()
+    dw, ww = _dev(np.asarray(w).reshape(-1, 4))
+    k = int(ww.shape[0]).bit_length() - 1
+    dv, vv = _dev(np.asarray(variables, np.uint32))
+    aa, ap = _ext4(alpha)
+    return _run(lib().whir_combine_pows(ctx.ptr, dw.ptr, k, dv.ptr, vv.size, ap, shift, None), "whir_combine_pows",
+                dw, ww.size).reshape(-1, 4)
+
+
 class Prover:
     """Device workspace for one proof shape (whir_prover)."""
 

@@ -858,3 +858,4 @@ extern "C" int whir_device_synchronize(void) {
 }
 
 #include "server.h"
+#include "components.h"
