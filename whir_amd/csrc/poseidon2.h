@@ -231,6 +231,17 @@ KB_HD void internal_lazy(uint32_t (&x)[W], int r, const Poseidon2Lazy<W>& C) {
 // Poseidon2 permutation of width W. In: s = words in [0, p). Out: s[i] for the
 // lanes i in OUT (a bit mask) are the reference's output words in [0, p); the
 // other lanes hold scaled values and must not be used.
+// Internal rounds R..RP-2 by template recursion, so that every round is
+// straight-line code with compile-time constant offsets (a counted loop here
+// makes the compiler re-load the round constants through indexed scalar loads).
+template <int W, int R>
+KB_HD void internal_rounds_lazy(uint32_t (&s)[W], const Poseidon2Lazy<W>& C) {
+  if constexpr (R < Poseidon2Lazy<W>::RP - 1) {
+    internal_lazy<W, false>(s, R, C);
+    internal_rounds_lazy<W, R + 1>(s, C);
+  }
+}
+
 template <int W, uint32_t OUT = (1u << W) - 1u>
 KB_HD void permute_lazy(uint32_t (&s)[W], const Poseidon2Lazy<W>& C) {
   constexpr int RP = Poseidon2Lazy<W>::RP;
@@ -238,8 +249,7 @@ KB_HD void permute_lazy(uint32_t (&s)[W], const Poseidon2Lazy<W>& C) {
 #pragma unroll
   for (int r = 0; r < 3; ++r) external_lazy<W, LAZY_ADD_ALL>(s, C.lin_add[r + 1]);
   external_lazy<W, LAZY_ADD_FIRST>(s, C.lin_add[4]);
-#pragma unroll
-  for (int r = 0; r < RP - 1; ++r) internal_lazy<W, false>(s, r, C);
+  internal_rounds_lazy<W, 0>(s, C);
   internal_lazy<W, true>(s, RP - 1, C);
 #pragma unroll
   for (int r = 0; r < 3; ++r) external_lazy<W, LAZY_ADD_ALL>(s, C.lin_add[6 + r]);
