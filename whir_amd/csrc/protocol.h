@@ -387,13 +387,17 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t*
 }
 
 // STIR indexes: n draws of `bits` bits, sorted and deduplicated (whir.rs:63-77).
+// The draws are a serial chain on wave 0; the sort is a bitonic network over
+// the next power of two (padding sorts last), then a workgroup prefix count
+// over "first of its value" flags gives each distinct index its rank.
 __global__ void __launch_bounds__(SERIAL_THREADS) stir_sample_kernel(DevTx* t, uint32_t* log, JobScalars* js,
                                                                      uint32_t n, uint32_t bits, uint32_t* idx_out,
                                                                      uint32_t* count_out) {
   __shared__ uint32_t s_v[MAX_STIR];
-  __shared__ uint8_t s_first[MAX_STIR];
-  __shared__ uint32_t s_count;
+  __shared__ uint32_t s_scan[SERIAL_THREADS];
   const uint32_t tid = threadIdx.x;
+  uint32_t np = 1;
+  while (np < n) np <<= 1;
   if (is_wave0()) {
     WaveTx tx;
     tx_begin(tx, t, log, js);
@@ -403,26 +407,42 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_sample_kernel(DevTx* t, u
     }
     tx_end(tx, t, js);
   }
-  if (tid == 0) s_count = 0;
+  for (uint32_t i = n + tid; i < np; i += SERIAL_THREADS) s_v[i] = 0xFFFFFFFFu;  // > any index (bits <= 31)
   __syncthreads();
-  for (uint32_t i = tid; i < n; i += SERIAL_THREADS) {
-    const uint32_t v = s_v[i];
-    bool first = true;
-    for (uint32_t j = 0; j < i; ++j) first &= s_v[j] != v;
-    s_first[i] = first;
-    if (first) atomicAdd(&s_count, 1u);
+  for (uint32_t size = 2; size <= np; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t i = tid; i < np / 2; i += SERIAL_THREADS) {
+        const uint32_t lo = 2 * i - (i & (stride - 1));  // index with bit `stride` clear
+        const uint32_t hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const uint32_t a = s_v[lo], b = s_v[hi];
+        if ((a > b) == up) {
+          s_v[lo] = b;
+          s_v[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
   }
+  // per thread: a contiguous chunk of the sorted values; count the firsts
+  const uint32_t chunk = (n + SERIAL_THREADS - 1) / SERIAL_THREADS;
+  const uint32_t b0 = tid * chunk, b1 = b0 + chunk < n ? b0 + chunk : n;
+  uint32_t cnt = 0;
+  for (uint32_t i = b0; i < b1; ++i) cnt += (i == 0 || s_v[i] != s_v[i - 1]) ? 1u : 0u;
+  s_scan[tid] = cnt;
   __syncthreads();
-  for (uint32_t i = tid; i < n; i += SERIAL_THREADS) {
-    if (!s_first[i]) continue;
-    const uint32_t v = s_v[i];
-    uint32_t rank = 0;
-    for (uint32_t j = 0; j < n; ++j) rank += (s_first[j] && s_v[j] < v) ? 1u : 0u;
-    idx_out[rank] = v;
+  for (uint32_t off = 1; off < SERIAL_THREADS; off <<= 1) {
+    const uint32_t add = tid >= off ? s_scan[tid - off] : 0u;
+    __syncthreads();
+    s_scan[tid] += add;
+    __syncthreads();
   }
-  if (tid == 0) {
-    *count_out = s_count;
-    js->stir_count = s_count;
+  uint32_t rank = s_scan[tid] - cnt;  // exclusive prefix
+  for (uint32_t i = b0; i < b1; ++i)
+    if (i == 0 || s_v[i] != s_v[i - 1]) idx_out[rank++] = s_v[i];
+  if (tid == SERIAL_THREADS - 1) {
+    *count_out = s_scan[tid];
+    js->stir_count = s_scan[tid];
   }
 }
 
