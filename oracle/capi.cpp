@@ -213,4 +213,95 @@ int oracle_whir_verify(int k, int folding, int rate, int ir, int soundness, int 
   return s;
 }
 
+// ---- multi-polynomial stacking ----
+// Polys cross as one concatenated array in id order; claims as parallel arrays
+// (poly id, 1 = draws a fresh point). Exceptions (a malformed stack) return -2.
+static std::vector<int> k_vec(int n, const int* k) { return std::vector<int>(k, k + n); }
+static std::vector<std::vector<F>> polys_from(int n, const int* k, const uint32_t* data) {
+  std::vector<std::vector<F>> out;
+  size_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    const size_t len = (size_t)1 << k[i];
+    std::vector<F> v(len);
+    memcpy(v.data(), data + off, len * 4);
+    off += len;
+    out.push_back(std::move(v));
+  }
+  return out;
+}
+static std::vector<StackClaim> claims_from(int n, const int* poly, const uint8_t* fresh) {
+  std::vector<StackClaim> out((size_t)n);
+  for (int i = 0; i < n; ++i) out[i] = StackClaim{poly[i], fresh[i] ? 1 : 0};
+  return out;
+}
+
+int oracle_stack_layout(int n_polys, const int* k_polys, int k_svo, int* k_out, uint64_t* offsets, int* order) {
+  try {
+    const StackLayout L = stack_layout(k_vec(n_polys, k_polys), k_svo);
+    *k_out = L.k;
+    for (int i = 0; i < n_polys; ++i) {
+      offsets[i] = L.offset[i];
+      order[i] = L.order[i];
+    }
+    return 0;
+  } catch (const std::exception&) {
+    return -2;
+  }
+}
+
+int oracle_stack_claim_ranks(int n_polys, const int* k_polys, int k_svo, int n_claims, const int* claim_poly,
+                             const uint8_t* claim_new, uint64_t* ranks) {
+  try {
+    const StackLayout L = stack_layout(k_vec(n_polys, k_polys), k_svo);
+    const auto r = stack_claim_ranks(L, claims_from(n_claims, claim_poly, claim_new));
+    for (int i = 0; i < n_claims; ++i) ranks[i] = r[i];
+    return 0;
+  } catch (const std::exception&) {
+    return -2;
+  }
+}
+
+int oracle_stack_virtual_check(int n_polys, const int* k_polys, const uint32_t* polys, int k_svo,
+                               const uint32_t* x) {
+  try {
+    return stack_virtual_check(polys_from(n_polys, k_polys, polys), k_svo, get_e(x));
+  } catch (const std::exception&) {
+    return -2;
+  }
+}
+
+int oracle_whir_prove_stacked(int k, int folding, int rate, int ir, int soundness, int security, int n_polys,
+                              const int* k_polys, const uint32_t* polys, int n_claims, const int* claim_poly,
+                              const uint8_t* claim_new, uint8_t* proof, size_t cap, size_t* len,
+                              uint32_t* checkpoint, uint32_t* root) {
+  try {
+    Params p{k, folding, rate, ir, soundness, security};
+    ProveResult r = whir_prove_stacked(p, polys_from(n_polys, k_polys, polys),
+                                       claims_from(n_claims, claim_poly, claim_new));
+    *len = r.proof.size();
+    if (checkpoint) *checkpoint = r.checkpoint.v;
+    if (root) memcpy(root, r.root.d, 32);
+    if (r.proof.size() > cap) return -1;
+    memcpy(proof, r.proof.data(), r.proof.size());
+    return 0;
+  } catch (const std::exception&) {
+    return -2;
+  }
+}
+
+int oracle_whir_verify_stacked(int k, int folding, int rate, int ir, int soundness, int security, int n_polys,
+                               const int* k_polys, int n_claims, const int* claim_poly, const uint8_t* claim_new,
+                               const uint8_t* proof, size_t len, uint32_t* checkpoint) {
+  try {
+    Params p{k, folding, rate, ir, soundness, security};
+    F cp{0};
+    int s = whir_verify_stacked(p, proof, len, k_vec(n_polys, k_polys),
+                                claims_from(n_claims, claim_poly, claim_new), cp);
+    if (checkpoint) *checkpoint = cp.v;
+    return s;
+  } catch (const std::exception&) {
+    return -2;
+  }
+}
+
 }  // extern "C"

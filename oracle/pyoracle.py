@@ -78,6 +78,15 @@ def lib():
         L.oracle_whir_prove.argtypes = [ctypes.c_int] * 7 + [u32p, u8p, ctypes.c_size_t,
                                                              ctypes.POINTER(ctypes.c_size_t), u32p, u32p]
         L.oracle_whir_verify.argtypes = [ctypes.c_int] * 7 + [u8p, ctypes.c_size_t, u32p]
+        i32p = ctypes.POINTER(ctypes.c_int)
+        u64p = ctypes.POINTER(ctypes.c_uint64)
+        L.oracle_stack_layout.argtypes = [ctypes.c_int, i32p, ctypes.c_int, i32p, u64p, i32p]
+        L.oracle_stack_claim_ranks.argtypes = [ctypes.c_int, i32p, ctypes.c_int, ctypes.c_int, i32p, u8p, u64p]
+        L.oracle_stack_virtual_check.argtypes = [ctypes.c_int, i32p, u32p, ctypes.c_int, u32p]
+        L.oracle_whir_prove_stacked.argtypes = ([ctypes.c_int] * 7 + [i32p, u32p, ctypes.c_int, i32p, u8p, u8p,
+                                                ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), u32p, u32p])
+        L.oracle_whir_verify_stacked.argtypes = ([ctypes.c_int] * 7 + [i32p, ctypes.c_int, i32p, u8p, u8p,
+                                                 ctypes.c_size_t, u32p])
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_get_threads.restype = ctypes.c_int
         _lib = L
@@ -349,3 +358,81 @@ def whir_verify(k, folding, rate, ir, soundness, security, n_points, proof):
     rc = lib().oracle_whir_verify(k, folding, rate, ir, soundness, security, n_points,
                                   _p(b, u8p), b.size, _p(cp))
     return rc, int(cp[0])
+
+
+# ---------------- multi-polynomial stacking ----------------
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def _claims(claims):
+    """claims: list of (poly id, fresh point flag) in transcript order."""
+    cp = _i32([c[0] for c in claims])
+    cn = np.ascontiguousarray(np.asarray([1 if c[1] else 0 for c in claims], dtype=np.uint8))
+    return cp, cn
+
+
+def stack_layout(k_polys, k_svo):
+    """(k, offsets per poly id, poly ids in Selector order)."""
+    kp = _i32(k_polys)
+    kout = np.zeros(1, np.int32)
+    offs = np.zeros(len(kp), np.uint64)
+    order = np.zeros(len(kp), np.int32)
+    i32p = ctypes.POINTER(ctypes.c_int)
+    rc = lib().oracle_stack_layout(len(kp), _p(kp, i32p), k_svo, _p(kout, i32p),
+                                   _p(offs, ctypes.POINTER(ctypes.c_uint64)), _p(order, i32p))
+    if rc:
+        raise ValueError("bad stack")
+    return int(kout[0]), [int(o) for o in offs], [int(o) for o in order]
+
+
+def stack_claim_ranks(k_polys, k_svo, claims):
+    kp = _i32(k_polys)
+    cp, cn = _claims(claims)
+    out = np.zeros(len(claims), np.uint64)
+    i32p = ctypes.POINTER(ctypes.c_int)
+    rc = lib().oracle_stack_claim_ranks(len(kp), _p(kp, i32p), k_svo, len(claims), _p(cp, i32p), _p(cn, u8p),
+                                        _p(out, ctypes.POINTER(ctypes.c_uint64)))
+    if rc:
+        raise ValueError("bad stack")
+    return [int(r) for r in out]
+
+
+def stack_virtual_check(polys, k_svo, x):
+    kp = _i32([int(np.log2(len(p))) for p in polys])
+    data = _u32(np.concatenate([np.asarray(p, np.uint32) for p in polys]))
+    xe = _u32(x)
+    i32p = ctypes.POINTER(ctypes.c_int)
+    return lib().oracle_stack_virtual_check(len(kp), _p(kp, i32p), _p(data), k_svo, _p(xe))
+
+
+def whir_prove_stacked(k, folding, rate, ir, soundness, security, polys, claims):
+    kp = _i32([int(np.log2(len(p))) for p in polys])
+    data = _u32(np.concatenate([np.asarray(p, np.uint32) for p in polys]))
+    cp, cn = _claims(claims)
+    i32p = ctypes.POINTER(ctypes.c_int)
+    cap = 1 << 16
+    while True:
+        buf = np.zeros(cap, np.uint8)
+        ln = ctypes.c_size_t()
+        chk = np.zeros(1, np.uint32)
+        root = np.zeros(8, np.uint32)
+        rc = lib().oracle_whir_prove_stacked(k, folding, rate, ir, soundness, security, len(kp), _p(kp, i32p),
+                                             _p(data), len(claims), _p(cp, i32p), _p(cn, u8p), _p(buf, u8p), cap,
+                                             ctypes.byref(ln), _p(chk), _p(root))
+        if rc == -2:
+            raise ValueError("bad stack")
+        if rc == 0:
+            return bytes(buf[: ln.value]), int(chk[0]), root
+        cap = ln.value
+
+
+def whir_verify_stacked(k, folding, rate, ir, soundness, security, k_polys, claims, proof):
+    kp = _i32(k_polys)
+    cp, cn = _claims(claims)
+    b = np.frombuffer(bytes(proof), dtype=np.uint8).copy()
+    chk = np.zeros(1, np.uint32)
+    i32p = ctypes.POINTER(ctypes.c_int)
+    rc = lib().oracle_whir_verify_stacked(k, folding, rate, ir, soundness, security, len(kp), _p(kp, i32p),
+                                          len(claims), _p(cp, i32p), _p(cn, u8p), _p(b, u8p), b.size, _p(chk))
+    return rc, int(chk[0])

@@ -685,22 +685,159 @@ std::vector<F> rand_poly(uint64_t seed, int k) {
   return v;
 }
 
+// ======================= multi-polynomial stacking =======================
+bool selector_less(int ka, size_t ia, int kb, size_t ib) {
+  // derived Ord of Selector(Vec<bool>) (sumcheck/mod.rs:30-37): lexicographic over
+  // the bits (index >> i) & 1, i = 0, 1, ..., with false < true; on a common
+  // prefix the shorter selector sorts first.
+  const int n = std::min(ka, kb);
+  for (int i = 0; i < n; ++i) {
+    const int a = (int)((ia >> i) & 1), b = (int)((ib >> i) & 1);
+    if (a != b) return a < b;
+  }
+  return ka < kb;
+}
+
+StackLayout stack_layout(const std::vector<int>& k_polys, int k_svo) {
+  // ProverStack::layout (prover.rs:92-124) = VerifierLayout::new (verifier.rs:152-174):
+  // stable sort by k, then walk from the largest (for equal k, the last
+  // registered first) assigning consecutive offsets.
+  const size_t n = k_polys.size();
+  if (n == 0) throw std::runtime_error("empty stack");
+  std::vector<int> ids(n);
+  for (size_t i = 0; i < n; ++i) ids[i] = (int)i;
+  std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) { return k_polys[a] < k_polys[b]; });
+  size_t total = 0;
+  for (int kp : k_polys) total += (size_t)1 << kp;
+  StackLayout L;
+  while (((size_t)1 << L.k) < total) ++L.k;  // log2_ceil_usize
+  if (L.k < k_svo) throw std::runtime_error("stack smaller than the SVO rounds");
+  L.offset.resize(n);
+  L.sel_k.resize(n);
+  L.sel_index.resize(n);
+  size_t off = 0;
+  for (size_t j = n; j-- > 0;) {
+    const int id = ids[j], kp = k_polys[id];
+    L.sel_k[id] = L.k - kp;
+    L.sel_index[id] = off >> kp;  // off is a multiple of 2^kp (larger blocks come first)
+    L.offset[id] = off;
+    off += (size_t)1 << kp;
+  }
+  L.order = ids;
+  std::sort(L.order.begin(), L.order.end(), [&](int a, int b) {
+    return selector_less(L.sel_k[a], L.sel_index[a], L.sel_k[b], L.sel_index[b]);
+  });
+  return L;
+}
+
+std::vector<F> stack_polys(const StackLayout& L, const std::vector<std::vector<F>>& polys) {
+  // prover.rs:114-118: zero vector of 2^k, each poly copied at selector.index << k_poly
+  std::vector<F> out((size_t)1 << L.k, F{0});
+  for (size_t id = 0; id < polys.size(); ++id)
+    memcpy(out.data() + L.offset[id], polys[id].data(), polys[id].size() * sizeof(F));
+  return out;
+}
+
+std::vector<size_t> stack_claim_ranks(const StackLayout& L, const std::vector<StackClaim>& claims) {
+  // claims_in_layout_order (prover.rs:155-166): layout (Selector) order, then each
+  // poly's claims in the order they were made (poly_to_claim_map).
+  std::vector<size_t> rank(claims.size());
+  size_t r = 0;
+  for (int id : L.order)
+    for (size_t c = 0; c < claims.size(); ++c)
+      if (claims[c].poly == id) rank[c] = r++;
+  return rank;
+}
+
+static Point hypercube(size_t value, int k) {  // point.rs:85-96
+  Point p(k);
+  for (int i = 0; i < k; ++i) p[i] = ((value >> i) & 1) ? eone() : ezero();
+  return p;
+}
+
+static int log2_exact(size_t n) {
+  int k = 0;
+  while (((size_t)1 << k) < n) ++k;
+  if (((size_t)1 << k) != n) throw std::runtime_error("poly size is not a power of two");
+  return k;
+}
+
+static void check_claims(const std::vector<int>& kp, const std::vector<StackClaim>& claims, int f) {
+  // ProverStack::eval asserts point.k() == k_poly (prover.rs:73); SvoPoint::new
+  // needs k_poly >= k_svo.
+  for (size_t c = 0; c < claims.size(); ++c) {
+    const int id = claims[c].poly;
+    if (id < 0 || id >= (int)kp.size() || kp[id] < f) throw std::runtime_error("bad claim poly");
+    if (!claims[c].new_point && (c == 0 || kp[claims[c - 1].poly] != kp[id]))
+      throw std::runtime_error("shared point of a different size");
+  }
+}
+
+// write_virtual_eval (prover.rs:231-277) for point `pt` over the stacked polys:
+// each poly's claim at pt[0..k_poly) weighted by eq(selector, pt[k_poly..k)).
+static E virtual_claim(const StackLayout& L, const std::vector<std::vector<F>>& polys, int f, const Point& pt,
+                       const std::vector<SvoCoeffs>& svo, Accs& accs) {
+  std::vector<SvoClaim> local;
+  std::vector<E> wts;
+  local.reserve(L.order.size());
+  E eval = ezero();
+  for (int id : L.order) {
+    const int kp = (int)(L.k - L.sel_k[id]);
+    Point lp(pt.begin(), pt.begin() + kp), rest(pt.begin() + kp, pt.end());
+    const E wgt = eval_eq_xy(hypercube(L.sel_index[id], L.sel_k[id]), rest);
+    local.push_back(svo_eval(f, lp, polys[id]));
+    eval = eadd(eval, emul(wgt, local.back().eval));
+    wts.push_back(wgt);
+  }
+  std::vector<const SvoClaim*> ptrs;
+  for (auto& c : local) ptrs.push_back(&c);
+  accs = svo_accumulators(svo, ptrs, wts);
+  return eval;
+}
+
+int stack_virtual_check(const std::vector<std::vector<F>>& polys, int k_svo, E x) {
+  std::vector<int> kp;
+  for (auto& p : polys) kp.push_back(log2_exact(p.size()));
+  const StackLayout L = stack_layout(kp, k_svo);
+  const Point pt = expand(L.k, x);
+  auto svo = svo_new(k_svo);
+  Accs accs;
+  const E eval = virtual_claim(L, polys, k_svo, pt, svo, accs);
+  const SvoClaim whole = svo_eval(k_svo, pt, stack_polys(L, polys));
+  const Accs whole_accs = svo_accumulators(svo, {&whole}, {eone()});
+  return eval == whole.eval && eval == eval_base(stack_polys(L, polys), pt) && accs == whole_accs;
+}
+
 ProveResult whir_prove(const Params& p, const std::vector<F>& poly, int n_points) {
+  // one registered poly, each claim at a fresh point (test.rs:218-227)
+  return whir_prove_stacked(p, {poly}, std::vector<StackClaim>((size_t)n_points, StackClaim{0, 1}));
+}
+
+ProveResult whir_prove_stacked(const Params& p, const std::vector<std::vector<F>>& polys,
+                               const std::vector<StackClaim>& claims) {
   const Hashers& H = Hashers::get();
   Writer w(&H.p16);
-  const int k = p.k, f = p.folding;
-  if ((size_t)1 << k != poly.size() || f > k) throw std::runtime_error("bad shape");
+  const int f = p.folding;
+  std::vector<int> kp;
+  for (auto& q : polys) kp.push_back(log2_exact(q.size()));
+  const StackLayout L = stack_layout(kp, f);
+  if (L.k != p.k) throw std::runtime_error("stacked k differs from the WHIR k");  // whir.rs:185
+  check_claims(kp, claims, f);
+  const int k = p.k;
+  const std::vector<F> poly = stack_polys(L, polys);
 
-  // ---- commit (whir.rs:127-138, 19-36) ----
+  // ---- commit the stacked vector (whir.rs:127-138, 19-36) ----
   Tree base = commit_rows(poly, (size_t)1 << (k - f), (size_t)1 << f, (size_t)1 << (k + p.rate - f));
   w.write_digest(base.root().d);
 
-  // ---- evaluation claims (test.rs:223-227; prover.rs:64-90) ----
-  std::vector<SvoClaim> claims;
-  for (int i = 0; i < n_points; ++i) {
-    Point pt = expand(k, w.draw_e());
-    claims.push_back(svo_eval(f, pt, poly));
-    w.write_e(claims.back().eval);
+  // ---- evaluation claims (ProverStack::eval, prover.rs:64-90; draws as in
+  // test.rs:223-227 and sumcheck/test.rs:113-121) ----
+  std::vector<SvoClaim> cl;
+  Point pt;
+  for (const StackClaim& c : claims) {
+    if (c.new_point) pt = expand(kp[c.poly], w.draw_e());
+    cl.push_back(svo_eval(f, pt, polys[c.poly]));
+    w.write_e(cl.back().eval);
   }
 
   // ---- open: OOD virtual claims (whir.rs:187-192; prover.rs:231-277) ----
@@ -709,31 +846,30 @@ ProveResult whir_prove(const Params& p, const std::vector<F>& poly, int n_points
   std::vector<Accs> vaccs;
   int n_ood0 = n_ood_queries(p.soundness, p.security_level, k, p.rate, EXT_BITS);
   for (int i = 0; i < n_ood0; ++i) {
-    Point pt = expand(k, w.draw_e());
-    vclaims.push_back(svo_eval(f, pt, poly));
-    vaccs.push_back(svo_accumulators(svo, {&vclaims.back()}, {eone()}));
+    const Point vp = expand(k, w.draw_e());
+    SvoClaim vc;
+    vc.svo.assign(vp.begin(), vp.begin() + f);
+    vc.rest.assign(vp.begin() + f, vp.end());
+    vaccs.emplace_back();
+    vc.eval = virtual_claim(L, polys, f, vp, svo, vaccs.back());
+    vclaims.push_back(std::move(vc));
     w.write_e(vclaims.back().eval);
   }
 
   // ---- new_prover: SVO rounds (prover.rs:279-353) ----
   Prover sc;
   E alpha = w.draw_e();
-  std::vector<E> alphas;
+  const std::vector<size_t> rank = stack_claim_ranks(L, claims);
+  std::vector<E> alphas(cl.size());
+  for (size_t c = 0; c < cl.size(); ++c) alphas[c] = epow(alpha, rank[c]);
   {
-    E a = eone();
-    for (size_t i = 0; i < claims.size(); ++i) {
-      alphas.push_back(a);
-      a = emul(a, alpha);
-    }
-  }
-  {
-    std::vector<E> ev;
-    for (auto& c : claims) ev.push_back(c.eval);
+    std::vector<E> ev(cl.size());  // claims in layout order, then the virtual ones (prover.rs:168-174)
+    for (size_t c = 0; c < cl.size(); ++c) ev[rank[c]] = cl[c].eval;
     for (auto& c : vclaims) ev.push_back(c.eval);
     sc.sum = horner(ev, alpha);
   }
   std::vector<Accs> caccs;
-  for (size_t i = 0; i < claims.size(); ++i) caccs.push_back(svo_accumulators(svo, {&claims[i]}, {alphas[i]}));
+  for (size_t i = 0; i < cl.size(); ++i) caccs.push_back(svo_accumulators(svo, {&cl[i]}, {alphas[i]}));
   Point rs;
   for (int ri = 0; ri < f; ++ri) {
     std::vector<E> wts = lagrange_weights_012_multi(rs);
@@ -743,7 +879,7 @@ ProveResult whir_prove(const Params& p, const std::vector<F>& poly, int n_points
         acc0[u] = eadd(acc0[u], a[ri][0][u]);
         acc2[u] = eadd(acc2[u], a[ri][1][u]);
       }
-    E av = epow(alpha, claims.size());
+    E av = epow(alpha, cl.size());
     for (auto& a : vaccs) {
       for (size_t u = 0; u < wts.size(); ++u) {
         acc0[u] = eadd(acc0[u], emul(av, a[ri][0][u]));
@@ -762,20 +898,33 @@ ProveResult whir_prove(const Params& p, const std::vector<F>& poly, int n_points
     sc.sum = extrapolate_012(c0, esub(sc.sum, c0), c2, r);
     rs.push_back(r);
   }
-  // expression (prover.rs:184-229)
-  sc.poly = compress_lo(poly, rs);
+  // ---- expression (prover.rs:184-229) ----
+  // compress_stacked: each poly compressed into its block at selector.index << (k_poly - f)
+  sc.poly.assign((size_t)1 << (k - f), ezero());
+  for (int id : L.order) {
+    const std::vector<E> blk = compress_lo(polys[id], rs);
+    std::copy(blk.begin(), blk.end(), sc.poly.begin() + (L.sel_index[id] << (kp[id] - f)));
+  }
   sc.weights.assign(sc.poly.size(), ezero());
   {
-    E ai = eone();
-    auto add_claim = [&](const SvoClaim& c) {
-      E scale = emul(ai, eval_eq_xy(c.svo, rs));
+    auto add_eq = [&](const SvoClaim& c, E scale_a, size_t off) {
+      const E scale = emul(scale_a, eval_eq_xy(c.svo, rs));
       std::vector<E> eq = eq_table(c.rest, scale);
 #pragma omp parallel for
-      for (long long x = 0; x < (long long)eq.size(); ++x) sc.weights[x] = eadd(sc.weights[x], eq[x]);
-      ai = emul(ai, alpha);
+      for (long long x = 0; x < (long long)eq.size(); ++x) sc.weights[off + x] = eadd(sc.weights[off + x], eq[x]);
     };
-    for (auto& c : claims) add_claim(c);
-    for (auto& c : vclaims) add_claim(c);
+    // claims in layout order, each into its poly's block (prover.rs:190-199)
+    E ai = eone();
+    for (int id : L.order)
+      for (size_t c = 0; c < cl.size(); ++c)
+        if (claims[c].poly == id) {
+          add_eq(cl[c], ai, L.sel_index[id] << (kp[id] - f));
+          ai = emul(ai, alpha);
+        }
+    for (auto& c : vclaims) {  // virtual claims over the whole vector (prover.rs:201-207)
+      add_eq(c, ai, 0);
+      ai = emul(ai, alpha);
+    }
   }
   sc.rs = rs;
   sc.alpha = alpha;
@@ -879,27 +1028,44 @@ static int reduce(Reader& r, E& sum, E& out) {  // verifier.rs:8-20
 }
 
 int whir_verify(const Params& p, const uint8_t* proof, size_t len, int n_points, F& checkpoint) {
-  // whir.rs:361-522 with the test's claim reading (test.rs:237-250).
+  // the test's claim reading (test.rs:237-250): one poly, a fresh point per claim
+  return whir_verify_stacked(p, proof, len, {p.k}, std::vector<StackClaim>((size_t)n_points, StackClaim{0, 1}),
+                             checkpoint);
+}
+
+int whir_verify_stacked(const Params& p, const uint8_t* proof, size_t len, const std::vector<int>& k_polys,
+                        const std::vector<StackClaim>& claims, F& checkpoint) {
+  // whir.rs:361-522 with VerifierStack::read_eval / VerifierLayout (verifier.rs:100-212).
   const Hashers& H = Hashers::get();
   Reader r(&H.p16, proof, len);
   const int k = p.k, f = p.folding;
+  const StackLayout L = stack_layout(k_polys, f);
+  if (L.k != k) return ERR_VERIFY;
+  check_claims(k_polys, claims, f);
   Digest comm;
   TRY(r.read_digest(comm.d));
-  std::vector<Point> pts;
-  std::vector<E> evals;
-  for (int i = 0; i < n_points; ++i) {
-    Point pt = expand(k, r.draw_e());
+  const std::vector<size_t> rank = stack_claim_ranks(L, claims);
+  std::vector<Point> pts(claims.size());
+  std::vector<E> evals(claims.size());
+  Point pt;
+  for (size_t c = 0; c < claims.size(); ++c) {
+    const int id = claims[c].poly;
+    if (claims[c].new_point) pt = expand(k_polys[id], r.draw_e());
     E ev;
     TRY(r.read_e(ev));
-    pts.push_back(pt);
-    evals.push_back(ev);
+    // lift: point ++ selector point (mod.rs:55-57), stored in layout order (verifier.rs:204-212)
+    Point lifted = pt;
+    const Point sel = hypercube(L.sel_index[id], L.sel_k[id]);
+    lifted.insert(lifted.end(), sel.begin(), sel.end());
+    pts[rank[c]] = std::move(lifted);
+    evals[rank[c]] = ev;
   }
   int n_ood0 = n_ood_queries(p.soundness, p.security_level, k, p.rate, EXT_BITS);
   for (int i = 0; i < n_ood0; ++i) {  // verifier.rs:176-189
-    Point pt = expand(k, r.draw_e());
+    Point vp = expand(k, r.draw_e());
     E ev;
     TRY(r.read_e(ev));
-    pts.push_back(pt);
+    pts.push_back(vp);
     evals.push_back(ev);
   }
   // SumcheckVerifier::new (verifier.rs:227-255)

@@ -2,7 +2,8 @@
 //
 // CPU restatement of the reference's WHIR prover and verifier for one
 // multilinear polynomial with n evaluation claims (the path exercised by
-// `run_whir_poseidon`, crates/whir/src/test.rs:174-254):
+// `run_whir_poseidon`, crates/whir/src/test.rs:174-254), and for a stack of
+// several polynomials committed as one (ProverStack / ProverLayout):
 //   params      crates/whir/src/params.rs:6-202, whir.rs:140-166
 //   poly / eq   crates/poly/src/{point,poly,eq,lib}.rs
 //   DFT         Plonky3 Radix2DFTSmallBatch semantics (natural order, subgroup)
@@ -79,6 +80,35 @@ struct ProveResult {
 };
 ProveResult whir_prove(const Params& p, const std::vector<F>& poly, int n_points);
 int whir_verify(const Params& p, const uint8_t* proof, size_t len, int n_points, F& checkpoint);
+
+// ---- multi-polynomial stacking (sumcheck/prover.rs:31-124,137-277; verifier.rs:100-212; mod.rs:30-58) ----
+// One evaluation claim: the poly it is on and whether it draws a fresh point
+// (expand(k_poly, draw())) or reuses the previous claim's point (then the two
+// polys have the same k). The list is in transcript order.
+struct StackClaim {
+  int poly;
+  int new_point;
+};
+struct StackLayout {
+  int k = 0;                      // log2_ceil(sum of poly sizes)
+  std::vector<size_t> offset;     // per poly id: first index in the stacked vector
+  std::vector<int> sel_k;         // per poly id: selector variables, k - k_poly
+  std::vector<size_t> sel_index;  // per poly id: selector index, offset >> k_poly
+  std::vector<int> order;         // poly ids in Selector order (the BTreeMap's iteration)
+};
+bool selector_less(int ka, size_t ia, int kb, size_t ib);
+StackLayout stack_layout(const std::vector<int>& k_polys, int k_svo);
+std::vector<F> stack_polys(const StackLayout& L, const std::vector<std::vector<F>>& polys);
+// alpha exponent of each claim: its position in claims_in_layout_order (prover.rs:155-174)
+std::vector<size_t> stack_claim_ranks(const StackLayout& L, const std::vector<StackClaim>& claims);
+ProveResult whir_prove_stacked(const Params& p, const std::vector<std::vector<F>>& polys,
+                               const std::vector<StackClaim>& claims);
+int whir_verify_stacked(const Params& p, const uint8_t* proof, size_t len, const std::vector<int>& k_polys,
+                        const std::vector<StackClaim>& claims, F& checkpoint);
+// write_virtual_eval's value and SVO accumulators computed per stacked poly
+// (prover.rs:241-256) against one claim on the stacked vector (its debug check,
+// prover.rs:258-266): returns 1 when both agree.
+int stack_virtual_check(const std::vector<std::vector<F>>& polys, int k_svo, E x);
 
 std::vector<F> rand_poly(uint64_t seed, int k);  // Poly::<F>::rand(&mut rng(seed), k)
 
