@@ -157,6 +157,34 @@ int whir_prover_enqueue(whir_prover* prover, const uint32_t* d_poly, void* strea
 int whir_prover_result(whir_prover* prover, uint8_t* proof, size_t cap, size_t* len, uint32_t* checkpoint,
                        uint32_t* root8);
 
+/* ---- multi-polynomial stacking ----
+ * Several multilinear polys committed and opened as one (ProverStack::register /
+ * eval / layout, crates/whir/src/sumcheck/prover.rs:31-124; VerifierStack /
+ * VerifierLayout, verifier.rs:100-212; Selector, mod.rs:30-58). The stacked
+ * vector is the layout's poly (prover.rs:114-118): polys sorted by size, the
+ * largest first (for equal sizes the last registered first), zero-padded to
+ * 2^k, k = ceil(log2(sum of sizes)). The transcript is: the stacked
+ * commitment's root, then each evaluation claim in order (a fresh point
+ * expand(k_poly, draw()) or the previous claim's point, then its value), then
+ * Whir::open(.., layout) (whir.rs:168-359). */
+typedef struct {
+  int32_t n_polys;
+  const int32_t* k_polys;      /* log2 size of each registered poly, registration order; >= folding */
+  int32_t n_claims;
+  const int32_t* claim_poly;   /* poly id of each evaluation claim, transcript order */
+  const uint8_t* claim_fresh;  /* 1: the claim draws its point; 0: the previous claim's point (same size) */
+} whir_stack;
+
+/* params->k must be the stacked k; params->n_points is ignored (n_claims is used). */
+int whir_prover_create_stacked(whir_ctx* ctx, const whir_params* params, const whir_stack* stack,
+                               whir_prover** out);
+/* Offset (words) of poly `id` in whir_prover_input(prover); -1 if out of range. */
+int64_t whir_prover_stack_offset(const whir_prover* prover, int32_t id);
+/* Lays out d_polys[id] (2^k_polys[id] words each, device) into the prover's
+ * stacked input with the zero tail (prover.rs:114-118), then enqueues the
+ * proof of that vector as whir_prover_enqueue does. */
+int whir_prover_enqueue_stacked(whir_prover* prover, const uint32_t* const* d_polys, void* stream);
+
 /* Capture the whole proof (kernels + result copies) as a hipGraph on the first
  * enqueue and replay it afterwards; the graph proves the prover's own input
  * buffer (other d_poly are first copied into it on the stream). */

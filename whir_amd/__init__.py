@@ -20,6 +20,7 @@ from .api import (  # noqa: F401
     Context,
     Prover,
     Server,
+    StackedProver,
     Whir,
     params,
     prover_bytes_for,

@@ -36,6 +36,16 @@ class whir_params(ctypes.Structure):
     ]
 
 
+class whir_stack(ctypes.Structure):
+    _fields_ = [
+        ("n_polys", ctypes.c_int32),
+        ("k_polys", ctypes.POINTER(ctypes.c_int32)),
+        ("n_claims", ctypes.c_int32),
+        ("claim_poly", ctypes.POINTER(ctypes.c_int32)),
+        ("claim_fresh", ctypes.POINTER(ctypes.c_uint8)),
+    ]
+
+
 # (name, restype, argtypes) for every entry point declared in include/whir_gpu.h
 _vp = ctypes.c_void_p
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -65,6 +75,10 @@ SIGNATURES = [
     ("whir_prover_enqueue", ctypes.c_int, [_vp, _vp, _vp]),
     ("whir_prover_result", ctypes.c_int,
      [_vp, _u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
+    ("whir_prover_create_stacked", ctypes.c_int,
+     [_vp, ctypes.POINTER(whir_params), ctypes.POINTER(whir_stack), ctypes.POINTER(_vp)]),
+    ("whir_prover_stack_offset", ctypes.c_int64, [_vp, ctypes.c_int32]),
+    ("whir_prover_enqueue_stacked", ctypes.c_int, [_vp, ctypes.POINTER(_vp), _vp]),
     ("whir_prover_set_graph", ctypes.c_int, [_vp, ctypes.c_int]),
     ("whir_prover_query", ctypes.c_int, [_vp]),
     ("whir_prover_device_bytes", ctypes.c_size_t, [_vp]),

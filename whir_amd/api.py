@@ -3,7 +3,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import check, lib, whir_params
+from ._lib import check, lib, whir_params, whir_stack
 
 UNIQUE_DECODING, JOHNSON_BOUND, CAPACITY_BOUND = 0, 1, 2
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -298,6 +298,54 @@ class Prover:
             self.close()
         except Exception:
             pass
+
+
+class StackedProver(Prover):
+    """Device workspace for one stacked proof shape: `k_polys` registered polys
+    (ProverStack::register, crates/whir/src/sumcheck/prover.rs:59-62) and
+    evaluation claims `claims` = [(poly id, fresh point), ...] in transcript
+    order (ProverStack::eval, prover.rs:64-90). ``params.k`` must be the
+    stacked k; the proof is Whir::open over the stack's layout (whir.rs:168-359)
+    after the commitment of the stacked vector."""
+
+    def __init__(self, params, k_polys, claims, ctx=None):
+        self.ctx = ctx or default_context()
+        self.params = params
+        self.k_polys = [int(k) for k in k_polys]
+        kp = np.asarray(self.k_polys, np.int32)
+        cp = np.asarray([int(c[0]) for c in claims], np.int32)
+        cf = np.asarray([1 if c[1] else 0 for c in claims], np.uint8)
+        st = whir_stack(len(kp), kp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(cp),
+                        cp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cf.ctypes.data_as(_u8p))
+        p = ctypes.c_void_p()
+        check(lib().whir_prover_create_stacked(self.ctx.ptr, ctypes.byref(params), ctypes.byref(st),
+                                               ctypes.byref(p)), "whir_prover_create_stacked")
+        self.ptr = p.value
+        self.capacity = lib().whir_prover_proof_capacity(self.ptr)
+        self.input_ptr = lib().whir_prover_input(self.ptr)
+
+    def offset(self, poly_id):
+        return lib().whir_prover_stack_offset(self.ptr, poly_id)
+
+    def upload(self, polys):
+        """Copies the polys to the device and lays them out with the zero tail."""
+        bufs = []
+        for kp, poly in zip(self.k_polys, polys):
+            a = _u32(poly)
+            assert a.size == 1 << kp
+            b = DeviceBuffer(a.nbytes)
+            b.upload(a)
+            bufs.append(b)
+        ptrs = (ctypes.c_void_p * len(bufs))(*[b.ptr for b in bufs])
+        self._staged = (bufs, ptrs)
+
+    def enqueue(self, d_polys=None, stream=None):
+        """Lays out device polys (default: the staged upload) and enqueues the proof."""
+        if d_polys is None:
+            ptrs = self._staged[1]
+        else:
+            ptrs = (ctypes.c_void_p * len(d_polys))(*d_polys)
+        check(lib().whir_prover_enqueue_stacked(self.ptr, ptrs, stream), "whir_prover_enqueue_stacked")
 
 
 class Whir:

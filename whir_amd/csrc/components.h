@@ -205,12 +205,12 @@ extern "C" int whir_compress_hi(whir_ctx* c, const uint32_t* d_poly, uint32_t k,
   const unsigned grid = 1u << m1;
   const Ext* eq1 = pool + (1ull << m0);
   switch (k - m) {
-    case 0: hipLaunchKernelGGL(compress_hi_kernel<0>, dim3(grid), dim3(SC_THREADS), 0, s, d_poly, pool, eq1, m0, partials); break;
-    case 1: hipLaunchKernelGGL(compress_hi_kernel<1>, dim3(grid), dim3(SC_THREADS), 0, s, d_poly, pool, eq1, m0, partials); break;
-    case 2: hipLaunchKernelGGL(compress_hi_kernel<2>, dim3(grid), dim3(SC_THREADS), 0, s, d_poly, pool, eq1, m0, partials); break;
-    case 3: hipLaunchKernelGGL(compress_hi_kernel<3>, dim3(grid), dim3(SC_THREADS), 0, s, d_poly, pool, eq1, m0, partials); break;
-    case 4: hipLaunchKernelGGL(compress_hi_kernel<4>, dim3(grid), dim3(SC_THREADS), 0, s, d_poly, pool, eq1, m0, partials); break;
-    default: hipLaunchKernelGGL(compress_hi_kernel<5>, dim3(grid), dim3(SC_THREADS), 0, s, d_poly, pool, eq1, m0, partials); break;
+    case 0: hipLaunchKernelGGL(compress_hi_kernel<0>, dim3(grid), dim3(SC_THREADS), 0, s, d_poly, pool, eq1, m0, partials, nullptr, 0u); break;
+    case 1: hipLaunchKernelGGL(compress_hi_kernel<1>, dim3(grid), dim3(SC_THREADS), 0, s, d_poly, pool, eq1, m0, partials, nullptr, 0u); break;
+    case 2: hipLaunchKernelGGL(compress_hi_kernel<2>, dim3(grid), dim3(SC_THREADS), 0, s, d_poly, pool, eq1, m0, partials, nullptr, 0u); break;
+    case 3: hipLaunchKernelGGL(compress_hi_kernel<3>, dim3(grid), dim3(SC_THREADS), 0, s, d_poly, pool, eq1, m0, partials, nullptr, 0u); break;
+    case 4: hipLaunchKernelGGL(compress_hi_kernel<4>, dim3(grid), dim3(SC_THREADS), 0, s, d_poly, pool, eq1, m0, partials, nullptr, 0u); break;
+    default: hipLaunchKernelGGL(compress_hi_kernel<5>, dim3(grid), dim3(SC_THREADS), 0, s, d_poly, pool, eq1, m0, partials, nullptr, 0u); break;
   }
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, partials, grid, W, (Ext*)d_out);
   HIP_TRY(hipFreeAsync(partials, s));

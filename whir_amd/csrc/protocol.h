@@ -14,7 +14,7 @@ namespace kb {
 
 constexpr int MAX_K = 32;
 constexpr int MAX_F = 5;
-constexpr int MAX_SVO_CLAIMS = 10;
+constexpr int MAX_CLAIMS = 1024;  // evaluation + virtual (OOD) claims of one proof
 constexpr int MAX_STIR = 1024;
 constexpr int SERIAL_THREADS = 256;
 
@@ -92,10 +92,11 @@ __global__ void tx_init_kernel(DevTx* t, JobScalars* js, uint32_t cap) {
 // Where the next claim's draw goes (a claim kernel can finish one claim and
 // begin the next in the same launch).
 struct ClaimBegin {
-  ClaimSlot* slot;  // nullptr: nothing to begin
+  ClaimSlot* slot;  // first slot of the group; nullptr: nothing to begin
   Ext* eq0;
   Ext* eq1;
   uint32_t k, fsv, m0, m1;
+  uint32_t count;   // claims sharing the point (slots slot[0 .. count))
 };
 
 struct ClaimScratch {
@@ -112,13 +113,13 @@ __device__ void claim_begin_body(WaveTx& tx, const ClaimBegin& cb, ClaimScratch&
     if (threadIdx.x == 0) {
       for (uint32_t i = 0; i < cb.k; ++i) {
         sc.pt[i] = z;
-        cb.slot->point[i] = z;
         z = esqr(z);
       }
-      cb.slot->k = cb.k;
     }
   }
   __syncthreads();
+  for (uint32_t it = threadIdx.x; it < cb.count * cb.k; it += blockDim.x) cb.slot[it / cb.k].point[it % cb.k] = sc.pt[it % cb.k];
+  for (uint32_t c = threadIdx.x; c < cb.count; c += blockDim.x) cb.slot[c].k = cb.k;
   build_eq_block(cb.eq0, sc.pt + cb.fsv, (int)cb.m0, eone(), sc.tl, sc.th);
   build_eq_block(cb.eq1, sc.pt + cb.fsv + cb.m0, (int)cb.m1, eone(), sc.tl, sc.th);
 }
@@ -152,60 +153,70 @@ __global__ void __launch_bounds__(SERIAL_THREADS) claim_begin_kernel(DevTx* t, u
   if (is_wave0()) tx_end(tx, t, js);
 }
 
-// Reduce the evaluation partials and write the claim's value (prover.rs:87,276;
-// whir.rs:230). fsv > 0: partials hold 2^fsv compressed values per block
-// (SvoPoint::eval, svo.rs:77-91); fsv == 0: one value per block (eval_ext).
-// Then begin the next claim when `next.slot` is set.
+// Reduce the evaluation partials of a group of `count` claims drawn at one
+// point and write their values in order (prover.rs:64-90,276; whir.rs:230).
+// fsv > 0: claim c's partials hold 2^fsv compressed values per block, at
+// partials[(c * n_part + p) * 2^fsv + i] (SvoPoint::eval, svo.rs:77-91);
+// fsv == 0: one value per block (eval_ext). Then begin the next group when
+// `next.slot` is set.
 __global__ void __launch_bounds__(SERIAL_THREADS) claim_end_kernel(DevTx* t, uint32_t* log, JobScalars* js,
-                                                                   ClaimSlot* slot, const Ext* partials,
-                                                                   uint32_t n_part, uint32_t fsv, ClaimBegin next) {
+                                                                   ClaimSlot* slot, uint32_t count,
+                                                                   const Ext* partials, uint32_t n_part, uint32_t fsv,
+                                                                   ClaimBegin next) {
   __shared__ Ext sh[SERIAL_THREADS];
-  __shared__ Ext s_comp[1 << MAX_F];
   __shared__ Ext s_eq[1 << MAX_F];
+  __shared__ Ext s_eval[MAX_CLAIMS];
   __shared__ ClaimScratch sc;
   const uint32_t W = 1u << fsv;
   const uint32_t tid = threadIdx.x;
-  const uint32_t lanes_per = SERIAL_THREADS / W;
-  {
-    const uint32_t i = tid % W, p0 = tid / W;
+  // eq(svo)[i] for i < W, svo = point[0..fsv) (the group's shared point)
+  if (tid < W) {
+    Ext e = eone();
+    for (uint32_t j = 0; j < fsv; ++j) {
+      const Ext z = slot[0].point[j];
+      e = emul(e, ((tid >> j) & 1) ? z : esub(eone(), z));
+    }
+    s_eq[tid] = e;
+  }
+  // items (claim, column): `lanes` threads per item, a power of two; a pass
+  // holds whole claims (W divides the items of a pass)
+  const uint32_t items = count * W;
+  uint32_t lanes = 1;
+  while (lanes * 2 * items <= SERIAL_THREADS) lanes *= 2;
+  const uint32_t per_pass = SERIAL_THREADS / lanes;
+  for (uint32_t base = 0; base < items; base += per_pass) {
+    const uint32_t item = base + tid / lanes, lane = tid % lanes;
     Ext acc = ezero();
-    for (uint32_t p = p0; p < n_part; p += lanes_per) acc = eadd(acc, ld_ext(partials + (uint64_t)p * W + i));
+    if (item < items) {
+      const Ext* pc = partials + (uint64_t)(item / W) * n_part * W + (item % W);
+      for (uint32_t p = lane; p < n_part; p += lanes) acc = eadd(acc, ld_ext(pc + (uint64_t)p * W));
+    }
     sh[tid] = acc;
     __syncthreads();
-    if (tid < W) {
-      Ext s = ezero();
-      for (uint32_t r = 0; r < lanes_per; ++r) s = eadd(s, sh[r * W + tid]);
-      s_comp[tid] = s;
-      slot->compressed[tid] = s;
+    Ext comp = ezero();
+    if (lane == 0 && item < items) {
+      comp = sh[tid];
+      for (uint32_t r = 1; r < lanes; ++r) comp = eadd(comp, sh[tid + r]);
+      slot[item / W].compressed[item % W] = comp;
     }
     __syncthreads();
-  }
-  Ext eval;
-  if (fsv == 0) {
-    eval = s_comp[0];
-  } else {
-    if (tid < 64) {
-      // eq(svo)[i] for i < W, svo = point[0..fsv)
-      for (uint32_t i = tid; i < W; i += 64) {
-        Ext e = eone();
-        for (uint32_t j = 0; j < fsv; ++j) {
-          const Ext z = slot->point[j];
-          e = emul(e, ((i >> j) & 1) ? z : esub(eone(), z));
-        }
-        s_eq[i] = e;
-      }
+    if (lane == 0) sh[tid] = emul(comp, s_eq[item % W]);
+    __syncthreads();
+    // one thread per claim of the pass: eval = sum_i eq(svo)[i] * compressed[i]
+    const uint32_t c_in = tid;  // claim within the pass
+    if (c_in < per_pass / W && base + c_in * W < items) {
+      Ext e = ezero();
+      for (uint32_t i = 0; i < W; ++i) e = eadd(e, sh[(c_in * W + i) * lanes]);
+      const uint32_t c = base / W + c_in;
+      s_eval[c] = e;
+      slot[c].eval = e;
     }
     __syncthreads();
-    Ext part = ezero();
-    if (tid < 64)
-      for (uint32_t i = tid; i < W; i += 64) part = eadd(part, emul(s_comp[i], s_eq[i]));
-    eval = wave_sum_ext(part);  // meaningful in wave 0
   }
   WaveTx tx;
   if (is_wave0()) {
-    if (tid == 0) slot->eval = eval;
     tx_begin(tx, t, log, js);
-    tx.write_e(eval);
+    for (uint32_t c = 0; c < count; ++c) tx.write_e(s_eval[c]);
   }
   if (next.slot) claim_begin_body(tx, next, sc);
   if (is_wave0()) tx_end(tx, t, js);
@@ -214,22 +225,29 @@ __global__ void __launch_bounds__(SERIAL_THREADS) claim_end_kernel(DevTx* t, uin
 // ProverLayout::new_prover (prover.rs:279-353) with Svo::calculate_accumulators
 // (svo.rs:224-278) and the expression scales (prover.rs:184-208):
 //   draw alpha; f SVO rounds (write c0, c2; draw r); eq(rs) into js->eq_rs;
-//   eq1 tables of every claim scaled by alpha^c * eq(svo_c, rs).
-// Claims are in layout order (user claims, then the OOD "virtual" claims).
+//   eq1 tables of every claim scaled by alpha^rank * eq(svo_c, rs).
+// The accumulators are linear in the claims' partial evaluations, so a group's
+// claims (one point) are combined with their alpha powers first and the
+// accumulators are taken once per group, as the reference does per point
+// (prover.rs:300-313). Claim c has alpha power alpha^desc[c].rank (user claims
+// in layout order, then the virtual claims).
+// Scratch: pe_buf (n_claims x PE_PER), gpe_buf (n_groups x PE_PER); eq1_pref:
+// prefix sums of the claims' eq1 sizes (n_claims + 1).
 __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t* log, JobScalars* js,
-                                                             const ClaimSlot* slots, uint32_t n_claims, uint32_t f,
-                                                             Ext* pool, uint64_t pool_stride, uint32_t m0,
-                                                             uint32_t m1) {
-  // partial evals pe[c][2^l - 2 + a] (l = 1..f), accumulators acc[c][ri][side][u]
+                                                             const ClaimSlot* slots, const ClaimDesc* desc,
+                                                             uint32_t n_claims, const uint32_t* group_start,
+                                                             uint32_t n_groups, uint32_t f, Ext* pool, Ext* pe_buf,
+                                                             Ext* gpe_buf, const uint32_t* eq1_pref) {
+  // partial evals pe[c][2^l - 2 + a] (l = 1..f), accumulators tot[ri][side][u]
   constexpr int PE_PER = (2 << MAX_F) - 2;
   constexpr int ACC_PER = 243 - 1;  // sum_{ri<5} 2 * 3^ri
-  __shared__ Ext s_pe[MAX_SVO_CLAIMS * PE_PER];
-  __shared__ Ext s_acc[MAX_SVO_CLAIMS * ACC_PER];
   __shared__ Ext s_tot[ACC_PER];
-  __shared__ Ext s_apow[MAX_SVO_CLAIMS];
+  __shared__ Ext s_part[SERIAL_THREADS];
+  __shared__ Ext s_apow[MAX_CLAIMS];  // alpha^rank, then the expression scales
+  __shared__ uint32_t s_pref[MAX_CLAIMS + 1];
   __shared__ Ext s_rs[MAX_F];
   __shared__ Ext s_lag[MAX_F][3];
-  __shared__ Ext s_scale[MAX_SVO_CLAIMS];
+  __shared__ Ext s_alpha;
   const uint32_t tid = threadIdx.x;
   const uint32_t W = 1u << f;
   uint32_t acc_per = 0, pow3 = 1;
@@ -255,76 +273,90 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t*
       }
       s = eadd(s, emul(sl.compressed[(b << l) + a], e));
     }
-    s_pe[c * PE_PER + o] = s;
+    pe_buf[(uint64_t)c * PE_PER + o] = s;
   }
-  __syncthreads();
+  for (uint32_t c = tid; c <= n_claims; c += SERIAL_THREADS) s_pref[c] = eq1_pref[c];
 
-  // (2) accumulators per claim (weight 1)
-  for (uint32_t it = tid; it < n_claims * acc_per; it += SERIAL_THREADS) {
-    const uint32_t c = it / acc_per;
-    uint32_t o = it % acc_per;
-    uint32_t ri = 0, p3 = 1;
-    while (o >= 2 * p3) {
-      o -= 2 * p3;
-      p3 *= 3;
-      ++ri;
-    }
-    const uint32_t side = o / p3, u = o % p3;
-    const uint32_t l = ri + 1;
-    uint32_t pt[MAX_F];
-    uint32_t uu = u;
-    for (uint32_t j = 0; j < ri; ++j) {
-      pt[j] = uu % 3;
-      uu /= 3;
-    }
-    pt[ri] = side ? 2u : 0u;
-    const ClaimSlot& sl = slots[c];
-    // eval_eq_xy(pt, svo[0..l)) with pt_j in {0,1,2}: 1-y, y, 3y-1
-    Ext exy = eone();
-    for (uint32_t j = 0; j < l; ++j) {
-      const Ext y = sl.point[j];
-      Ext term;
-      if (pt[j] == 0) term = esub(eone(), y);
-      else if (pt[j] == 1) term = y;
-      else term = esub(eadd(edbl(y), y), eone());
-      exy = emul(exy, term);
-    }
-    // <pe[l], eq(pt)>; eq(pt)[a] = prod_j (a_j ? pt_j : 1 - pt_j), small integers
-    Ext dot = ezero();
-    for (uint32_t a = 0; a < (1u << l); ++a) {
-      int32_t coef = 1;
-      for (uint32_t j = 0; j < l; ++j) coef *= ((a >> j) & 1) ? (int32_t)pt[j] : 1 - (int32_t)pt[j];
-      if (coef == 0) continue;
-      const uint32_t mag = from_canonical((uint32_t)(coef < 0 ? -coef : coef));
-      const Ext term = emul_base(s_pe[c * PE_PER + (1u << l) - 2 + a], mag);
-      dot = coef < 0 ? esub(dot, term) : eadd(dot, term);
-    }
-    s_acc[c * ACC_PER + (it % acc_per)] = emul(exy, dot);
-  }
-  __syncthreads();
-
-  // (3) draw alpha, powers alpha^c
+  // (2) draw alpha; alpha^rank per claim
   WaveTx tx;
   if (is_wave0()) {
     tx_begin(tx, t, log, js);
     const Ext alpha = tx.sample_ext();
     if (tid == 0) {
-      Ext a = eone();
-      for (uint32_t c = 0; c < n_claims; ++c) {
-        s_apow[c] = a;
-        a = emul(a, alpha);
-      }
+      s_alpha = alpha;
       js->alpha = alpha;
     }
   }
   __syncthreads();
-  // (4) combined accumulators
-  for (uint32_t o = tid; o < acc_per; o += SERIAL_THREADS) {
+  for (uint32_t c = tid; c < n_claims; c += SERIAL_THREADS) s_apow[c] = epow(s_alpha, desc[c].rank);
+  __syncthreads();
+
+  // (3) per group: gpe[g][o] = sum_{c in g} alpha^rank_c * pe[c][o]
+  for (uint32_t it = tid; it < n_groups * pe_per; it += SERIAL_THREADS) {
+    const uint32_t g = it / pe_per, o = it % pe_per;
     Ext s = ezero();
-    for (uint32_t c = 0; c < n_claims; ++c) s = eadd(s, emul(s_apow[c], s_acc[c * ACC_PER + o]));
-    s_tot[o] = s;
+    for (uint32_t c = group_start[g]; c < group_start[g + 1]; ++c)
+      s = eadd(s, emul(s_apow[c], pe_buf[(uint64_t)c * PE_PER + o]));
+    gpe_buf[(uint64_t)g * PE_PER + o] = s;
   }
   __syncthreads();
+
+  // (4) accumulators, summed over groups: `chunks` threads per entry split the groups
+  const uint32_t chunks = acc_per < SERIAL_THREADS ? SERIAL_THREADS / acc_per : 1;
+  for (uint32_t o0 = 0; o0 < acc_per; o0 += SERIAL_THREADS / chunks) {
+    const uint32_t oi = o0 + tid % (SERIAL_THREADS / chunks), ch = tid / (SERIAL_THREADS / chunks);
+    Ext total = ezero();
+    if (oi < acc_per && ch < chunks) {
+      uint32_t o = oi, ri = 0, p3 = 1;
+      while (o >= 2 * p3) {
+        o -= 2 * p3;
+        p3 *= 3;
+        ++ri;
+      }
+      const uint32_t side = o / p3, u = o % p3;
+      const uint32_t l = ri + 1;
+      uint32_t pt[MAX_F];
+      uint32_t uu = u;
+      for (uint32_t j = 0; j < ri; ++j) {
+        pt[j] = uu % 3;
+        uu /= 3;
+      }
+      pt[ri] = side ? 2u : 0u;
+      for (uint32_t g = ch; g < n_groups; g += chunks) {
+        const ClaimSlot& sl = slots[group_start[g]];
+        // eval_eq_xy(pt, svo[0..l)) with pt_j in {0,1,2}: 1-y, y, 3y-1
+        Ext exy = eone();
+        for (uint32_t j = 0; j < l; ++j) {
+          const Ext y = sl.point[j];
+          Ext term;
+          if (pt[j] == 0) term = esub(eone(), y);
+          else if (pt[j] == 1) term = y;
+          else term = esub(eadd(edbl(y), y), eone());
+          exy = emul(exy, term);
+        }
+        // <gpe[l], eq(pt)>; eq(pt)[a] = prod_j (a_j ? pt_j : 1 - pt_j), small integers
+        const Ext* gp = gpe_buf + (uint64_t)g * PE_PER + (1u << l) - 2;
+        Ext dot = ezero();
+        for (uint32_t a = 0; a < (1u << l); ++a) {
+          int32_t coef = 1;
+          for (uint32_t j = 0; j < l; ++j) coef *= ((a >> j) & 1) ? (int32_t)pt[j] : 1 - (int32_t)pt[j];
+          if (coef == 0) continue;
+          const uint32_t mag = from_canonical((uint32_t)(coef < 0 ? -coef : coef));
+          const Ext term = emul_base(gp[a], mag);
+          dot = coef < 0 ? esub(dot, term) : eadd(dot, term);
+        }
+        total = eadd(total, emul(exy, dot));
+      }
+    }
+    s_part[tid] = total;
+    __syncthreads();
+    if (tid < SERIAL_THREADS / chunks && o0 + tid < acc_per) {
+      Ext s = ezero();
+      for (uint32_t c = 0; c < chunks; ++c) s = eadd(s, s_part[c * (SERIAL_THREADS / chunks) + tid]);
+      s_tot[o0 + tid] = s;
+    }
+    __syncthreads();
+  }
   // (5) SVO rounds
   if (is_wave0()) {
     uint32_t off = 0, p3 = 1;
@@ -362,27 +394,41 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t*
     tx_end(tx, t, js);
   }
   __syncthreads();
-  // (6) eq(rs) and claim scales alpha^c * eq_xy(svo_c, rs)
+  // (6) eq(rs) and claim scales alpha^rank * eq_xy(svo_c, rs)
   if (tid < W) {
     Ext e = eone();
     for (uint32_t j = 0; j < f; ++j) e = emul(e, ((tid >> j) & 1) ? s_rs[j] : esub(eone(), s_rs[j]));
     js->eq_rs[tid] = e;
   }
-  if (tid < n_claims) {
-    Ext e = s_apow[tid];
+  for (uint32_t c = tid; c < n_claims; c += SERIAL_THREADS) {
+    Ext e = s_apow[c];
     for (uint32_t j = 0; j < f; ++j) {
-      const Ext x = slots[tid].point[j], y = s_rs[j];
+      const Ext x = slots[c].point[j], y = s_rs[j];
       // (y x) * 2 - x - y + 1
       e = emul(e, eadd(esub(esub(edbl(emul(y, x)), x), y), eone()));
     }
-    s_scale[tid] = e;
+    s_apow[c] = e;
   }
   __syncthreads();
-  const uint64_t n0 = 1ull << m0, n1 = 1ull << m1;
-  for (uint64_t it = tid; it < (uint64_t)n_claims * n1; it += SERIAL_THREADS) {
-    const uint64_t c = it / n1, x = it % n1;
-    Ext* p = pool + c * pool_stride + n0 + x;
-    st_ext(p, emul(ld_ext(p), s_scale[c]));
+  // (7) scaled eq1 tables: the group's later claims copy the first claim's
+  // (still unscaled) table, then every first claim scales its own in place.
+  const uint32_t total = s_pref[n_claims];
+  for (int phase = 0; phase < 2; ++phase) {
+    for (uint32_t it = tid; it < total; it += SERIAL_THREADS) {
+      uint32_t lo = 0, hi = n_claims;  // last c with s_pref[c] <= it
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (s_pref[mid] <= it) lo = mid;
+        else hi = mid;
+      }
+      const uint32_t c = lo, x = it - s_pref[c];
+      const ClaimDesc& d = desc[c];
+      const bool first = d.eq1_off == d.lead_eq1;
+      if (first != (phase == 1)) continue;
+      Ext* dst = pool + d.eq1_off + x;
+      st_ext(dst, emul(ld_ext(pool + d.lead_eq1 + x), s_apow[c]));
+    }
+    __syncthreads();
   }
 }
 

@@ -10,6 +10,35 @@
 namespace kb {
 
 constexpr int SC_THREADS = 256;
+constexpr int MAX_CLASSES = 32;  // distinct poly sizes of a stack
+
+// Static description of one claim of the SVO phase (an evaluation claim on a
+// stacked poly, or a virtual claim on the whole stacked vector), fixed by the
+// job shape and uploaded once. Claims drawn at one point form a group; its
+// first claim's tables are built at the draw.
+struct ClaimDesc {
+  uint64_t eq0_off;   // pool offset (Ext) of eq over the low m0 rest variables (the group's)
+  uint64_t eq1_off;   // pool offset of this claim's eq over the high m1 rest variables
+  uint64_t lead_eq1;  // pool offset of the group's first eq1 table (== eq1_off for the first claim)
+  uint64_t poly_off;  // offset of the claim's poly in the stacked base vector
+  uint32_t m0, m1;    // rest variables k_poly - f = m0 + m1
+  uint32_t rank;      // alpha exponent: position in layout order (prover.rs:155-174)
+  uint32_t pad;
+};
+
+// Size classes of a stack in the folded (k - f variable) vector, for the
+// expression build: class c covers [start[c], start[c+1]) in blocks of
+// 2^log[c]; block b of class c is segment seg0[c] + b, whose claims are
+// seg_claims[seg_start[s] .. seg_start[s+1]).
+struct StackBlocks {
+  uint32_t n_class;
+  uint32_t log[MAX_CLASSES];
+  uint32_t seg0[MAX_CLASSES];
+  uint64_t start[MAX_CLASSES + 1];
+  const uint32_t* seg_start;
+  const uint32_t* seg_claims;
+};
+
 
 __device__ __forceinline__ Ext ld_ext(const Ext* p) {
   const uint4 v = *reinterpret_cast<const uint4*>(p);
@@ -70,10 +99,16 @@ __device__ void build_eq_block(Ext* out, const Ext* z, int m, Ext scale, Ext* sL
 // ---- claim evaluation on the base polynomial (SplitEq::compress_hi, eq.rs:290-398) ----
 // out[i] = sum_h eq(rest)[h] * poly[h * W + i] for i < W = 2^F_LOG; the block for
 // h1 = blockIdx.x sums over h0 < 2^m0 and writes W partial values * eq1[h1].
+// blockIdx.y selects one of the claims sharing the point: claim c0 + y reads
+// its poly at desc[c0 + y].poly_off (desc == nullptr: offset 0) and writes its
+// partials after the previous claims' gridDim.x * W.
 template <int F_LOG>
 __global__ void __launch_bounds__(SC_THREADS) compress_hi_kernel(const uint32_t* __restrict__ poly,
                                                              const Ext* __restrict__ eq0, const Ext* __restrict__ eq1,
-                                                             uint32_t m0, Ext* __restrict__ partials) {
+                                                             uint32_t m0, Ext* __restrict__ partials,
+                                                             const ClaimDesc* __restrict__ desc, uint32_t c0) {
+  if (desc) poly += desc[c0 + blockIdx.y].poly_off;
+  partials += (uint64_t)blockIdx.y * gridDim.x * (1u << F_LOG);
   constexpr int W = 1 << F_LOG;
   constexpr int CPT = W < 4 ? W : 4;        // columns per thread
   constexpr int TPR = W / CPT;              // threads per row
@@ -129,11 +164,16 @@ __global__ void __launch_bounds__(SC_THREADS) eval_ext_kernel(const Ext* __restr
 // epoly[j] = sum_{i<W} eq_rs[i] * poly[j*W + i]            (compress_lo, eq.rs:491-533)
 // ew[j]    = sum_c eq0_c[j & (2^m0-1)] * eq1s_c[j >> m0]    (combine_into, eq.rs:541-621;
 //            eq1s_c already carries alpha^c * eq(svo_c, rs))
+// Claims [full0, full0 + n_full) span the whole vector (their tables split at
+// m0); the others are on one stacked poly's block (prover.rs:190-199), found
+// through `sb` (n_class == 0: none).
 template <int F_LOG>
 __global__ void __launch_bounds__(SC_THREADS) expr_build_kernel(const uint32_t* __restrict__ poly,
                                                             const Ext* __restrict__ eq_rs, const Ext* __restrict__ pool,
-                                                            uint32_t n_claims, uint64_t claim_stride, uint32_t m0,
-                                                            uint64_t n, Ext* __restrict__ epoly, Ext* __restrict__ ew) {
+                                                            const ClaimDesc* __restrict__ desc, uint32_t full0,
+                                                            uint32_t n_full, uint32_t m0, uint64_t n,
+                                                            Ext* __restrict__ epoly, Ext* __restrict__ ew,
+                                                            StackBlocks sb) {
   constexpr int W = 1 << F_LOG;
   __shared__ Ext s_eq[W];
   if (threadIdx.x < W) s_eq[threadIdx.x] = eq_rs[threadIdx.x];
@@ -157,11 +197,21 @@ __global__ void __launch_bounds__(SC_THREADS) expr_build_kernel(const uint32_t* 
   }
   st_ext(epoly + j, acc);
   const uint64_t lo = j & ((1ull << m0) - 1), hi = j >> m0;
-  const uint64_t n0 = 1ull << m0;
   Ext w = ezero();
-  for (uint32_t c = 0; c < n_claims; ++c) {
-    const Ext* base = pool + c * claim_stride;
-    w = eadd(w, emul(ld_ext(base + lo), ld_ext(base + n0 + hi)));
+  for (uint32_t c = full0; c < full0 + n_full; ++c)
+    w = eadd(w, emul(ld_ext(pool + desc[c].eq0_off + lo), ld_ext(pool + desc[c].eq1_off + hi)));
+  if (sb.n_class) {
+    uint32_t cl = 0;
+    while (cl < sb.n_class && j >= sb.start[cl + 1]) ++cl;
+    if (cl < sb.n_class) {
+      const uint64_t rel = j - sb.start[cl];
+      const uint32_t seg = sb.seg0[cl] + (uint32_t)(rel >> sb.log[cl]);
+      const uint64_t x = rel & ((1ull << sb.log[cl]) - 1);
+      for (uint32_t q = sb.seg_start[seg]; q < sb.seg_start[seg + 1]; ++q) {
+        const ClaimDesc& d = desc[sb.seg_claims[q]];
+        w = eadd(w, emul(ld_ext(pool + d.eq0_off + (x & ((1ull << d.m0) - 1))), ld_ext(pool + d.eq1_off + (x >> d.m0))));
+      }
+    }
   }
   st_ext(ew + j, w);
 }

@@ -17,6 +17,7 @@
 #include "protocol.h"
 #include "host_params.h"
 #include "host_rng.h"
+#include "stack.h"
 #include "../../include/whir_gpu.h"
 
 using namespace kb;
@@ -352,6 +353,13 @@ struct whir_prover {
   ClaimSlot* slots;
   Ext* svo_pool;
   Ext* ood_pool;
+  ClaimDesc* desc;         // per SVO-phase claim (device copy of hdesc)
+  uint32_t* group_start;   // first claim of each point group, n_groups + 1
+  uint32_t* eq1_pref;      // prefix sums of the claims' eq1 table sizes
+  uint32_t* seg_start;     // stacked polys in offset order: their claims (CSR)
+  uint32_t* seg_claims;
+  Ext* pe_buf;             // svo_kernel scratch
+  Ext* gpe_buf;
   Ext* partials;
   uint32_t* stir_idx;
   uint32_t* stir_count;
@@ -360,7 +368,17 @@ struct whir_prover {
   Ext* powR;
   uint64_t svo_stride, ood_stride;
   int m0_svo, m1_svo;
-  int n_svo_claims;
+  int n_svo_claims;        // evaluation claims + virtual claims
+  // stack layout and the SVO-phase claim table (one poly: a stack of one)
+  whir_host::StackPlan stack;
+  struct Group {
+    uint32_t c0, count, k;
+  };
+  std::vector<Group> groups;
+  std::vector<ClaimDesc> hdesc;
+  std::vector<uint32_t> h_group_start, h_eq1_pref, h_seg_start, h_seg_claims;
+  uint32_t full0 = 0, n_full = 0;  // claims spanning the whole stacked vector
+  StackBlocks sb{};
   uint32_t* host_log = nullptr;  // pinned
   JobScalars* host_js = nullptr;
   DevTx* host_tx = nullptr;
@@ -387,14 +405,108 @@ struct ArenaItem {
 };
 
 // Fills P's plan-derived fields and the list of device buffers of one proof.
-static int prover_layout(whir_prover* P, const whir_params* prm, std::vector<ArenaItem>& items) {
+// Claim groups, pool offsets and expression blocks of the SVO phase: the
+// evaluation claims (transcript order) then n_ood0 virtual claims on the whole
+// stacked vector (write_virtual_eval, prover.rs:231-277: its per-poly sum is
+// the stacked vector's claim at the same point).
+static int claim_table(whir_prover* P, int n_ood0, uint64_t& pool_ext, uint64_t& part_ext) {
+  const whir_host::StackPlan& S = P->stack;
+  const int f = P->plan.f, k = S.k;
+  const int n_user = (int)S.claim_poly.size();
+  const int nc = n_user + n_ood0;
+  P->groups.clear();
+  for (int c = 0; c < n_user; ++c) {
+    const uint32_t kc = (uint32_t)S.k_polys[S.claim_poly[c]];
+    if (S.claim_fresh[c]) P->groups.push_back({(uint32_t)c, 1u, kc});
+    else P->groups.back().count++;
+  }
+  for (int v = 0; v < n_ood0; ++v) P->groups.push_back({(uint32_t)(n_user + v), 1u, (uint32_t)k});
+  P->hdesc.assign((size_t)nc, ClaimDesc{});
+  P->h_group_start.clear();
+  P->h_eq1_pref.assign((size_t)nc + 1, 0);
+  pool_ext = 0;
+  part_ext = 0;
+  for (const auto& G : P->groups) {
+    P->h_group_start.push_back(G.c0);
+    const uint32_t m = G.k - (uint32_t)f, m0 = m / 2, m1 = m - m / 2;
+    part_ext = std::max<uint64_t>(part_ext, (uint64_t)G.count << (m1 + f));
+    for (uint32_t j = 0; j < G.count; ++j) {
+      const uint32_t c = G.c0 + j;
+      ClaimDesc& d = P->hdesc[c];
+      d.m0 = m0;
+      d.m1 = m1;
+      if (j == 0) {
+        d.eq0_off = pool_ext;
+        d.eq1_off = pool_ext + (1ull << m0);
+        pool_ext += (1ull << m0) + (1ull << m1);
+      } else {
+        d.eq0_off = P->hdesc[G.c0].eq0_off;
+        d.eq1_off = pool_ext;
+        pool_ext += 1ull << m1;
+      }
+      d.lead_eq1 = P->hdesc[G.c0].eq1_off;
+      d.poly_off = (int)c < n_user ? S.offset[S.claim_poly[c]] : 0;
+      d.rank = (int)c < n_user ? S.rank[c] : c;
+      P->h_eq1_pref[c + 1] = P->h_eq1_pref[c] + (1u << m1);
+    }
+  }
+  P->h_group_start.push_back((uint32_t)nc);
+  // whole-vector claims: the virtual ones, and every claim of a one-poly stack
+  const bool one = S.k_polys.size() == 1;
+  P->full0 = one ? 0u : (uint32_t)n_user;
+  P->n_full = (uint32_t)nc - P->full0;
+  // blocks: polys in offset order, grouped into runs of equal size
+  P->sb = StackBlocks{};
+  P->h_seg_start.assign(1, 0);
+  P->h_seg_claims.clear();
+  if (!one) {
+    std::vector<int> by_off(S.k_polys.size());
+    for (size_t i = 0; i < by_off.size(); ++i) by_off[i] = (int)i;
+    std::sort(by_off.begin(), by_off.end(), [&](int a, int b) { return S.offset[a] < S.offset[b]; });
+    for (size_t sgi = 0; sgi < by_off.size(); ++sgi) {
+      const int id = by_off[sgi];
+      const uint32_t lg = (uint32_t)(S.k_polys[id] - f);
+      if (P->sb.n_class == 0 || P->sb.log[P->sb.n_class - 1] != lg) {
+        if (P->sb.n_class == (uint32_t)MAX_CLASSES) return WHIR_EINVAL;
+        P->sb.log[P->sb.n_class] = lg;
+        P->sb.seg0[P->sb.n_class] = (uint32_t)sgi;
+        P->sb.start[P->sb.n_class] = S.offset[id] >> f;
+        P->sb.n_class++;
+      }
+      P->sb.start[P->sb.n_class] = (S.offset[id] >> f) + (1ull << lg);
+      for (int c = 0; c < n_user; ++c)
+        if (S.claim_poly[c] == id) P->h_seg_claims.push_back((uint32_t)c);
+      P->h_seg_start.push_back((uint32_t)P->h_seg_claims.size());
+    }
+  }
+  if (P->h_seg_claims.empty()) P->h_seg_claims.push_back(0);
+  return WHIR_OK;
+}
+
+static int prover_layout(whir_prover* P, const whir_params* prm, const whir_stack* st,
+                         std::vector<ArenaItem>& items) {
+  // one poly unless a stack is given: n_points claims, each at a fresh point (test.rs:223-227)
+  whir_host::StackPlan S;
+  if (st) {
+    if (!whir_host::make_stack_plan(st->k_polys, st->n_polys, st->claim_poly, st->claim_fresh, st->n_claims,
+                                    prm->folding, S))
+      return WHIR_EINVAL;
+  } else {
+    if (prm->n_points < 0 || prm->n_points > MAX_CLAIMS) return WHIR_EINVAL;
+    const int32_t kk = prm->k;
+    std::vector<int32_t> cp((size_t)prm->n_points, 0);
+    std::vector<uint8_t> cf((size_t)prm->n_points, 1);
+    if (!whir_host::make_stack_plan(&kk, 1, cp.data(), cf.data(), prm->n_points, prm->folding, S))
+      return WHIR_EINVAL;
+  }
+  if (S.k != prm->k) return WHIR_EINVAL;
   Plan pl = whir_host::make_plan(prm->k, prm->folding, prm->rate, prm->initial_reduction, prm->soundness,
-                                 prm->security_level, prm->n_points);
+                                 prm->security_level, (int)S.claim_poly.size());
   if (!pl.ok) return WHIR_EINVAL;
   const int k = pl.k, f = pl.f;
   if (k > 28 || f > MAX_F || pl.k + pl.rate - f > 24) return WHIR_EINVAL;
   const int n_svo = pl.n_points + pl.n_ood0;
-  if (n_svo < 1 || n_svo > MAX_SVO_CLAIMS) return WHIR_EINVAL;
+  if (n_svo < 1 || n_svo > MAX_CLAIMS) return WHIR_EINVAL;
   int max_ood = 0, max_stir = pl.final_n_stir;
   uint64_t max_cw_words = 0;
   for (const RoundPlan& r : pl.rounds) {
@@ -406,6 +518,9 @@ static int prover_layout(whir_prover* P, const whir_params* prm, std::vector<Are
   if (max_stir > MAX_STIR || max_ood > 16) return WHIR_EINVAL;
   P->plan = pl;
   P->n_svo_claims = n_svo;
+  P->stack = std::move(S);
+  uint64_t pool_ext = 0, group_part_ext = 0;
+  if (int rc = claim_table(P, pl.n_ood0, pool_ext, group_part_ext)) return rc;
   const int m = k - f;
   P->m0_svo = m / 2;
   P->m1_svo = m - m / 2;
@@ -418,7 +533,8 @@ static int prover_layout(whir_prover* P, const whir_params* prm, std::vector<Are
   const uint64_t n_ext = 1ull << m;
   const int m_pow = std::max(0, k - 2 * f);
   const uint64_t pow_l = 1ull << (m_pow / 2), pow_r = 1ull << (m_pow - m_pow / 2);
-  const uint64_t part_ext = std::max<uint64_t>({(1ull << P->m1_svo) << f, 1ull << P->m1_svo, 4096});
+  const uint64_t part_ext = std::max<uint64_t>({group_part_ext, 1ull << P->m1_svo, 4096});
+  constexpr uint64_t PE_PER = (2 << MAX_F) - 2;
   items = {
       {(void**)&P->tx, sizeof(DevTx)},
       {(void**)&P->js, sizeof(JobScalars)},
@@ -436,7 +552,14 @@ static int prover_layout(whir_prover* P, const whir_params* prm, std::vector<Are
       {(void**)&P->ew[0], n_ext * 16},
       {(void**)&P->ew[1], n_ext * 16},
       {(void**)&P->slots, sizeof(ClaimSlot) * (n_svo + 16)},
-      {(void**)&P->svo_pool, P->svo_stride * n_svo * 16},
+      {(void**)&P->svo_pool, pool_ext * 16},
+      {(void**)&P->desc, sizeof(ClaimDesc) * n_svo},
+      {(void**)&P->group_start, P->h_group_start.size() * 4},
+      {(void**)&P->eq1_pref, P->h_eq1_pref.size() * 4},
+      {(void**)&P->seg_start, P->h_seg_start.size() * 4},
+      {(void**)&P->seg_claims, P->h_seg_claims.size() * 4},
+      {(void**)&P->pe_buf, PE_PER * n_svo * 16},
+      {(void**)&P->gpe_buf, PE_PER * P->groups.size() * 16},
       {(void**)&P->ood_pool, P->ood_stride * std::max(1, max_ood) * 16},
       {(void**)&P->partials, part_ext * 16},
       {(void**)&P->stir_idx, MAX_STIR * 4},
@@ -463,16 +586,16 @@ static size_t arena_size(const std::vector<ArenaItem>& items) {
 extern "C" size_t whir_prover_bytes_for(const whir_params* prm) {
   whir_prover tmp;
   std::vector<ArenaItem> items;
-  if (!prm || prover_layout(&tmp, prm, items)) return 0;
+  if (!prm || prover_layout(&tmp, prm, nullptr, items)) return 0;
   return arena_size(items);
 }
 
-extern "C" int whir_prover_create(whir_ctx* c, const whir_params* prm, whir_prover** out) {
+static int prover_create(whir_ctx* c, const whir_params* prm, const whir_stack* st, whir_prover** out) {
   if (!c || !prm || !out) return WHIR_EINVAL;
   whir_prover* P = new whir_prover();
   P->ctx = c;
   std::vector<ArenaItem> items;
-  int rc = prover_layout(P, prm, items);
+  int rc = prover_layout(P, prm, st, items);
   if (rc) {
     delete P;
     return rc;
@@ -496,8 +619,42 @@ extern "C" int whir_prover_create(whir_ctx* c, const whir_params* prm, whir_prov
     whir_prover_destroy(P);
     return WHIR_EOOM;
   }
+  // the static claim table of this shape
+  const struct {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  } up[] = {
+      {P->desc, P->hdesc.data(), P->hdesc.size() * sizeof(ClaimDesc)},
+      {P->group_start, P->h_group_start.data(), P->h_group_start.size() * 4},
+      {P->eq1_pref, P->h_eq1_pref.data(), P->h_eq1_pref.size() * 4},
+      {P->seg_start, P->h_seg_start.data(), P->h_seg_start.size() * 4},
+      {P->seg_claims, P->h_seg_claims.data(), P->h_seg_claims.size() * 4},
+  };
+  for (const auto& u : up)
+    if (hipMemcpy(u.dst, u.src, u.bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      whir_prover_destroy(P);
+      return WHIR_EDEVICE;
+    }
+  P->sb.seg_start = P->seg_start;
+  P->sb.seg_claims = P->seg_claims;
   *out = P;
   return WHIR_OK;
+}
+
+extern "C" int whir_prover_create(whir_ctx* c, const whir_params* prm, whir_prover** out) {
+  return prover_create(c, prm, nullptr, out);
+}
+
+extern "C" int whir_prover_create_stacked(whir_ctx* c, const whir_params* prm, const whir_stack* st,
+                                          whir_prover** out) {
+  if (!st) return WHIR_EINVAL;
+  return prover_create(c, prm, st, out);
+}
+
+extern "C" int64_t whir_prover_stack_offset(const whir_prover* P, int32_t id) {
+  if (!P || id < 0 || id >= (int32_t)P->stack.k_polys.size()) return -1;
+  return (int64_t)P->stack.offset[id];
 }
 
 extern "C" int whir_prover_destroy(whir_prover* P) {
@@ -550,15 +707,15 @@ extern "C" uint32_t* whir_prover_input(whir_prover* P) { return P ? P->input : n
 static unsigned blocks_for(uint64_t n, unsigned per = SC_THREADS) { return (unsigned)((n + per - 1) / per); }
 
 template <int FL>
-static void launch_compress_hi(unsigned grid, hipStream_t s, const uint32_t* poly, const Ext* eq0, const Ext* eq1,
-                               uint32_t m0, Ext* partials) {
-  hipLaunchKernelGGL(compress_hi_kernel<FL>, dim3(grid), dim3(SC_THREADS), 0, s, poly, eq0, eq1, m0, partials);
+static void launch_compress_hi(dim3 grid, hipStream_t s, const uint32_t* poly, const Ext* eq0, const Ext* eq1,
+                               uint32_t m0, Ext* partials, const ClaimDesc* desc, uint32_t c0) {
+  hipLaunchKernelGGL(compress_hi_kernel<FL>, grid, dim3(SC_THREADS), 0, s, poly, eq0, eq1, m0, partials, desc, c0);
 }
 template <int FL>
-static void launch_expr(unsigned grid, hipStream_t s, const uint32_t* poly, const Ext* eq_rs, const Ext* pool,
-                        uint32_t n_claims, uint64_t stride, uint32_t m0, uint64_t n, Ext* ep, Ext* ew) {
-  hipLaunchKernelGGL(expr_build_kernel<FL>, dim3(grid), dim3(SC_THREADS), 0, s, poly, eq_rs, pool, n_claims, stride,
-                     m0, n, ep, ew);
+static void launch_expr(unsigned grid, hipStream_t s, const uint32_t* poly, const Ext* eq_rs, const whir_prover* P,
+                        uint32_t m0, uint64_t n, Ext* ep, Ext* ew) {
+  hipLaunchKernelGGL(expr_build_kernel<FL>, dim3(grid), dim3(SC_THREADS), 0, s, poly, eq_rs, (const Ext*)P->svo_pool,
+                     (const ClaimDesc*)P->desc, P->full0, P->n_full, m0, n, ep, ew, P->sb);
 }
 
 #define FSWITCH(f, CALL)          \
@@ -592,39 +749,43 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
   const uint32_t* root0 = P->tree0 + (2 * H0 - 2) * 8;
 
   // ---- evaluation claims + OOD virtual claims (prover.rs:64-90, 231-277) ----
-  // transcript.write(root) runs in the first claim's kernel; each claim's end
-  // runs in the same kernel as the next claim's draw.
-  const uint32_t m0 = P->m0_svo, m1 = P->m1_svo;
-  auto svo_claim = [&](int ci) {
+  // Claims drawn at one point form a group (one compress launch over its
+  // polys). transcript.write(root) runs in the first group's kernel; each
+  // group's end runs in the same kernel as the next group's draw.
+  const uint32_t m0 = P->m0_svo;
+  const int n_groups = (int)P->groups.size();
+  auto group_begin = [&](int g) {
     ClaimBegin cb{};
-    if (ci < P->n_svo_claims) {
-      cb.slot = P->slots + ci;
-      cb.eq0 = P->svo_pool + ci * P->svo_stride;
-      cb.eq1 = cb.eq0 + (1ull << m0);
-      cb.k = (uint32_t)k;
+    if (g < n_groups) {
+      const whir_prover::Group& G = P->groups[g];
+      const ClaimDesc& d = P->hdesc[G.c0];
+      cb.slot = P->slots + G.c0;
+      cb.count = G.count;
+      cb.eq0 = P->svo_pool + d.eq0_off;
+      cb.eq1 = P->svo_pool + d.eq1_off;
+      cb.k = G.k;
       cb.fsv = (uint32_t)f;
-      cb.m0 = m0;
-      cb.m1 = m1;
+      cb.m0 = d.m0;
+      cb.m1 = d.m1;
     }
     return cb;
   };
-  if (P->n_svo_claims > 0)
-    hipLaunchKernelGGL(claim_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, root0,
-                       svo_claim(0));
-  else
-    hipLaunchKernelGGL(write_root_kernel, dim3(1), dim3(64), 0, s, P->tx, P->log, P->js, root0);
-  for (int ci = 0; ci < P->n_svo_claims; ++ci) {
-    const ClaimBegin cb = svo_claim(ci);
-    FSWITCH(f, launch_compress_hi<FL>(1u << m1, s, poly, cb.eq0, cb.eq1, m0, P->partials));
-    hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots + ci,
-                       P->partials, 1u << m1, (uint32_t)f, svo_claim(ci + 1));
+  hipLaunchKernelGGL(claim_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, root0,
+                     group_begin(0));
+  for (int g = 0; g < n_groups; ++g) {
+    const ClaimBegin cb = group_begin(g);
+    const whir_prover::Group& G = P->groups[g];
+    FSWITCH(f, launch_compress_hi<FL>(dim3(1u << cb.m1, G.count), s, poly, cb.eq0, cb.eq1, cb.m0, P->partials,
+                                      P->desc, G.c0));
+    hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, cb.slot, G.count,
+                       P->partials, 1u << cb.m1, (uint32_t)f, group_begin(g + 1));
   }
   // ---- SVO rounds + expression (prover.rs:279-353, 184-229) ----
-  hipLaunchKernelGGL(svo_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots,
-                     (uint32_t)P->n_svo_claims, (uint32_t)f, P->svo_pool, P->svo_stride, m0, m1);
+  hipLaunchKernelGGL(svo_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots, P->desc,
+                     (uint32_t)P->n_svo_claims, P->group_start, (uint32_t)n_groups, (uint32_t)f, P->svo_pool,
+                     P->pe_buf, P->gpe_buf, P->eq1_pref);
   uint64_t n = 1ull << (k - f);
-  FSWITCH(f, launch_expr<FL>(blocks_for(n), s, poly, P->js->eq_rs, P->svo_pool, (uint32_t)P->n_svo_claims,
-                             P->svo_stride, m0, n, P->ep[0], P->ew[0]));
+  FSWITCH(f, launch_expr<FL>(blocks_for(n), s, poly, P->js->eq_rs, P, m0, n, P->ep[0], P->ew[0]));
   int cur = 0;
   uint32_t rs_idx = (uint32_t)f;
 
@@ -651,6 +812,7 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
         cb.fsv = 0;
         cb.m0 = om0;
         cb.m1 = om1;
+        cb.count = 1;
       }
       return cb;
     };
@@ -663,7 +825,7 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
       const ClaimBegin cb = ood_claim(o);
       hipLaunchKernelGGL(eval_ext_kernel, dim3(1u << om1), dim3(SC_THREADS), 0, s, P->ep[cur], cb.eq0, cb.eq1, om0,
                          P->partials);
-      hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, cb.slot,
+      hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, cb.slot, 1u,
                          P->partials, 1u << om1, 0u, ood_claim(o + 1));
     }
     // STIR queries (whir.rs:237-293)
@@ -778,6 +940,32 @@ extern "C" int whir_prover_enqueue(whir_prover* P, const uint32_t* d_poly, void*
   }
   HIP_TRY(hipEventRecord(P->done, s));
   return WHIR_OK;
+}
+
+__global__ void zero_words_kernel(uint32_t* p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ProverStack::layout's copy (prover.rs:114-118) into the prover's input, then the proof.
+extern "C" int whir_prover_enqueue_stacked(whir_prover* P, const uint32_t* const* d_polys, void* stream) {
+  if (!P || !d_polys) return WHIR_EINVAL;
+  hipStream_t s = pick_stream(P->ctx, stream);
+  const whir_host::StackPlan& S = P->stack;
+  uint64_t used = 0;
+  for (size_t id = 0; id < S.k_polys.size(); ++id) {
+    if (!d_polys[id]) return WHIR_EINVAL;
+    const uint64_t len = 1ull << S.k_polys[id];
+    uint32_t* dst = P->input + S.offset[id];
+    if (d_polys[id] != dst) HIP_TRY(hipMemcpyAsync(dst, d_polys[id], len * 4, hipMemcpyDeviceToDevice, s));
+    used += len;
+  }
+  const uint64_t n = 1ull << S.k;  // the polys fill [0, used) from the largest down
+  if (used < n)
+    hipLaunchKernelGGL(zero_words_kernel, dim3(blocks_for(n - used)), dim3(SC_THREADS), 0, s, P->input + used,
+                       n - used);
+  HIP_TRY(hipGetLastError());
+  return whir_prover_enqueue(P, P->input, stream);
 }
 
 // WHIR_OK when the last enqueued proof has finished, 7 while it is running.
