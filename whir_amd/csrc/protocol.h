@@ -178,38 +178,46 @@ __global__ void __launch_bounds__(SERIAL_THREADS) claim_end_kernel(DevTx* t, uin
     }
     s_eq[tid] = e;
   }
-  // items (claim, column): `lanes` threads per item, a power of two; a pass
-  // holds whole claims (W divides the items of a pass)
-  const uint32_t items = count * W;
-  uint32_t lanes = 1;
-  while (lanes * 2 * items <= SERIAL_THREADS) lanes *= 2;
-  const uint32_t per_pass = SERIAL_THREADS / lanes;
-  for (uint32_t base = 0; base < items; base += per_pass) {
-    const uint32_t item = base + tid / lanes, lane = tid % lanes;
+  // A pass covers cpp claims with S = 256 / cpp threads each (S a multiple of
+  // W). Thread (claim c, sub) sums elements sub, sub + S, ... of claim c's
+  // n_part * W partials (all in column sub mod W): consecutive threads read
+  // consecutive 16 B, eight loads in flight.
+  const uint32_t per_claim = n_part * W;
+  const uint32_t cpp = per_claim >= SERIAL_THREADS ? 1u : SERIAL_THREADS / per_claim;
+  const uint32_t S = SERIAL_THREADS / cpp;
+  for (uint32_t base = 0; base < count; base += cpp) {
+    const uint32_t c = base + tid / S, sub = tid % S;
     Ext acc = ezero();
-    if (item < items) {
-      const Ext* pc = partials + (uint64_t)(item / W) * n_part * W + (item % W);
-      for (uint32_t p = lane; p < n_part; p += lanes) acc = eadd(acc, ld_ext(pc + (uint64_t)p * W));
+    if (c < count) {
+      const Ext* pc = partials + (uint64_t)c * per_claim;
+      uint32_t e = sub;
+      for (; e + 7 * S < per_claim; e += 8 * S) {
+        Ext v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = ld_ext(pc + e + u * S);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = eadd(acc, v[u]);
+      }
+      for (; e < per_claim; e += S) acc = eadd(acc, ld_ext(pc + e));
     }
     sh[tid] = acc;
     __syncthreads();
+    // column i of claim c: the S / W threads with sub = i mod W
+    const uint32_t ci = tid / W, i = tid % W;  // ci < cpp when tid < cpp * W
     Ext comp = ezero();
-    if (lane == 0 && item < items) {
-      comp = sh[tid];
-      for (uint32_t r = 1; r < lanes; ++r) comp = eadd(comp, sh[tid + r]);
-      slot[item / W].compressed[item % W] = comp;
+    if (ci < cpp && base + ci < count) {
+      for (uint32_t r = i; r < S; r += W) comp = eadd(comp, sh[ci * S + r]);
+      slot[base + ci].compressed[i] = comp;
     }
     __syncthreads();
-    if (lane == 0) sh[tid] = emul(comp, s_eq[item % W]);
+    if (ci < cpp) sh[tid] = emul(comp, s_eq[i]);
     __syncthreads();
     // one thread per claim of the pass: eval = sum_i eq(svo)[i] * compressed[i]
-    const uint32_t c_in = tid;  // claim within the pass
-    if (c_in < per_pass / W && base + c_in * W < items) {
+    if (tid < cpp && base + tid < count) {
       Ext e = ezero();
-      for (uint32_t i = 0; i < W; ++i) e = eadd(e, sh[(c_in * W + i) * lanes]);
-      const uint32_t c = base / W + c_in;
-      s_eval[c] = e;
-      slot[c].eval = e;
+      for (uint32_t j = 0; j < W; ++j) e = eadd(e, sh[tid * W + j]);
+      s_eval[base + tid] = e;
+      slot[base + tid].eval = e;
     }
     __syncthreads();
   }
@@ -245,6 +253,7 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t*
   __shared__ Ext s_part[SERIAL_THREADS];
   __shared__ Ext s_apow[MAX_CLAIMS];  // alpha^rank, then the expression scales
   __shared__ uint32_t s_pref[MAX_CLAIMS + 1];
+  __shared__ uint32_t s_e1[MAX_CLAIMS], s_le[MAX_CLAIMS];  // eq1_off, lead_eq1
   __shared__ Ext s_rs[MAX_F];
   __shared__ Ext s_lag[MAX_F][3];
   __shared__ Ext s_alpha;
@@ -276,6 +285,10 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t*
     pe_buf[(uint64_t)c * PE_PER + o] = s;
   }
   for (uint32_t c = tid; c <= n_claims; c += SERIAL_THREADS) s_pref[c] = eq1_pref[c];
+  for (uint32_t c = tid; c < n_claims; c += SERIAL_THREADS) {
+    s_e1[c] = (uint32_t)desc[c].eq1_off;
+    s_le[c] = (uint32_t)desc[c].lead_eq1;
+  }
 
   // (2) draw alpha; alpha^rank per claim
   WaveTx tx;
@@ -412,21 +425,37 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t*
   __syncthreads();
   // (7) scaled eq1 tables: the group's later claims copy the first claim's
   // (still unscaled) table, then every first claim scales its own in place.
+  // Four loads in flight per thread; claim offsets come from LDS.
   const uint32_t total = s_pref[n_claims];
   for (int phase = 0; phase < 2; ++phase) {
-    for (uint32_t it = tid; it < total; it += SERIAL_THREADS) {
-      uint32_t lo = 0, hi = n_claims;  // last c with s_pref[c] <= it
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (s_pref[mid] <= it) lo = mid;
-        else hi = mid;
+    for (uint32_t it0 = tid; it0 < total; it0 += 4 * SERIAL_THREADS) {
+      Ext v[4];
+      uint32_t dst[4], cl[4];
+      bool use[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t it = it0 + (uint32_t)u * SERIAL_THREADS;
+        use[u] = false;
+        dst[u] = 0;
+        cl[u] = 0;
+        if (it < total) {
+          uint32_t lo = 0, hi = n_claims;  // last c with s_pref[c] <= it
+          while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (s_pref[mid] <= it) lo = mid;
+            else hi = mid;
+          }
+          const uint32_t x = it - s_pref[lo];
+          const bool first = s_e1[lo] == s_le[lo];
+          use[u] = first == (phase == 1);
+          dst[u] = s_e1[lo] + x;
+          cl[u] = lo;
+          if (use[u]) v[u] = ld_ext(pool + s_le[lo] + x);
+        }
       }
-      const uint32_t c = lo, x = it - s_pref[c];
-      const ClaimDesc& d = desc[c];
-      const bool first = d.eq1_off == d.lead_eq1;
-      if (first != (phase == 1)) continue;
-      Ext* dst = pool + d.eq1_off + x;
-      st_ext(dst, emul(ld_ext(pool + d.lead_eq1 + x), s_apow[c]));
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (use[u]) st_ext(pool + dst[u], emul(v[u], s_apow[cl[u]]));
     }
     __syncthreads();
   }
@@ -668,7 +697,20 @@ __global__ void __launch_bounds__(64) write_final_kernel(DevTx* t, uint32_t* log
                                                          uint32_t n) {
   WaveTx tx;
   tx_begin(tx, t, log, js);
-  for (uint32_t i = 0; i < n; ++i) tx.write_e(poly[i]);
+  // 16 extension elements per wave load (one word per lane), then read back
+  // lane by lane, so the duplex chain does not wait on memory
+  const uint32_t* words = reinterpret_cast<const uint32_t*>(poly);
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t i0 = 0; i0 < n; i0 += 16) {
+    const uint32_t v = i0 * 4 + lane < n * 4 ? words[i0 * 4 + lane] : 0u;
+    const uint32_t m = n - i0 < 16 ? n - i0 : 16u;
+    for (uint32_t j = 0; j < m; ++j) {
+      Ext e;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) e.c[c] = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)(4 * j + c));
+      tx.write_e(e);
+    }
+  }
   tx_end(tx, t, js);
 }
 

@@ -121,6 +121,7 @@ __global__ void __launch_bounds__(SC_THREADS) compress_hi_kernel(const uint32_t*
   Ext acc[CPT];
 #pragma unroll
   for (int c = 0; c < CPT; ++c) acc[c] = ezero();
+#pragma unroll 4
   for (uint64_t h0 = rl; h0 < n0; h0 += RPP) {
     const Ext e = ld_ext(eq0 + h0);
     const uint32_t* row = poly + ((h0 + (h1 << m0)) << F_LOG) + cg * CPT;

@@ -451,6 +451,7 @@ static int claim_table(whir_prover* P, int n_ood0, uint64_t& pool_ext, uint64_t&
     }
   }
   P->h_group_start.push_back((uint32_t)nc);
+  if (pool_ext >> 32) return WHIR_EINVAL;  // svo_kernel keeps pool offsets in 32 bits
   // whole-vector claims: the virtual ones, and every claim of a one-poly stack
   const bool one = S.k_polys.size() == 1;
   P->full0 = one ? 0u : (uint32_t)n_user;
