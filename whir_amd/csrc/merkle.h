@@ -76,8 +76,12 @@ __global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_kernel(const uint
 // barrier: its n0/G nodes stay in LDS from layer to layer (each layer is also
 // written out for the openings). Launch with G = 1 and one workgroup for the
 // final layers.
-constexpr int COOP_THREADS = 1024;  // 64 compressions in flight per workgroup
-constexpr uint32_t COOP_MAX_NODES = 256;  // n0 / G
+// A workgroup is one wave per SIMD (16 compressions in flight), so a level of
+// at most 16 compressions per workgroup costs one cooperative P16 latency; the
+// host cuts the top of a tree into launches of at most COOP_MAX_NODES nodes
+// per residue class (five levels each).
+constexpr int COOP_THREADS = 256;
+constexpr uint32_t COOP_MAX_NODES = 32;  // n0 / G
 
 __global__ void __launch_bounds__(COOP_THREADS) merkle_coop_subtrees_kernel(uint32_t* layer, uint32_t n0, uint32_t G) {
   __shared__ uint32_t s_node[COOP_MAX_NODES * 8];

@@ -247,7 +247,6 @@ static int leaves_enqueue(const uint32_t* rows, uint64_t n_rows, uint32_t row_wo
 // Wide layers use one compression per lane (throughput-bound); from
 // MERKLE_COOP_FROM nodes down, cooperative subtrees (latency-bound part).
 constexpr uint64_t MERKLE_COOP_FROM = 1ull << 15;
-constexpr uint64_t MERKLE_COOP_STAGE1 = 128;  // nodes per workgroup in the first cooperative stage
 
 static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, uint32_t* layers, hipStream_t s,
                           hipEvent_t* leaf_ev = nullptr) {
@@ -265,18 +264,16 @@ static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, u
     cur += n * 8;
     n = half;
   }
-  if (n > 1) {
-    // first stage: residue classes of MERKLE_COOP_STAGE1 nodes; then one
-    // workgroup for the last <= COOP_MAX_NODES nodes
-    if (n > COOP_MAX_NODES) {
-      const uint32_t G = (uint32_t)(n / MERKLE_COOP_STAGE1);
-      hipLaunchKernelGGL(merkle_coop_subtrees_kernel, dim3(G), dim3(COOP_THREADS), 0, s, cur, (uint32_t)n, G);
-      while (n > G) {  // advance to the G-node layer
-        cur += n * 8;
-        n /= 2;
-      }
+  // cooperative stages: G residue classes of at most COOP_MAX_NODES nodes,
+  // each reduced to its node of the G-node layer
+  while (n > 1) {
+    const uint64_t kk = std::min<uint64_t>(n, COOP_MAX_NODES);
+    const uint32_t G = (uint32_t)(n / kk);
+    hipLaunchKernelGGL(merkle_coop_subtrees_kernel, dim3(G), dim3(COOP_THREADS), 0, s, cur, (uint32_t)n, G);
+    while (n > G) {  // advance to the G-node layer
+      cur += n * 8;
+      n /= 2;
     }
-    hipLaunchKernelGGL(merkle_coop_subtrees_kernel, dim3(1), dim3(COOP_THREADS), 0, s, cur, (uint32_t)n, 1u);
   }
   return WHIR_OK;
 }
