@@ -118,14 +118,15 @@ __global__ void __launch_bounds__(SC_THREADS) compress_hi_kernel(const uint32_t*
   const uint32_t cg = t % TPR, rl = t / TPR;
   const uint64_t h1 = blockIdx.x;
   const uint64_t n0 = 1ull << m0;
-  Ext acc[CPT];
+  // Lazy sums: two Montgomery products per REDC (2 p^2 < p 2^32), each REDC
+  // output (< 2p) added unreduced in 64 bits; one modular fix per output.
+  uint64_t acc[CPT][4];
 #pragma unroll
-  for (int c = 0; c < CPT; ++c) acc[c] = ezero();
-#pragma unroll 4
-  for (uint64_t h0 = rl; h0 < n0; h0 += RPP) {
-    const Ext e = ld_ext(eq0 + h0);
+  for (int c = 0; c < CPT; ++c)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[c][k] = 0;
+  auto load_row = [&](uint64_t h0, uint32_t (&v)[CPT]) {
     const uint32_t* row = poly + ((h0 + (h1 << m0)) << F_LOG) + cg * CPT;
-    uint32_t v[CPT];
     if constexpr (CPT == 4) {
       const uint4 q = *reinterpret_cast<const uint4*>(row);
       v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
@@ -133,11 +134,35 @@ __global__ void __launch_bounds__(SC_THREADS) compress_hi_kernel(const uint32_t*
 #pragma unroll
       for (int c = 0; c < CPT; ++c) v[c] = row[c];
     }
+  };
+  uint64_t h0 = rl;
+#pragma unroll 2
+  for (; h0 + RPP < n0; h0 += 2 * RPP) {
+    const Ext e0 = ld_ext(eq0 + h0), e1 = ld_ext(eq0 + h0 + RPP);
+    uint32_t v0[CPT], v1[CPT];
+    load_row(h0, v0);
+    load_row(h0 + RPP, v1);
 #pragma unroll
-    for (int c = 0; c < CPT; ++c) acc[c] = eadd(acc[c], emul_base(e, v[c]));
+    for (int c = 0; c < CPT; ++c)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc[c][k] += redc(mad64(e0.c[k], v0[c], (uint64_t)e1.c[k] * v1[c]));
+  }
+  if (h0 < n0) {
+    const Ext e0 = ld_ext(eq0 + h0);
+    uint32_t v0[CPT];
+    load_row(h0, v0);
+#pragma unroll
+    for (int c = 0; c < CPT; ++c)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc[c][k] += redc((uint64_t)e0.c[k] * v0[c]);
   }
 #pragma unroll
-  for (int c = 0; c < CPT; ++c) sh[t * CPT + c] = acc[c];
+  for (int c = 0; c < CPT; ++c) {
+    Ext r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r.c[k] = canon2(mulr(redc(acc[c][k]), R2MOD));  // acc < 2^8 * 2p
+    sh[t * CPT + c] = r;
+  }
   __syncthreads();
   if (t < (uint32_t)W) {
     const uint32_t g = t / CPT, c = t % CPT;
@@ -182,20 +207,26 @@ __global__ void __launch_bounds__(SC_THREADS) expr_build_kernel(const uint32_t* 
   const uint64_t j = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
   if (j >= n) return;
   const uint32_t* row = poly + (j << F_LOG);
-  Ext acc = ezero();
+  // lazy: two products per REDC, REDC outputs summed in 64 bits, one fix per coefficient
+  uint64_t a4[4] = {0, 0, 0, 0};
   if constexpr (W >= 4) {
 #pragma unroll
     for (int i = 0; i < W; i += 4) {
       const uint4 q = *reinterpret_cast<const uint4*>(row + i);
-      acc = eadd(acc, emul_base(s_eq[i], q.x));
-      acc = eadd(acc, emul_base(s_eq[i + 1], q.y));
-      acc = eadd(acc, emul_base(s_eq[i + 2], q.z));
-      acc = eadd(acc, emul_base(s_eq[i + 3], q.w));
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        a4[k] += (uint64_t)redc(mad64(s_eq[i].c[k], q.x, (uint64_t)s_eq[i + 1].c[k] * q.y)) +
+                 redc(mad64(s_eq[i + 2].c[k], q.z, (uint64_t)s_eq[i + 3].c[k] * q.w));
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < W; ++i) acc = eadd(acc, emul_base(s_eq[i], row[i]));
+    for (int i = 0; i < W; ++i)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) a4[k] += redc((uint64_t)s_eq[i].c[k] * row[i]);
   }
+  Ext acc;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) acc.c[k] = canon2(mulr(redc(a4[k]), R2MOD));
   st_ext(epoly + j, acc);
   const uint64_t lo = j & ((1ull << m0) - 1), hi = j >> m0;
   Ext w = ezero();
