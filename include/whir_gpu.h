@@ -65,6 +65,12 @@ int whir_ctx_destroy(whir_ctx* ctx);
  * to what three passes need), workgroups of up to 2^log_threads threads (6..8).
  * Results do not depend on it. Defaults 8, 8. */
 int whir_ctx_set_ntt_tuning(whir_ctx* ctx, int max_log_pass, int log_threads);
+/* NTT pass layout of this context (no counterpart in the reference): for a
+ * transform that needs three passes, first_pass_log in 2..4 makes the first
+ * pass an LDS-free radix-2^first_pass_log register pass (0: three even passes);
+ * log_vec in 0..2 sets the words per lane and line (1, 2, 4). Results do not
+ * depend on it. Defaults 4, 2. */
+int whir_ctx_set_ntt_shape(whir_ctx* ctx, int first_pass_log, int log_vec);
 
 /* ---- component entry points ---- */
 

@@ -69,6 +69,34 @@ def test_dft_batch(gpu_ctx, oracle, n_in, log_h, width):
     assert np.array_equal(got, want)
 
 
+# pass layouts of whir_ctx_set_ntt_tuning / whir_ctx_set_ntt_shape on one fresh
+# context: (max_log_pass, log_threads, first_pass_log, log_vec); a small
+# max_log_pass makes these shapes take three passes
+NTT_LAYOUTS = [(5, 8, 4, 2), (5, 8, 3, 2), (5, 7, 2, 1), (4, 6, 4, 0), (8, 8, 4, 2), (6, 8, 0, 1)]
+LAYOUT_CASES = [(1 << 12, 13, 16), (1 << 11, 13, 64), (1 << 13, 14, 8), (1 << 10, 12, 4), (1 << 17, 18, 16)]
+
+
+@pytest.mark.parametrize("layout", NTT_LAYOUTS, ids=lambda t: "-".join(map(str, t)))
+def test_dft_pass_layouts(oracle, layout):
+    import whir_amd
+    from whir_amd._lib import lib
+
+    ctx = whir_amd.Context(0, 1000)
+    try:
+        assert lib().whir_ctx_set_ntt_tuning(ctx.ptr, layout[0], layout[1]) == 0
+        assert lib().whir_ctx_set_ntt_shape(ctx.ptr, layout[2], layout[3]) == 0
+        for n_in, log_h, width in LAYOUT_CASES:
+            rng = np.random.default_rng(n_in + log_h + width)
+            mat = _rand_field(rng, (n_in, width))
+            print(f"dft layout={layout} n_in={n_in} log_h={log_h} width={width}", flush=True)
+            got = whir_amd.dft_batch(mat, log_h, ctx=ctx)
+            padded = np.zeros((1 << log_h, width), np.uint32)
+            padded[:n_in] = mat
+            assert np.array_equal(got, oracle.dft_batch(padded)), (layout, n_in, log_h, width)
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("log_h,row_words", [(0, 16), (1, 2), (3, 16), (4, 8), (6, 64), (10, 16), (11, 64), (13, 16)])
 def test_merkle_commit(gpu_ctx, oracle, log_h, row_words):
     import whir_amd

@@ -211,6 +211,22 @@ __global__ void __launch_bounds__(256) ntt_kernel(NttArgs a) {
     }
   }
   dft_regs<LR1, V>(x);
+  if constexpr (LR2 == 0) {
+    // register-only pass (an early pass of a multi-pass transform): output t
+    // of this line is x[t]; no exchange, no LDS
+    if (a.twiddle_out && m) {
+      const uint32_t st = tw24(a.tw_lo, a.tw_hi, m << (24 - log_mprev));  // w_{M_{p-1}}^m
+      uint32_t cur = st;
+#pragma unroll
+      for (int ta = 1; ta < R1; ++ta) {
+        vscale<V>(x[ta], cur);
+        cur = mul(cur, st);
+      }
+    }
+#pragma unroll
+    for (int ta = 0; ta < R1; ++ta) vstore<V>(a.dst + base + (uint32_t)ta * jstride, x[ta]);
+    return;
+  }
   if (R2 > 1 && jb) {  // inner twiddles w_L^(j_b t_a)
     const uint32_t w1 = tw24(a.tw_lo, a.tw_hi, jb << (24 - LL));
     uint32_t cur = w1;

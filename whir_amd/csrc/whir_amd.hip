@@ -42,6 +42,10 @@ struct whir_ctx {
   // NTT tile shape: largest pass 2^ntt_max_logl rows, 2^ntt_log_threads threads per workgroup
   int ntt_max_logl = 8;
   int ntt_log_threads = 8;
+  // first pass of a transform of >= 3 passes as an LDS-free radix-2^ntt_first_log
+  // register pass (0: all passes even); vector width 2^ntt_max_lv words per line
+  int ntt_first_log = 4;
+  int ntt_max_lv = 2;
   uint32_t* tw = nullptr;  // tw_lo[4096] || tw_hi[4096]
   const uint32_t* tw_lo() const { return tw; }
   const uint32_t* tw_hi() const { return tw + 4096; }
@@ -112,6 +116,14 @@ extern "C" int whir_ctx_destroy(whir_ctx* c) {
   return WHIR_OK;
 }
 
+extern "C" int whir_ctx_set_ntt_shape(whir_ctx* c, int first_pass_log, int log_vec) {
+  if (!c || !(first_pass_log == 0 || (first_pass_log >= 2 && first_pass_log <= 4)) || log_vec < 0 || log_vec > 2)
+    return WHIR_EINVAL;
+  c->ntt_first_log = first_pass_log;
+  c->ntt_max_lv = log_vec;
+  return WHIR_OK;
+}
+
 extern "C" int whir_ctx_set_ntt_tuning(whir_ctx* c, int max_log_pass, int log_threads) {
   if (!c || max_log_pass < 1 || max_log_pass > 8 || log_threads < 6 || log_threads > 8) return WHIR_EINVAL;
   c->ntt_max_logl = max_log_pass;
@@ -129,9 +141,11 @@ static hipStream_t pick_stream(whir_ctx* c, void* s) { return s ? (hipStream_t)s
 struct NttPlan {
   int log_nn = 0, log_c = 0, log_n = 0, log_b = 0, lv = 0, P = 0;
   int logl[3] = {0, 0, 0};
+  bool reg_first = false;  // pass 0 is ntt_kernel<logl[0], 0, V> (no LDS)
 };
 
-static bool ntt_plan(uint64_t n_in, int log_nn, int log_c, NttPlan& pl, int max_logl) {
+static bool ntt_plan(uint64_t n_in, int log_nn, int log_c, NttPlan& pl, int max_logl, int first_log = 0,
+                     int max_lv = 2) {
   if (log_nn < 1 || log_nn > 24 || log_c < 0 || log_c > 10 || log_nn + log_c > 31) return false;
   if (n_in > (1ull << log_nn)) return false;
   int log_n = 1;
@@ -141,7 +155,18 @@ static bool ntt_plan(uint64_t n_in, int log_nn, int log_c, NttPlan& pl, int max_
   pl.log_c = log_c;
   pl.log_n = log_n;
   pl.log_b = log_c + (log_nn - log_n);
-  pl.lv = std::min(2, log_c);
+  pl.lv = std::min(std::min(2, max_lv), log_c);
+  pl.reg_first = false;
+  if (first_log > 0 && log_n > 2 * std::min(max_logl, 8) && log_n - first_log <= 16) {
+    // a transform that needs three passes anyway: a register-only first pass,
+    // then two even passes
+    pl.reg_first = true;
+    pl.P = 3;
+    pl.logl[0] = first_log;
+    pl.logl[1] = (log_n - first_log + 1) / 2;
+    pl.logl[2] = log_n - first_log - pl.logl[1];
+    return true;
+  }
   max_logl = std::max(max_logl, (log_n + 2) / 3);  // at most three passes
   pl.P = (log_n + max_logl - 1) / max_logl;
   if (pl.P > 3) return false;
@@ -154,6 +179,16 @@ static bool ntt_plan(uint64_t n_in, int log_nn, int log_c, NttPlan& pl, int max_
 }
 
 typedef void (*NttKernelFn)(NttArgs);
+
+template <int V>
+static NttKernelFn ntt_reg_kernel_for(int logl) {
+  switch (logl) {
+    case 2: return ntt_kernel<2, 0, V>;
+    case 3: return ntt_kernel<3, 0, V>;
+    case 4: return ntt_kernel<4, 0, V>;
+    default: return nullptr;
+  }
+}
 
 template <int V>
 static NttKernelFn ntt_kernel_for(int logl) {
@@ -175,11 +210,12 @@ static NttKernelFn ntt_kernel_for(int logl) {
 static int ntt_enqueue(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_nn, int log_c, uint32_t* out,
                        uint32_t* tmp, hipStream_t s) {
   NttPlan pl;
-  if (!ntt_plan(n_in, log_nn, log_c, pl, c->ntt_max_logl)) return WHIR_EINVAL;
+  if (!ntt_plan(n_in, log_nn, log_c, pl, c->ntt_max_logl, c->ntt_first_log, c->ntt_max_lv)) return WHIR_EINVAL;
   int used = 0;
   for (int p = 0; p < pl.P; ++p) {
     const int l = pl.logl[p];
-    const int lr2 = l / 2;
+    const bool reg = pl.reg_first && p == 0;
+    const int lr2 = reg ? 0 : l / 2;
     used += l;
     NttArgs a;
     memset(&a, 0, sizeof(a));
@@ -203,8 +239,11 @@ static int ntt_enqueue(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_n
     a.dst = p + 1 == pl.P ? out : tmp;
     const unsigned grid = 1u << (log_lines - (int)a.log_qv);
     const unsigned threads = 1u << (lr2 + (int)a.log_qv);
-    const size_t lds = ((size_t)1 << (l + (int)a.log_qv + pl.lv)) * 4;
-    NttKernelFn fn = pl.lv == 2 ? ntt_kernel_for<4>(l) : pl.lv == 1 ? ntt_kernel_for<2>(l) : ntt_kernel_for<1>(l);
+    const size_t lds = reg ? 0 : ((size_t)1 << (l + (int)a.log_qv + pl.lv)) * 4;
+    NttKernelFn fn = reg ? (pl.lv == 2 ? ntt_reg_kernel_for<4>(l) : pl.lv == 1 ? ntt_reg_kernel_for<2>(l)
+                                                                            : ntt_reg_kernel_for<1>(l))
+                         : (pl.lv == 2 ? ntt_kernel_for<4>(l) : pl.lv == 1 ? ntt_kernel_for<2>(l)
+                                                                            : ntt_kernel_for<1>(l));
     if (!fn) return WHIR_EINVAL;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(threads), lds, s, a);
   }
@@ -220,7 +259,8 @@ extern "C" int whir_dft_batch(whir_ctx* c, const uint32_t* d_in, uint64_t n_in_r
   hipStream_t s = pick_stream(c, stream);
   uint32_t* tmp = nullptr;
   NttPlan pl;
-  if (!ntt_plan(n_in_rows, (int)log_h, log_c, pl, c->ntt_max_logl)) return WHIR_EINVAL;
+  if (!ntt_plan(n_in_rows, (int)log_h, log_c, pl, c->ntt_max_logl, c->ntt_first_log, c->ntt_max_lv))
+    return WHIR_EINVAL;
   if (pl.P > 1) HIP_TRY(hipMallocAsync((void**)&tmp, ((size_t)width << log_h) * 4, s));
   int rc = ntt_enqueue(c, d_in, n_in_rows, (int)log_h, log_c, d_out, tmp, s);
   if (tmp) HIP_TRY(hipFreeAsync(tmp, s));
