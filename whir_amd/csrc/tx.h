@@ -32,11 +32,6 @@ template <int CTRL>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
 }
-template <int CTRL>
-__device__ __forceinline__ uint64_t dpp64(uint64_t v) {
-  const uint32_t lo = dpp<CTRL>((uint32_t)v), hi = dpp<CTRL>((uint32_t)(v >> 32));
-  return ((uint64_t)hi << 32) | lo;
-}
 constexpr int DPP_QUAD_NEXT = 1 | (2 << 2) | (3 << 4) | (0 << 6);  // lane a <- a+1 (mod 4)
 constexpr int DPP_QUAD_XOR1 = 1 | (0 << 2) | (3 << 4) | (2 << 6);
 constexpr int DPP_QUAD_XOR2 = 2 | (3 << 2) | (0 << 4) | (1 << 6);
@@ -74,25 +69,32 @@ struct CoopP16 {
 // REDC(MDS-light(x) + add) across the 16 lanes of a row; x < 2p, result
 // < p + 2^7. Per chunk (quad) row a of M4 is S4 + x_a + 2 x_{a+1}; then every
 // lane adds the sum over the four chunks at its position (rotations by 4, 8).
+// The cross-lane sums run on the two 16-bit limbs of x (x & 0xffff, x >> 16) as
+// independent 32-bit DPP adds (every partial sum < 2^22): no carries and no
+// 64-bit moves, which shortens the dependency chain a single wavefront waits on.
 __device__ __forceinline__ uint32_t mds16_coop_lazy(uint32_t x, uint32_t add) {
-  const uint32_t xn = dpp<DPP_QUAD_NEXT>(x);
-  const uint64_t s2 = (uint64_t)x + dpp<DPP_QUAD_XOR1>(x);
-  const uint64_t s4 = s2 + dpp64<DPP_QUAD_XOR2>(s2);
-  const uint64_t y = mad_k<2>(xn, mad_k<1>(x, s4));
-  const uint64_t u = y + dpp64<DPP_ROR(4)>(y);
-  const uint64_t cs = u + dpp64<DPP_ROR(8)>(u);
-  return redc(mad_k<1>(add, y + cs));
+  const uint32_t a = x & 0xffffu, b = x >> 16;
+  const uint32_t an = dpp<DPP_QUAD_NEXT>(a), bn = dpp<DPP_QUAD_NEXT>(b);
+  uint32_t sa = a + dpp<DPP_QUAD_XOR1>(a), sb = b + dpp<DPP_QUAD_XOR1>(b);
+  sa += dpp<DPP_QUAD_XOR2>(sa);
+  sb += dpp<DPP_QUAD_XOR2>(sb);
+  const uint32_t ya = sa + (a + 2u * an), yb = sb + (b + 2u * bn);  // S4 + x_a + 2 x_{a+1}
+  uint32_t ua = ya + dpp<DPP_ROR(4)>(ya), ub = yb + dpp<DPP_ROR(4)>(yb);
+  ua += dpp<DPP_ROR(8)>(ua);
+  ub += dpp<DPP_ROR(8)>(ub);
+  return redc(mad64(yb + ub, 65536u, (uint64_t)(ya + ua + add)));  // y + column sums + add
 }
 
-__device__ __forceinline__ uint64_t row_sum16_64(uint32_t x) {
-  uint64_t v = (uint64_t)x + dpp<DPP_ROR(8)>(x);
-  v += dpp64<DPP_ROR(4)>(v);
-  v += dpp64<DPP_ROR(2)>(v);
-  v += dpp64<DPP_ROR(1)>(v);
-  return v;
+// REDC(a k) for a constant k with kp = k * (-p^-1) mod 2^32: the Montgomery
+// quotient a * kp does not wait for the product, so the chain is two deep.
+__device__ __forceinline__ uint32_t mulr_k(uint32_t a, uint32_t k, uint32_t kp) {
+  const uint64_t t = (uint64_t)a * k;
+  return (uint32_t)(mad64(a * kp, P, t) >> 32);
 }
 
 // In: s = the lane's state word in [0, p). Out: the permuted word in [0, p).
+// Internal rounds: lane 0's S-box (times kappa_r), then the row sum on 16-bit
+// limbs, then x_i <- REDC(sum + V_i x_i + c) as in internal_lazy.
 __device__ __forceinline__ void permute16_coop(uint32_t& s, const CoopP16& K) {
   const bool lane0 = (threadIdx.x & 15) == 0;
   uint32_t x = mds16_coop_lazy(s, K.la[0]);
@@ -100,11 +102,21 @@ __device__ __forceinline__ void permute16_coop(uint32_t& s, const CoopP16& K) {
   for (int r = 1; r < 5; ++r) x = mds16_coop_lazy(mulr(mulr(x, x), x), K.la[r]);
 #pragma unroll
   for (int r = 0; r < 20; ++r) {
-    const uint32_t c = mulr(mulr(mulr(x, x), x), K.kappa[r]);
+    const uint32_t c = mulr_k(mulr(mulr(x, x), x), K.kappa[r], K.kappa[r] * NPINV);
     x = lane0 ? c : x;
-    const uint64_t sum = row_sum16_64(x);
+    uint32_t a = x & 0xffffu, b = x >> 16;
+    a += dpp<DPP_ROR(8)>(a);
+    b += dpp<DPP_ROR(8)>(b);
+    a += dpp<DPP_ROR(4)>(a);
+    b += dpp<DPP_ROR(4)>(b);
+    a += dpp<DPP_ROR(2)>(a);
+    b += dpp<DPP_ROR(2)>(b);
+    a += dpp<DPP_ROR(1)>(a);
+    b += dpp<DPP_ROR(1)>(b);
     const uint32_t y = lane0 ? 2u * P - x : x;
-    const uint64_t t = mad_k<1>(r < 19 ? K.ia[r + 1] : K.la[5], mad64(y, K.d, sum));
+    const uint32_t ia = r < 19 ? K.ia[r + 1] : K.la[5];
+    // sum = b 2^16 + a (a, b < 2^20); t = y d + sum + c < 2 p^2 + 2^37
+    const uint64_t t = mad64(y, K.d, mad64(b, 65536u, (uint64_t)(a + ia)));
     x = r < 19 ? redc(t) : canon2(redc(t));
   }
 #pragma unroll
