@@ -97,7 +97,8 @@ def test_dft_pass_layouts(oracle, layout):
         ctx.close()
 
 
-@pytest.mark.parametrize("log_h,row_words", [(0, 16), (1, 2), (3, 16), (4, 8), (6, 64), (10, 16), (11, 64), (13, 16)])
+@pytest.mark.parametrize("log_h,row_words", [(0, 16), (1, 2), (3, 16), (4, 8), (6, 64), (10, 16), (11, 64), (13, 16),
+                                             (15, 64), (16, 16), (17, 32)])
 def test_merkle_commit(gpu_ctx, oracle, log_h, row_words):
     import whir_amd
 

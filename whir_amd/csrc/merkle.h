@@ -9,6 +9,7 @@
 #include "field.h"
 #include "poseidon2.h"
 #include "tx.h"
+#include "p2pair.h"
 
 namespace kb {
 
@@ -66,6 +67,71 @@ __global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_kernel(const uint
   uint4* po = reinterpret_cast<uint4*>(next + j * 8);
   po[0] = make_uint4(o[0], o[1], o[2], o[3]);
   po[1] = make_uint4(o[4], o[5], o[6], o[7]);
+}
+
+// Two lanes per row (p2pair.h): lane 0 takes words 0..11 of each 16-word chunk,
+// lane 1 words 12..15 (its words 0..3; its words 4..11 are the capacity).
+// ROW_WORDS is a multiple of 16.
+template <int ROW_WORDS>
+__global__ void __launch_bounds__(MERKLE_THREADS) merkle_leaves_pair_kernel(const uint32_t* __restrict__ rows,
+                                                                        uint64_t n_rows,
+                                                                        uint32_t* __restrict__ leaves) {
+  static_assert(ROW_WORDS % 16 == 0, "whole chunks");
+  const uint64_t i = ((uint64_t)blockIdx.x * MERKLE_THREADS + threadIdx.x) >> 1;
+  if (i >= n_rows) return;  // both lanes of a pair leave together
+  PairLane<24> L;
+  L.init();
+  const uint32_t* row = rows + i * ROW_WORDS;
+  uint32_t s[12];
+#pragma unroll
+  for (int j = 0; j < 12; ++j) s[j] = 0u;
+  constexpr int CHUNKS = ROW_WORDS / 16;
+  constexpr uint32_t OUT = CHUNKS > 1 ? 0xFF00FFu : 0xFFu;
+#pragma unroll 1
+  for (int c = 0; c < CHUNKS; ++c) {
+    const uint4* q = reinterpret_cast<const uint4*>(row + c * 16);
+    if (!L.h) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const uint4 v = q[j];
+        s[4 * j] = v.x;
+        s[4 * j + 1] = v.y;
+        s[4 * j + 2] = v.z;
+        s[4 * j + 3] = v.w;
+      }
+    } else {
+      const uint4 v = q[3];
+      s[0] = v.x;
+      s[1] = v.y;
+      s[2] = v.z;
+      s[3] = v.w;
+    }
+    permute_pair<24, OUT>(s, L);
+  }
+  if (!L.h) {
+    uint4* out = reinterpret_cast<uint4*>(leaves + i * 8);
+    out[0] = make_uint4(s[0], s[1], s[2], s[3]);
+    out[1] = make_uint4(s[4], s[5], s[6], s[7]);
+  }
+}
+
+// parent[j] = C(prev[j], prev[j + half]) on a lane pair: lane 0 holds the left
+// digest (words 0..7), lane 1 the right one (words 8..15).
+__global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_pair_kernel(const uint32_t* __restrict__ prev,
+                                                                       uint64_t half, uint32_t* __restrict__ next) {
+  const uint64_t j = ((uint64_t)blockIdx.x * MERKLE_THREADS + threadIdx.x) >> 1;
+  if (j >= half) return;
+  PairLane<16> L;
+  L.init();
+  const uint4* src = reinterpret_cast<const uint4*>(prev + (j + (L.h ? half : 0)) * 8);
+  const uint4 a = src[0], b = src[1];
+  uint32_t s[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  permute_pair<16, 0xFFu>(s, L);
+  if (!L.h) {
+    uint4* po = reinterpret_cast<uint4*>(next + j * 8);
+    po[0] = make_uint4(s[0], s[1], s[2], s[3]);
+    po[1] = make_uint4(s[4], s[5], s[6], s[7]);
+  }
 }
 
 // Upper layers with cooperative compressions (16 lanes per P16, tx.h), which

@@ -270,9 +270,25 @@ extern "C" int whir_dft_batch(whir_ctx* c, const uint32_t* d_in, uint64_t n_in_r
 
 // ============================ Merkle ============================
 
+// Trees of at most this many rows (compressions per layer) use two lanes per
+// permutation (p2pair.h): below ~2^16 one lane's instruction stream is exposed
+// (measured on MI355X: 2^15 rows of 64 words 55.7 -> 38.2 us, a 2^15-compression
+// layer 15.3 -> 12.0 us); from 2^17 up the lane-per-permutation kernels win.
+constexpr uint64_t MERKLE_PAIR_MAX = 1ull << 15;
+
 static int leaves_enqueue(const uint32_t* rows, uint64_t n_rows, uint32_t row_words, uint32_t* leaves,
                           hipStream_t s) {
   const dim3 grid((unsigned)((n_rows + MERKLE_THREADS - 1) / MERKLE_THREADS)), block(MERKLE_THREADS);
+  if (n_rows <= MERKLE_PAIR_MAX && row_words % 16 == 0) {
+    const dim3 grid2((unsigned)((2 * n_rows + MERKLE_THREADS - 1) / MERKLE_THREADS));
+    switch (row_words) {
+#define LEAF_PAIR_CASE(W) \
+  case W: hipLaunchKernelGGL(merkle_leaves_pair_kernel<W>, grid2, block, 0, s, rows, n_rows, leaves); return WHIR_OK;
+      LEAF_PAIR_CASE(16) LEAF_PAIR_CASE(32) LEAF_PAIR_CASE(64) LEAF_PAIR_CASE(128)
+#undef LEAF_PAIR_CASE
+      default: break;
+    }
+  }
   switch (row_words) {
 #define LEAF_CASE(W) \
   case W: hipLaunchKernelGGL(merkle_leaves_kernel<W>, grid, block, 0, s, rows, n_rows, leaves); break;
@@ -299,8 +315,12 @@ static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, u
   uint64_t n = H;
   while (n > MERKLE_COOP_FROM) {
     const uint64_t half = n / 2;
-    hipLaunchKernelGGL(merkle_layer_kernel, dim3((unsigned)((half + MERKLE_THREADS - 1) / MERKLE_THREADS)),
-                       dim3(MERKLE_THREADS), 0, s, cur, half, cur + n * 8);
+    if (half <= MERKLE_PAIR_MAX)
+      hipLaunchKernelGGL(merkle_layer_pair_kernel, dim3((unsigned)((2 * half + MERKLE_THREADS - 1) / MERKLE_THREADS)),
+                         dim3(MERKLE_THREADS), 0, s, cur, half, cur + n * 8);
+    else
+      hipLaunchKernelGGL(merkle_layer_kernel, dim3((unsigned)((half + MERKLE_THREADS - 1) / MERKLE_THREADS)),
+                         dim3(MERKLE_THREADS), 0, s, cur, half, cur + n * 8);
     cur += n * 8;
     n = half;
   }
