@@ -461,32 +461,20 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t*
   }
 }
 
-// STIR indexes: n draws of `bits` bits, sorted and deduplicated (whir.rs:63-77).
-// The draws are a serial chain on wave 0; the sort is a bitonic network over
+// Sort s_v[0..n) and write its distinct values in ascending order to out[],
+// with the whole workgroup (s_scan: blockDim.x words). A bitonic network over
 // the next power of two (padding sorts last), then a workgroup prefix count
-// over "first of its value" flags gives each distinct index its rank.
-__global__ void __launch_bounds__(SERIAL_THREADS) stir_sample_kernel(DevTx* t, uint32_t* log, JobScalars* js,
-                                                                     uint32_t n, uint32_t bits, uint32_t* idx_out,
-                                                                     uint32_t* count_out) {
-  __shared__ uint32_t s_v[MAX_STIR];
-  __shared__ uint32_t s_scan[SERIAL_THREADS];
-  const uint32_t tid = threadIdx.x;
+// over "first of its value" flags gives each distinct value its rank. Returns
+// the number of distinct values (in every thread). n <= MAX_STIR.
+__device__ uint32_t sort_dedup_block(uint32_t* s_v, uint32_t n, uint32_t* s_scan, uint32_t* out) {
+  const uint32_t tid = threadIdx.x, T = blockDim.x;
   uint32_t np = 1;
   while (np < n) np <<= 1;
-  if (is_wave0()) {
-    WaveTx tx;
-    tx_begin(tx, t, log, js);
-    for (uint32_t i = 0; i < n; ++i) {
-      const uint32_t v = tx.sample_bits(bits);
-      if (tid == 0) s_v[i] = v;
-    }
-    tx_end(tx, t, js);
-  }
-  for (uint32_t i = n + tid; i < np; i += SERIAL_THREADS) s_v[i] = 0xFFFFFFFFu;  // > any index (bits <= 31)
+  for (uint32_t i = n + tid; i < np; i += T) s_v[i] = 0xFFFFFFFFu;  // > any index (bits <= 31)
   __syncthreads();
   for (uint32_t size = 2; size <= np; size <<= 1) {
     for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      for (uint32_t i = tid; i < np / 2; i += SERIAL_THREADS) {
+      for (uint32_t i = tid; i < np / 2; i += T) {
         const uint32_t lo = 2 * i - (i & (stride - 1));  // index with bit `stride` clear
         const uint32_t hi = lo + stride;
         const bool up = (lo & size) == 0;
@@ -500,13 +488,13 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_sample_kernel(DevTx* t, u
     }
   }
   // per thread: a contiguous chunk of the sorted values; count the firsts
-  const uint32_t chunk = (n + SERIAL_THREADS - 1) / SERIAL_THREADS;
+  const uint32_t chunk = (n + T - 1) / T;
   const uint32_t b0 = tid * chunk, b1 = b0 + chunk < n ? b0 + chunk : n;
   uint32_t cnt = 0;
   for (uint32_t i = b0; i < b1; ++i) cnt += (i == 0 || s_v[i] != s_v[i - 1]) ? 1u : 0u;
   s_scan[tid] = cnt;
   __syncthreads();
-  for (uint32_t off = 1; off < SERIAL_THREADS; off <<= 1) {
+  for (uint32_t off = 1; off < T; off <<= 1) {
     const uint32_t add = tid >= off ? s_scan[tid - off] : 0u;
     __syncthreads();
     s_scan[tid] += add;
@@ -514,11 +502,46 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_sample_kernel(DevTx* t, u
   }
   uint32_t rank = s_scan[tid] - cnt;  // exclusive prefix
   for (uint32_t i = b0; i < b1; ++i)
-    if (i == 0 || s_v[i] != s_v[i - 1]) idx_out[rank++] = s_v[i];
-  if (tid == SERIAL_THREADS - 1) {
-    *count_out = s_scan[tid];
-    js->stir_count = s_scan[tid];
+    if (i == 0 || s_v[i] != s_v[i - 1]) out[rank++] = s_v[i];
+  const uint32_t total = s_scan[T - 1];
+  __syncthreads();
+  return total;
+}
+
+// STIR indexes: n draws of `bits` bits, sorted and deduplicated (whir.rs:63-77).
+// The draws are a serial chain on wave 0.
+__global__ void __launch_bounds__(SERIAL_THREADS) stir_sample_kernel(DevTx* t, uint32_t* log, JobScalars* js,
+                                                                     uint32_t n, uint32_t bits, uint32_t* idx_out,
+                                                                     uint32_t* count_out) {
+  __shared__ uint32_t s_v[MAX_STIR];
+  __shared__ uint32_t s_scan[SERIAL_THREADS];
+  if (is_wave0()) {
+    WaveTx tx;
+    tx_begin(tx, t, log, js);
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t v = tx.sample_bits(bits);
+      if (threadIdx.x == 0) s_v[i] = v;
+    }
+    tx_end(tx, t, js);
   }
+  const uint32_t count = sort_dedup_block(s_v, n, s_scan, idx_out);
+  if (threadIdx.x == 0) {
+    *count_out = count;
+    js->stir_count = count;
+  }
+}
+
+// Word w of query `index`'s opening: the row (row_words words), then the
+// `depth` sibling digests bottom-up (sibling at level i of the node on the
+// path = (index mod 2^(depth-i)) ^ 2^(depth-i-1); layers stored leaves first).
+__device__ __forceinline__ uint32_t opening_word(const uint32_t* cw, uint32_t row_words, const uint32_t* tree,
+                                                 uint32_t depth, uint32_t index, uint32_t w) {
+  if (w < row_words) return cw[(uint64_t)index * row_words + w];
+  const uint32_t e = w - row_words, i = e >> 3;
+  const uint32_t mid = 1u << (depth - i - 1);
+  const uint32_t sib = (index & ((mid << 1) - 1)) ^ mid;
+  const uint64_t layer_off = (2ull << depth) - (2ull << (depth - i));  // sum_{j<i} 2^(depth-j)
+  return tree[(layer_off + sib) * 8 + (e & 7)];
 }
 
 // Merkle openings of the sorted queries (MerkleTree::query, poseidon_packed.rs:
@@ -541,18 +564,9 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_query_kernel(const DevTx*
   const uint32_t index = idx[q];
   const uint64_t base = (uint64_t)t->log_words + (uint64_t)q * per;
   const bool fits = base + per <= t->log_cap;
-  if (fits) {
-    for (uint32_t w = threadIdx.x; w < row_words; w += SERIAL_THREADS)
-      log[base + w] = cw[(uint64_t)index * row_words + w];
-    for (uint32_t e = threadIdx.x; e < 8 * depth; e += SERIAL_THREADS) {
-      const uint32_t i = e >> 3, w = e & 7;
-      const uint32_t mid = 1u << (depth - i - 1);
-      const uint32_t cur = index & ((mid << 1) - 1);
-      const uint32_t sib = cur ^ mid;
-      const uint64_t layer_off = (2ull << depth) - (2ull << (depth - i));  // sum_{j<i} 2^(depth-j)
-      log[base + row_words + e] = tree[(layer_off + sib) * 8 + w];
-    }
-  }
+  if (fits)
+    for (uint32_t w = threadIdx.x; w < per; w += SERIAL_THREADS)
+      log[base + w] = opening_word(cw, row_words, tree, depth, index, w);
   if (mode == 1) {
     // rows index mod M + t*M, M = 2^(depth - d); each row has 2^f = row_words base words
     const uint32_t M = 1u << (depth - d);

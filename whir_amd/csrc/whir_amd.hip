@@ -15,6 +15,7 @@
 #include "merkle.h"
 #include "sumcheck.h"
 #include "protocol.h"
+#include "round_small.h"
 #include "host_params.h"
 #include "host_rng.h"
 #include "stack.h"
@@ -858,6 +859,17 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
     if ((rc = merkle_enqueue(cw, rp.log_rows, 4 * W, tree, s))) return rc;
     const uint64_t Hr = 1ull << rp.log_rows;
     const uint32_t* root = tree + (2 * Hr - 2) * 8;
+    if (r > 0 && small_round_fits(rp.kc, rp.n_ood, rp.n_stir)) {
+      // the whole round in one workgroup (round_small.h)
+      SmallRoundArgs sa{root, P->ep[cur], P->ew[cur], P->ep[cur ^ 1], P->ew[cur ^ 1], (uint32_t)rp.kc,
+                        (uint32_t)rp.n_ood, (uint32_t)rp.n_stir, (uint32_t)rp.kfd, (uint32_t)f, rs_idx, 4 * W,
+                        P->cw[(r - 1) & 1], P->tree[(r - 1) & 1]};
+      hipLaunchKernelGGL(small_round_kernel, dim3(1), dim3(SR_THREADS), 0, s, P->tx, P->log, P->js, sa);
+      n >>= f;
+      cur ^= f & 1;
+      rs_idx += f;
+      continue;
+    }
     // OOD claims (whir.rs:224-235), the first one drawn in the root's kernel
     const uint32_t om0 = rp.kc / 2, om1 = rp.kc - rp.kc / 2;
     auto ood_claim = [&](int o) {
@@ -936,21 +948,14 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
                          (const Ext*)nullptr, 0u, (uint64_t)0, P->powL, P->powR, P->stir_count, pm1);
     }
   }
-  // ---- final polynomial and queries (whir.rs:314-355) ----
-  hipLaunchKernelGGL(write_final_kernel, dim3(1), dim3(64), 0, s, P->tx, P->log, P->js, P->ep[cur], (uint32_t)n);
-  hipLaunchKernelGGL(stir_sample_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js,
-                     (uint32_t)pl.final_n_stir, (uint32_t)pl.final_kfd, P->stir_idx, P->stir_count);
-  if (pl.n_rounds > 0) {
+  // ---- final polynomial, queries and the test's checkpoint (whir.rs:314-355, test.rs:233) ----
+  {
     const int lr = pl.n_rounds - 1;
-    hipLaunchKernelGGL(stir_query_kernel, dim3(pl.final_n_stir), dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log,
-                       P->stir_idx, P->stir_count, P->cw[lr & 1], 4 * W, P->tree[lr & 1], (uint32_t)pl.final_kfd, 0,
-                       (uint32_t)f, P->qa);
-  } else {
-    hipLaunchKernelGGL(stir_query_kernel, dim3(pl.final_n_stir), dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log,
-                       P->stir_idx, P->stir_count, P->cw0, W, P->tree0, (uint32_t)pl.final_kfd, 0, (uint32_t)f,
-                       P->qa);
+    const bool base = pl.n_rounds == 0;
+    hipLaunchKernelGGL(final_phase_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->ep[cur],
+                       (uint32_t)n, (uint32_t)pl.final_n_stir, (uint32_t)pl.final_kfd,
+                       base ? P->cw0 : P->cw[lr & 1], base ? W : 4 * W, base ? P->tree0 : P->tree[lr & 1]);
   }
-  hipLaunchKernelGGL(checkpoint_kernel, dim3(1), dim3(64), 0, s, P->tx, P->log, P->js);
   HIP_TRY(hipGetLastError());
   return WHIR_OK;
 }
