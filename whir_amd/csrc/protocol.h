@@ -518,10 +518,7 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_sample_kernel(DevTx* t, u
   if (is_wave0()) {
     WaveTx tx;
     tx_begin(tx, t, log, js);
-    for (uint32_t i = 0; i < n; ++i) {
-      const uint32_t v = tx.sample_bits(bits);
-      if (threadIdx.x == 0) s_v[i] = v;
-    }
+    tx.sample_bits_many(n, bits, s_v);
     tx_end(tx, t, js);
   }
   const uint32_t count = sort_dedup_block(s_v, n, s_scan, idx_out);

@@ -148,11 +148,7 @@ __global__ void __launch_bounds__(SR_THREADS) small_round_kernel(DevTx* t, uint3
   }
 
   // ---- STIR queries: draws, sort, openings of the previous tree as hints (whir.rs:237-253) ----
-  if (is_wave0())
-    for (uint32_t i = 0; i < a.n_stir; ++i) {
-      const uint32_t v = tx.sample_bits(a.kfd);
-      if (tid == 0) s_v[i] = v;
-    }
+  if (is_wave0()) tx.sample_bits_many(a.n_stir, a.kfd, s_v);
   const uint32_t count = sort_dedup_block(s_v, a.n_stir, s_scan, s_idx);
   const uint32_t per = a.row_words + 8 * a.kfd;
   if (is_wave0()) {
@@ -297,10 +293,7 @@ __global__ void __launch_bounds__(SERIAL_THREADS) final_phase_kernel(DevTx* t, u
         tx.write_e(e);
       }
     }
-    for (uint32_t i = 0; i < n_stir; ++i) {
-      const uint32_t v = tx.sample_bits(kfd);
-      if (tid == 0) s_v[i] = v;
-    }
+    tx.sample_bits_many(n_stir, kfd, s_v);
   }
   const uint32_t count = sort_dedup_block(s_v, n_stir, s_scan, s_idx);
   const uint32_t per = row_words + 8 * kfd;

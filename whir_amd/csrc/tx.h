@@ -195,6 +195,22 @@ struct WaveTx {
   __device__ __forceinline__ uint32_t sample_bits(uint32_t bits) {
     return canonical(sample()) & ((1u << bits) - 1u);
   }
+  // out[0..n) = n successive sample_bits(bits). After a duplex the next pops
+  // are state[out_len - 1], state[out_len - 2], ..., so every draw that one
+  // duplex serves is taken at once by the lane holding its word.
+  __device__ __forceinline__ void sample_bits_many(uint32_t n, uint32_t bits, uint32_t* out) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t mask = (1u << bits) - 1u;
+    if (n == 0) return;
+    if (in_len != 0) duplex();
+    for (uint32_t i = 0; i < n;) {
+      if (out_len == 0) duplex();
+      const uint32_t take = out_len < n - i ? out_len : n - i;
+      if (lane < out_len && lane + take >= out_len) out[i + (out_len - 1 - lane)] = canonical(s) & mask;
+      i += take;
+      out_len -= take;
+    }
+  }
   __device__ __forceinline__ void hint_word(uint32_t v) {
     if ((threadIdx.x & 63) == 0) {
       if (log_words < log_cap) log[log_words] = v;
