@@ -46,14 +46,6 @@ struct QueryArrays {
   Ext* G;           // fold-with-code values, 2^d per query
 };
 
-__device__ __forceinline__ Ext wave_sum_ext(Ext v) {
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) v.c[c] = add(v.c[c], (uint32_t)__shfl_xor((int)v.c[c], m));
-  }
-  return v;
-}
 
 __device__ __forceinline__ bool is_wave0() { return threadIdx.x < 64; }
 
@@ -636,9 +628,19 @@ __global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(
   __shared__ Ext s_r;
   const uint32_t tid = threadIdx.x;
   Ext c0 = ezero(), c2 = ezero();
-  for (uint32_t p = tid; p < n_part; p += SERIAL_THREADS) {
-    c0 = eadd(c0, ld_ext(partials + 2 * p));
-    c2 = eadd(c2, ld_ext(partials + 2 * p + 1));
+  for (uint32_t p0 = tid; p0 < n_part; p0 += 4 * SERIAL_THREADS) {  // four pairs in flight
+    Ext a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t p = p0 + (uint32_t)u * SERIAL_THREADS;
+      a[u] = p < n_part ? ld_ext(partials + 2 * p) : ezero();
+      b[u] = p < n_part ? ld_ext(partials + 2 * p + 1) : ezero();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      c0 = eadd(c0, a[u]);
+      c2 = eadd(c2, b[u]);
+    }
   }
   const uint32_t count = fwc ? *count_ptr : 0;
   const uint32_t GS = 1u << MAX_F;

@@ -47,18 +47,28 @@ __device__ __forceinline__ Ext ld_ext(const Ext* p) {
 __device__ __forceinline__ void st_ext(Ext* p, const Ext& e) {
   *reinterpret_cast<uint4*>(p) = make_uint4(e.c[0], e.c[1], e.c[2], e.c[3]);
 }
-// Sum `n` Ext values held one per thread (n = blockDim) into lane 0 of the block.
+// Sum over the 64 lanes of a wavefront (butterfly: every lane gets the sum).
+__device__ __forceinline__ Ext wave_sum_ext(Ext v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v.c[c] = add(v.c[c], (uint32_t)__shfl_xor((int)v.c[c], m));
+  }
+  return v;
+}
+
+// Sum of one Ext per thread over a block of NT threads, returned to every
+// thread: a butterfly inside each wavefront, then the NT/64 wave sums from LDS
+// (sh: NT/64 entries; one barrier). Two calls in a row need a barrier between.
 template <int NT>
 __device__ __forceinline__ Ext block_reduce_ext(Ext v, Ext* sh) {
-  const int t = threadIdx.x;
-  sh[t] = v;
+  v = wave_sum_ext(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
+  Ext s = sh[0];
 #pragma unroll
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if (t < s) sh[t] = eadd(sh[t], sh[t + s]);
-    __syncthreads();
-  }
-  return sh[0];
+  for (int i = 1; i < NT / 64; ++i) s = eadd(s, sh[i]);
+  return s;
 }
 
 // eq table of m <= 2 EQ_LDS_LOG variables into `out` (global, 2^m entries,
