@@ -151,14 +151,17 @@ __global__ void __launch_bounds__(SERIAL_THREADS) claim_begin_kernel(DevTx* t, u
 // partials[(c * n_part + p) * 2^fsv + i] (SvoPoint::eval, svo.rs:77-91);
 // fsv == 0: one value per block (eval_ext). Then begin the next group when
 // `next.slot` is set.
-__global__ void __launch_bounds__(SERIAL_THREADS) claim_end_kernel(DevTx* t, uint32_t* log, JobScalars* js,
-                                                                   ClaimSlot* slot, uint32_t count,
-                                                                   const Ext* partials, uint32_t n_part, uint32_t fsv,
-                                                                   ClaimBegin next) {
-  __shared__ Ext sh[SERIAL_THREADS];
-  __shared__ Ext s_eq[1 << MAX_F];
-  __shared__ Ext s_eval[MAX_CLAIMS];
-  __shared__ ClaimScratch sc;
+struct ClaimEndScratch {
+  Ext sh[SERIAL_THREADS];
+  Ext eq[1 << MAX_F];
+};
+
+// The values of the `count` claims of a group into s_eval[] (every thread
+// returns after the last barrier); slots get their compressed values and eval.
+__device__ void claim_end_values(ClaimSlot* slot, uint32_t count, const Ext* partials, uint32_t n_part, uint32_t fsv,
+                                 Ext* s_eval, ClaimEndScratch& cs) {
+  Ext* sh = cs.sh;
+  Ext* s_eq = cs.eq;
   const uint32_t W = 1u << fsv;
   const uint32_t tid = threadIdx.x;
   // eq(svo)[i] for i < W, svo = point[0..fsv) (the group's shared point)
@@ -213,6 +216,16 @@ __global__ void __launch_bounds__(SERIAL_THREADS) claim_end_kernel(DevTx* t, uin
     }
     __syncthreads();
   }
+}
+
+__global__ void __launch_bounds__(SERIAL_THREADS) claim_end_kernel(DevTx* t, uint32_t* log, JobScalars* js,
+                                                                   ClaimSlot* slot, uint32_t count,
+                                                                   const Ext* partials, uint32_t n_part, uint32_t fsv,
+                                                                   ClaimBegin next) {
+  __shared__ ClaimEndScratch cs;
+  __shared__ Ext s_eval[MAX_CLAIMS];
+  __shared__ ClaimScratch sc;
+  claim_end_values(slot, count, partials, n_part, fsv, s_eval, cs);
   WaveTx tx;
   if (is_wave0()) {
     tx_begin(tx, t, log, js);
@@ -500,24 +513,66 @@ __device__ uint32_t sort_dedup_block(uint32_t* s_v, uint32_t n, uint32_t* s_scan
   return total;
 }
 
-// STIR indexes: n draws of `bits` bits, sorted and deduplicated (whir.rs:63-77).
-// The draws are a serial chain on wave 0.
-__global__ void __launch_bounds__(SERIAL_THREADS) stir_sample_kernel(DevTx* t, uint32_t* log, JobScalars* js,
-                                                                     uint32_t n, uint32_t bits, uint32_t* idx_out,
-                                                                     uint32_t* count_out) {
+// The STIR step of a round up to the fold's alpha, in one kernel:
+//   [the last OOD claim's value: write it (whir.rs:230)]
+//   n draws of `bits` bits -> sorted distinct indexes (whir.rs:63-77, 238);
+//   draw alpha (prover.rs:377-384; the openings written afterwards are hints,
+//   which the sponge does not absorb, so alpha can be drawn before them);
+//   scale the eq1 table of OOD claim c >= 1 by alpha^c; query scales
+//   alpha^(n_eq + q) (combine.rs:37-64, prover.rs:419-425).
+struct StirBegin {
+  ClaimSlot* end_slot;  // OOD claim whose value this kernel writes first (nullptr: none)
+  const Ext* partials;  // its eval_ext partials
+  uint32_t n_part;
+  uint32_t n, bits;
+  uint32_t* idx_out;
+  uint32_t* count_out;
+  Ext* eq_pool;         // OOD claims' split eq tables (eq0: 2^m0 entries, then eq1)
+  uint64_t eq_stride;
+  uint32_t n_eq, m0, m1;
+  QueryArrays qa;
+};
+
+__global__ void __launch_bounds__(SERIAL_THREADS) stir_begin_kernel(DevTx* t, uint32_t* log, JobScalars* js,
+                                                                    StirBegin a) {
+  __shared__ ClaimEndScratch cs;
+  __shared__ Ext s_eval[1];
   __shared__ uint32_t s_v[MAX_STIR];
   __shared__ uint32_t s_scan[SERIAL_THREADS];
+  __shared__ Ext s_alpha;
+  __shared__ Ext s_apow[16];
+  const uint32_t tid = threadIdx.x;
+  if (a.end_slot) claim_end_values(a.end_slot, 1, a.partials, a.n_part, 0, s_eval, cs);
   if (is_wave0()) {
     WaveTx tx;
     tx_begin(tx, t, log, js);
-    tx.sample_bits_many(n, bits, s_v);
+    if (a.end_slot) tx.write_e(s_eval[0]);
+    tx.sample_bits_many(a.n, a.bits, s_v);
+    const Ext al = tx.sample_ext();
     tx_end(tx, t, js);
+    if (tid == 0) {
+      s_alpha = al;
+      js->alpha = al;
+      Ext p = eone();
+      for (uint32_t c = 0; c < a.n_eq && c < 16; ++c) {
+        s_apow[c] = p;
+        p = emul(p, al);
+      }
+    }
   }
-  const uint32_t count = sort_dedup_block(s_v, n, s_scan, idx_out);
-  if (threadIdx.x == 0) {
-    *count_out = count;
+  const uint32_t count = sort_dedup_block(s_v, a.n, s_scan, a.idx_out);  // barriers make s_alpha visible
+  if (tid == 0) {
+    *a.count_out = count;
     js->stir_count = count;
   }
+  const uint64_t n0 = 1ull << a.m0, n1 = 1ull << a.m1;
+  for (uint64_t it = tid; it < (uint64_t)a.n_eq * n1; it += SERIAL_THREADS) {
+    const uint64_t c = it / n1, x = it % n1;
+    if (c == 0) continue;
+    Ext* p = a.eq_pool + c * a.eq_stride + n0 + x;
+    st_ext(p, emul(ld_ext(p), s_apow[c]));
+  }
+  for (uint32_t q = tid; q < count; q += SERIAL_THREADS) a.qa.scale[q] = epow(s_alpha, (uint64_t)a.n_eq + q);
 }
 
 // Word w of query `index`'s opening: the row (row_words words), then the
@@ -539,12 +594,15 @@ __device__ __forceinline__ uint32_t opening_word(const uint32_t* cw, uint32_t ro
 // blocks write in parallel and the next transcript kernel advances the log.
 // mode 1 (round 0, fold_with_code): also G[t] = <eq(rs), row(index mod M + t M)>
 // for t < 2^d (prover.rs:483-506) and ws/ts (prover.rs:463-481).
-// mode 2 (later rounds): pow-claim variable omega^index (whir.rs:242-246).
+// mode 2 (later rounds): pow-claim variable var = omega^index (whir.rs:242-246)
+// and the query's pow tables for combine_kernel: powL[q][e] = var^e (e < 2^m0),
+// powR[q][e] = scale_q var^(e 2^m0) (e < 2^m1).
 __global__ void __launch_bounds__(SERIAL_THREADS) stir_query_kernel(const DevTx* t, JobScalars* js, uint32_t* log,
                                                                     const uint32_t* idx, const uint32_t* count_ptr,
                                                                     const uint32_t* cw, uint32_t row_words,
                                                                     const uint32_t* tree, uint32_t depth, int mode,
-                                                                    uint32_t d, QueryArrays qa) {
+                                                                    uint32_t d, QueryArrays qa, uint32_t m0,
+                                                                    uint32_t m1, uint32_t* powL, Ext* powR) {
   const uint32_t count = *count_ptr;
   const uint32_t q = blockIdx.x;
   const uint32_t per = row_words + 8 * depth;
@@ -573,48 +631,18 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_query_kernel(const DevTx*
       qa.ts[q] = inv(pow(om, index & ((1u << (depth - 1)) - 1)));
     }
   } else if (mode == 2) {
+    const uint32_t var = pow(two_adic_generator(depth), index);
     if (threadIdx.x == 0) {
       qa.index[q] = index;
-      qa.var[q] = pow(two_adic_generator(depth), index);
+      qa.var[q] = var;
+    }
+    const uint32_t n0 = 1u << m0, n1 = 1u << m1;
+    const Ext scale = qa.scale[q];
+    for (uint32_t e = threadIdx.x; e < n0 + n1; e += SERIAL_THREADS) {
+      if (e < n0) powL[(uint64_t)q * n0 + e] = pow(var, e);
+      else st_ext(powR + (uint64_t)q * n1 + (e - n0), emul_base(scale, pow(var, (uint64_t)(e - n0) << m0)));
     }
   }
-}
-
-// SumcheckProver::fold / fold_with_code prologue (prover.rs:377-384, 399-419):
-// draw alpha; scale the eq1 table of OOD claim c by alpha^c; query scales
-// alpha^(n_eq + q).
-__global__ void __launch_bounds__(SERIAL_THREADS) fold_begin_kernel(DevTx* t, uint32_t* log, JobScalars* js,
-                                                                    Ext* eq_pool, uint64_t eq_stride, uint32_t n_eq,
-                                                                    uint32_t m0, uint32_t m1, QueryArrays qa,
-                                                                    const uint32_t* count_ptr) {
-  __shared__ Ext s_alpha;
-  __shared__ Ext s_apow[16];
-  const uint32_t tid = threadIdx.x;
-  if (is_wave0()) {
-    WaveTx tx;
-    tx_begin(tx, t, log, js);
-    const Ext a = tx.sample_ext();
-    tx_end(tx, t, js);
-    if (tid == 0) {
-      s_alpha = a;
-      js->alpha = a;
-      Ext p = eone();
-      for (uint32_t c = 0; c < n_eq && c < 16; ++c) {
-        s_apow[c] = p;
-        p = emul(p, a);
-      }
-    }
-  }
-  __syncthreads();
-  const uint64_t n0 = 1ull << m0, n1 = 1ull << m1;
-  for (uint64_t it = tid; it < (uint64_t)n_eq * n1; it += SERIAL_THREADS) {
-    const uint64_t c = it / n1, x = it % n1;
-    if (c == 0) continue;
-    Ext* p = eq_pool + c * eq_stride + n0 + x;
-    st_ext(p, emul(ld_ext(p), s_apow[c]));
-  }
-  const uint32_t count = count_ptr ? *count_ptr : 0;
-  for (uint32_t q = tid; q < count; q += SERIAL_THREADS) qa.scale[q] = epow(s_alpha, (uint64_t)n_eq + q);
 }
 
 // Reduce the (c0, c2) partials, add the fold_with_code query terms

@@ -895,26 +895,38 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
       const ClaimBegin cb = ood_claim(o);
       hipLaunchKernelGGL(eval_ext_kernel, dim3(1u << om1), dim3(SC_THREADS), 0, s, P->ep[cur], cb.eq0, cb.eq1, om0,
                          P->partials);
-      hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, cb.slot, 1u,
-                         P->partials, 1u << om1, 0u, ood_claim(o + 1));
+      if (o + 1 < rp.n_ood)
+        hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, cb.slot, 1u,
+                           P->partials, 1u << om1, 0u, ood_claim(o + 1));
     }
-    // STIR queries (whir.rs:237-293)
-    hipLaunchKernelGGL(stir_sample_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js,
-                       (uint32_t)rp.n_stir, (uint32_t)rp.kfd, P->stir_idx, P->stir_count);
+    // STIR draws (whir.rs:237-241) and the fold's alpha (prover.rs:377-384),
+    // after the last OOD claim's value
+    StirBegin sb{};
+    sb.end_slot = rp.n_ood > 0 ? P->slots + P->n_svo_claims + (rp.n_ood - 1) : nullptr;
+    sb.partials = P->partials;
+    sb.n_part = 1u << om1;
+    sb.n = (uint32_t)rp.n_stir;
+    sb.bits = (uint32_t)rp.kfd;
+    sb.idx_out = P->stir_idx;
+    sb.count_out = P->stir_count;
+    sb.eq_pool = P->ood_pool;
+    sb.eq_stride = P->ood_stride;
+    sb.n_eq = (uint32_t)rp.n_ood;
+    sb.m0 = om0;
+    sb.m1 = om1;
+    sb.qa = P->qa;
+    hipLaunchKernelGGL(stir_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, sb);
+    // STIR openings (whir.rs:242-293); later rounds also build the pow tables
+    // of combine_pows_scaled (prover.rs:419-425)
     if (r == 0) {
       hipLaunchKernelGGL(stir_query_kernel, dim3(rp.n_stir), dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log,
-                         P->stir_idx, P->stir_count, P->cw0, W, P->tree0, (uint32_t)rp.kfd, 1, (uint32_t)f, P->qa);
+                         P->stir_idx, P->stir_count, P->cw0, W, P->tree0, (uint32_t)rp.kfd, 1, (uint32_t)f, P->qa,
+                         0u, 0u, (uint32_t*)nullptr, (Ext*)nullptr);
     } else {
       hipLaunchKernelGGL(stir_query_kernel, dim3(rp.n_stir), dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log,
                          P->stir_idx, P->stir_count, P->cw[(r - 1) & 1], 4 * W, P->tree[(r - 1) & 1],
-                         (uint32_t)rp.kfd, 2, (uint32_t)f, P->qa);
+                         (uint32_t)rp.kfd, 2, (uint32_t)f, P->qa, om0, om1, P->powL, P->powR);
     }
-    // fold / fold_with_code (prover.rs:377-561)
-    hipLaunchKernelGGL(fold_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->ood_pool,
-                       P->ood_stride, (uint32_t)rp.n_ood, om0, om1, P->qa, P->stir_count);
-    if (r > 0)
-      hipLaunchKernelGGL(pow_tables_kernel, dim3(blocks_for((uint64_t)rp.n_stir * ((1ull << om0) + (1ull << om1)))),
-                         dim3(SC_THREADS), 0, s, P->qa.var, P->qa.scale, P->stir_count, om0, om1, P->powL, P->powR);
     hipLaunchKernelGGL(combine_kernel, dim3(blocks_for(n * COMBINE_SPLIT)), dim3(SC_THREADS), 0, s, P->ew[cur], n, om0, P->ood_pool,
                        (uint32_t)rp.n_ood, P->ood_stride, P->powL, P->powR, r > 0 ? P->stir_count : nullptr, om1);
     for (int t = 0; t < f; ++t) {
