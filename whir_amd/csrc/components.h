@@ -24,14 +24,13 @@ __global__ void __launch_bounds__(SERIAL_THREADS) eq_split_kernel(const Ext* __r
                                                                   uint32_t m1, Ext* pool, uint64_t stride) {
   __shared__ Ext s_z[MAX_K];
   __shared__ Ext s_scale;
-  __shared__ Ext tl[1 << EQ_LDS_LOG], th[1 << EQ_LDS_LOG];
+  __shared__ EqPairScratch sc;
   const uint32_t j = blockIdx.x;
   for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) s_z[i] = ld_ext(points + (uint64_t)j * k + i);
   if (threadIdx.x == 0) s_scale = emul(ext_of(scale), epow(ext_of(alpha), j));
   __syncthreads();
   Ext* eq0 = pool + j * stride;
-  build_eq_block(eq0, s_z, (int)m0, eone(), tl, th);
-  build_eq_block(eq0 + (1ull << m0), s_z + m0, (int)m1, s_scale, tl, th);
+  build_eq_pair(eq0, s_z, (int)m0, eone(), eq0 + (1ull << m0), s_z + m0, (int)m1, s_scale, sc);
 }
 
 // out[x] = eq0[x mod 2^m0] * eq1[x >> m0]

@@ -102,13 +102,27 @@ KB_HD Ext emul(const Ext& a, const Ext& b) {
   uint32_t r3 = reduce_sum4(a0 * b3 + a1 * b2 + a2 * b1 + a3 * b0);
   return Ext{{r0, r1, r2, r3}};
 }
-KB_HD Ext esqr(const Ext& a) { return emul(a, a); }
+// Square: the ten distinct products a_i a_j, the cross terms taken once with
+// a doubled factor (2 a_i < 2p < 2^32), so every sum stays below 4 p^2.
+KB_HD Ext esqr(const Ext& a) {
+  const uint64_t a0 = a.c[0], a1 = a.c[1], a2 = a.c[2], a3 = a.c[3];
+  const uint64_t d0 = 2 * a0, d1 = 2 * a1, d2 = 2 * a2;
+  uint32_t t4 = reduce_sum4(d1 * a3 + a2 * a2);
+  uint32_t t5 = reduce_sum4(d2 * a3);
+  uint32_t t6 = reduce64(a3 * a3);
+  uint32_t w4 = add(dbl(t4), t4), w5 = add(dbl(t5), t5), w6 = add(dbl(t6), t6);
+  uint32_t r0 = add(reduce64(a0 * a0), w4);
+  uint32_t r1 = add(reduce_sum4(d0 * a1), w5);
+  uint32_t r2 = add(reduce_sum4(d0 * a2 + a1 * a1), w6);
+  uint32_t r3 = reduce_sum4(d0 * a3 + d1 * a2);
+  return Ext{{r0, r1, r2, r3}};
+}
 KB_HD Ext epow(Ext a, uint64_t e) {
   Ext r = eone();
   while (e) {
     if (e & 1) r = emul(r, a);
-    a = emul(a, a);
     e >>= 1;
+    if (e) a = esqr(a);
   }
   return r;
 }

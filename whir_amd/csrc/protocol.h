@@ -93,8 +93,7 @@ struct ClaimBegin {
 
 struct ClaimScratch {
   Ext pt[MAX_K];
-  Ext tl[1 << EQ_LDS_LOG];
-  Ext th[1 << EQ_LDS_LOG];
+  EqPairScratch eq;
 };
 
 // Draw the claim point: point = expand(k, draw()) (test.rs:225-226, prover.rs:240,
@@ -112,8 +111,8 @@ __device__ void claim_begin_body(WaveTx& tx, const ClaimBegin& cb, ClaimScratch&
   __syncthreads();
   for (uint32_t it = threadIdx.x; it < cb.count * cb.k; it += blockDim.x) cb.slot[it / cb.k].point[it % cb.k] = sc.pt[it % cb.k];
   for (uint32_t c = threadIdx.x; c < cb.count; c += blockDim.x) cb.slot[c].k = cb.k;
-  build_eq_block(cb.eq0, sc.pt + cb.fsv, (int)cb.m0, eone(), sc.tl, sc.th);
-  build_eq_block(cb.eq1, sc.pt + cb.fsv + cb.m0, (int)cb.m1, eone(), sc.tl, sc.th);
+  build_eq_pair(cb.eq0, sc.pt + cb.fsv, (int)cb.m0, eone(), cb.eq1, sc.pt + cb.fsv + cb.m0, (int)cb.m1, eone(),
+                sc.eq);
 }
 
 // transcript.write(root) (poseidon_packed.rs:136-140): 8 hint words, then
@@ -158,6 +157,7 @@ struct ClaimEndScratch {
 
 // The values of the `count` claims of a group into s_eval[] (every thread
 // returns after the last barrier); slots get their compressed values and eval.
+// n_part is a power of two.
 __device__ void claim_end_values(ClaimSlot* slot, uint32_t count, const Ext* partials, uint32_t n_part, uint32_t fsv,
                                  Ext* s_eval, ClaimEndScratch& cs) {
   Ext* sh = cs.sh;
@@ -173,13 +173,16 @@ __device__ void claim_end_values(ClaimSlot* slot, uint32_t count, const Ext* par
     }
     s_eq[tid] = e;
   }
-  // A pass covers cpp claims with S = 256 / cpp threads each (S a multiple of
-  // W). Thread (claim c, sub) sums elements sub, sub + S, ... of claim c's
-  // n_part * W partials (all in column sub mod W): consecutive threads read
-  // consecutive 16 B, eight loads in flight.
+  // A pass covers cpp claims with S = 256 / cpp threads each (S, W powers of
+  // two, W <= S). Thread (claim c, sub) sums elements sub, sub + S, ... of
+  // claim c's n_part * W partials (all in column sub mod W): consecutive
+  // threads read consecutive 16 B, eight loads in flight. The columns are then
+  // summed by a butterfly over the claim's lanes of each wavefront (span
+  // lanes) and across its wpc wavefronts through LDS.
   const uint32_t per_claim = n_part * W;
   const uint32_t cpp = per_claim >= SERIAL_THREADS ? 1u : SERIAL_THREADS / per_claim;
   const uint32_t S = SERIAL_THREADS / cpp;
+  const uint32_t span = S < 64 ? S : 64, wpc = S / span;
   for (uint32_t base = 0; base < count; base += cpp) {
     const uint32_t c = base + tid / S, sub = tid % S;
     Ext acc = ezero();
@@ -195,24 +198,30 @@ __device__ void claim_end_values(ClaimSlot* slot, uint32_t count, const Ext* par
       }
       for (; e < per_claim; e += S) acc = eadd(acc, ld_ext(pc + e));
     }
-    sh[tid] = acc;
-    __syncthreads();
-    // column i of claim c: the S / W threads with sub = i mod W
-    const uint32_t ci = tid / W, i = tid % W;  // ci < cpp when tid < cpp * W
-    Ext comp = ezero();
-    if (ci < cpp && base + ci < count) {
-      for (uint32_t r = i; r < S; r += W) comp = eadd(comp, sh[ci * S + r]);
-      slot[base + ci].compressed[i] = comp;
+    for (uint32_t m = W; m < span; m <<= 1) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc.c[q] = add(acc.c[q], (uint32_t)__shfl_xor((int)acc.c[q], (int)m));
     }
+    if (sub % span < W) sh[(tid / S) * wpc * W + (sub / span) * W + sub % W] = acc;
     __syncthreads();
-    if (ci < cpp) sh[tid] = emul(comp, s_eq[i]);
-    __syncthreads();
-    // one thread per claim of the pass: eval = sum_i eq(svo)[i] * compressed[i]
-    if (tid < cpp && base + tid < count) {
-      Ext e = ezero();
-      for (uint32_t j = 0; j < W; ++j) e = eadd(e, sh[tid * W + j]);
-      s_eval[base + tid] = e;
-      slot[base + tid].eval = e;
+    // thread ci * W + i: column i of claim base + ci; eval by a butterfly over
+    // the claim's W consecutive lanes
+    const uint32_t ci = tid / W, i = tid % W;
+    const bool mine = ci < cpp && base + ci < count;
+    Ext prod = ezero();
+    if (mine) {
+      Ext comp = sh[ci * wpc * W + i];
+      for (uint32_t j = 1; j < wpc; ++j) comp = eadd(comp, sh[ci * wpc * W + j * W + i]);
+      slot[base + ci].compressed[i] = comp;
+      prod = emul(comp, s_eq[i]);
+    }
+    for (uint32_t m = 1; m < W; m <<= 1) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) prod.c[q] = add(prod.c[q], (uint32_t)__shfl_xor((int)prod.c[q], (int)m));
+    }
+    if (mine && i == 0) {
+      s_eval[base + ci] = prod;
+      slot[base + ci].eval = prod;
     }
     __syncthreads();
   }
@@ -225,12 +234,11 @@ __global__ void __launch_bounds__(SERIAL_THREADS) claim_end_kernel(DevTx* t, uin
   __shared__ ClaimEndScratch cs;
   __shared__ Ext s_eval[MAX_CLAIMS];
   __shared__ ClaimScratch sc;
-  claim_end_values(slot, count, partials, n_part, fsv, s_eval, cs);
   WaveTx tx;
-  if (is_wave0()) {
-    tx_begin(tx, t, log, js);
+  if (is_wave0()) tx_begin(tx, t, log, js);  // the sponge's loads overlap the reduction
+  claim_end_values(slot, count, partials, n_part, fsv, s_eval, cs);
+  if (is_wave0())
     for (uint32_t c = 0; c < count; ++c) tx.write_e(s_eval[c]);
-  }
   if (next.slot) claim_begin_body(tx, next, sc);
   if (is_wave0()) tx_end(tx, t, js);
 }
@@ -270,6 +278,8 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t*
     pow3 *= 3;
   }
   const uint32_t pe_per = 2 * W - 2;
+  WaveTx tx;
+  if (is_wave0()) tx_begin(tx, t, log, js);
 
   // (1) partial evals: pe[l][a] = sum_b comp[b*2^l + a] * eq(svo[l..f))[b]
   for (uint32_t it = tid; it < n_claims * pe_per; it += SERIAL_THREADS) {
@@ -296,9 +306,7 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t*
   }
 
   // (2) draw alpha; alpha^rank per claim
-  WaveTx tx;
   if (is_wave0()) {
-    tx_begin(tx, t, log, js);
     const Ext alpha = tx.sample_ext();
     if (tid == 0) {
       s_alpha = alpha;
@@ -542,10 +550,10 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_begin_kernel(DevTx* t, ui
   __shared__ Ext s_alpha;
   __shared__ Ext s_apow[16];
   const uint32_t tid = threadIdx.x;
+  WaveTx tx;
+  if (is_wave0()) tx_begin(tx, t, log, js);
   if (a.end_slot) claim_end_values(a.end_slot, 1, a.partials, a.n_part, 0, s_eval, cs);
   if (is_wave0()) {
-    WaveTx tx;
-    tx_begin(tx, t, log, js);
     if (a.end_slot) tx.write_e(s_eval[0]);
     tx.sample_bits_many(a.n, a.bits, s_v);
     const Ext al = tx.sample_ext();
@@ -655,6 +663,8 @@ __global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(
   __shared__ Ext sh[SERIAL_THREADS];
   __shared__ Ext s_r;
   const uint32_t tid = threadIdx.x;
+  WaveTx tx;
+  if (is_wave0()) tx_begin(tx, t, log, js);
   Ext c0 = ezero(), c2 = ezero();
   for (uint32_t p0 = tid; p0 < n_part; p0 += 4 * SERIAL_THREADS) {  // four pairs in flight
     Ext a[4], b[4];
@@ -690,8 +700,6 @@ __global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(
   __syncthreads();
   const Ext C2 = block_reduce_ext<SERIAL_THREADS>(c2, sh);
   if (is_wave0()) {
-    WaveTx tx;
-    tx_begin(tx, t, log, js);
     tx.write_e(C0);
     tx.write_e(C2);
     const Ext r = tx.sample_ext();

@@ -71,39 +71,75 @@ __device__ __forceinline__ Ext block_reduce_ext(Ext v, Ext* sh) {
   return s;
 }
 
-// eq table of m <= 2 EQ_LDS_LOG variables into `out` (global, 2^m entries,
-// times `scale`), whole block cooperating: a low table over the first
-// a = min(m, EQ_LDS_LOG) variables and a high table over the rest are doubled
-// in LDS (sL, sH: 2^EQ_LDS_LOG entries each), then one parallel pass writes
-// out[x] = sL[x mod 2^a] * sH[x >> a].
+// Split eq tables of two points at once, into global memory, whole block
+// (>= 128 threads) cooperating: table u (u < 2, m_u <= 2 EQ_LDS_LOG variables,
+// times scale_u) is eq(z_u)[x] = prod_j (x_j ? z_j : 1 - z_j). It is the
+// product of four sub-tables of <= 4 variables each (variables [0, a0),
+// [a0, a), [a, a + b0), [a + b0, m) with a = min(m, 8), a0 = min(a, 4),
+// b0 = min(m - a, 4)), each entry a product of its factors; pairs of
+// sub-tables are multiplied into the low (2^a) and high (2^(m - a)) tables in
+// LDS, and out[x] = lo[x mod 2^a] * hi[x >> a]. Two barriers; the scratch may
+// be reused after a further barrier.
 constexpr int EQ_LDS_LOG = 8;
+struct EqPairScratch {
+  Ext part[2][4][16];
+  Ext lo[2][1 << EQ_LDS_LOG];
+  Ext hi[2][1 << EQ_LDS_LOG];
+};
 
-__device__ void build_eq_block(Ext* out, const Ext* z, int m, Ext scale, Ext* sL, Ext* sH) {
-  const int a = m < EQ_LDS_LOG ? m : EQ_LDS_LOG, b = m - a;
-  if (threadIdx.x == 0) {
-    sL[0] = scale;
-    sH[0] = eone();
-  }
-  __syncthreads();
-  const int steps = a > b ? a : b;
-  for (int i = 0; i < steps; ++i) {
-    const uint32_t h = 1u << i;
-    for (uint32_t j = threadIdx.x; j < 2 * h; j += blockDim.x) {
-      const bool high = j >= h;
-      const uint32_t jj = high ? j - h : j;
-      if (high ? i < b : i < a) {
-        Ext* t = high ? sH : sL;
-        const Ext lo = t[jj];
-        const Ext hi = emul(lo, z[high ? a + i : i]);
-        t[jj + h] = hi;
-        t[jj] = esub(lo, hi);
+__device__ void build_eq_pair(Ext* out0, const Ext* z0, int m0, Ext scale0, Ext* out1, const Ext* z1, int m1,
+                              Ext scale1, EqPairScratch& sc) {
+  const uint32_t tid = threadIdx.x;
+  const int mu[2] = {m0, m1};
+  auto split = [](int m, int& a, int& a0, int& b0) {
+    a = m < EQ_LDS_LOG ? m : EQ_LDS_LOG;
+    a0 = a < 4 ? a : 4;
+    b0 = m - a < 4 ? m - a : 4;
+  };
+  if (tid < 128) {
+    const int u = (int)(tid >> 6), p = (int)((tid >> 4) & 3);
+    const uint32_t e = tid & 15;
+    const int m = mu[u];
+    const Ext* z = u ? z1 : z0;
+    int a, a0, b0;
+    split(m, a, a0, b0);
+    const int off = p == 0 ? 0 : p == 1 ? a0 : p == 2 ? a : a + b0;
+    const int nv = p == 0 ? a0 : p == 1 ? a - a0 : p == 2 ? b0 : m - a - b0;
+    Ext f[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      f[j] = eone();
+      if (j < nv) {
+        const Ext zj = z[off + j];
+        f[j] = ((e >> j) & 1) ? zj : esub(eone(), zj);
       }
     }
-    __syncthreads();
+    const Ext sc0 = p == 0 ? (u ? scale1 : scale0) : eone();
+    sc.part[u][p][e] = emul(emul(emul(f[0], f[1]), sc0), emul(f[2], f[3]));
   }
-  const uint32_t n = 1u << m, ma = (1u << a) - 1;
-  for (uint32_t x = threadIdx.x; x < n; x += blockDim.x) st_ext(out + x, emul(sL[x & ma], sH[x >> a]));
   __syncthreads();
+  for (uint32_t it = tid; it < 4u << EQ_LDS_LOG; it += blockDim.x) {
+    const int u = (int)(it >> (EQ_LDS_LOG + 1)), h = (int)((it >> EQ_LDS_LOG) & 1);
+    const uint32_t x = it & ((1u << EQ_LDS_LOG) - 1);
+    int a, a0, b0;
+    split(mu[u], a, a0, b0);
+    const int nb = h ? mu[u] - a : a, n_lo = h ? b0 : a0;
+    if (x >= (1u << nb)) continue;
+    const Ext v = emul(sc.part[u][2 * h][x & ((1u << n_lo) - 1)], sc.part[u][2 * h + 1][x >> n_lo]);
+    if (h) sc.hi[u][x] = v;
+    else sc.lo[u][x] = v;
+  }
+  __syncthreads();
+  const uint32_t n0 = 1u << m0, n1 = 1u << m1;
+  int a_0, a_1, t0, t1;
+  split(m0, a_0, t0, t1);
+  split(m1, a_1, t0, t1);
+  for (uint32_t it = tid; it < n0 + n1; it += blockDim.x) {
+    const int u = it >= n0;
+    const uint32_t x = u ? it - n0 : it;
+    const int a = u ? a_1 : a_0;
+    st_ext((u ? out1 : out0) + x, emul(sc.lo[u][x & ((1u << a) - 1)], sc.hi[u][x >> a]));
+  }
 }
 
 // ---- claim evaluation on the base polynomial (SplitEq::compress_hi, eq.rs:290-398) ----
