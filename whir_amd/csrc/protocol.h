@@ -281,23 +281,34 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t*
   WaveTx tx;
   if (is_wave0()) tx_begin(tx, t, log, js);
 
-  // (1) partial evals: pe[l][a] = sum_b comp[b*2^l + a] * eq(svo[l..f))[b]
-  for (uint32_t it = tid; it < n_claims * pe_per; it += SERIAL_THREADS) {
-    const uint32_t c = it / pe_per, o = it % pe_per;
-    uint32_t l = 1;
-    while ((2u << l) - 2 <= o) ++l;
-    const uint32_t a = o - ((1u << l) - 2);
-    const ClaimSlot& sl = slots[c];
-    Ext s = ezero();
-    for (uint32_t b = 0; b < (1u << (f - l)); ++b) {
+  // (1) partial evals: pe[l][a] = sum_b comp[b*2^l + a] * eq(svo[l..f))[b].
+  // One product per thread, item (claim c, level l, a, b) at lane offset
+  // (l - 1) W + a + b 2^l of claim c's f W items; the sum over b is a butterfly
+  // over the lanes 2^l, ..., W/2 apart inside the W-aligned group of (c, l).
+  const uint32_t n_items = n_claims * f * W;
+  for (uint32_t base = 0; base < n_items; base += SERIAL_THREADS) {
+    const uint32_t it = base + tid;
+    const bool valid = it < n_items;
+    const uint32_t c = valid ? it / (f * W) : 0u, rem = it % (f * W);
+    const uint32_t l = rem / W + 1, inner = rem % W;
+    const uint32_t a = inner & ((1u << l) - 1), b = inner >> l;
+    Ext v = ezero();
+    if (valid) {
+      const ClaimSlot& sl = slots[c];
       Ext e = eone();
-      for (uint32_t j = 0; j < f - l; ++j) {
+      for (uint32_t j = 0; j + l < f; ++j) {
         const Ext z = sl.point[l + j];
         e = emul(e, ((b >> j) & 1) ? z : esub(eone(), z));
       }
-      s = eadd(s, emul(sl.compressed[(b << l) + a], e));
+      v = emul(sl.compressed[(b << l) + a], e);
     }
-    pe_buf[(uint64_t)c * PE_PER + o] = s;
+    for (uint32_t m = 1; m < W; m <<= 1) {
+      Ext o;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) o.c[q] = (uint32_t)__shfl_xor((int)v.c[q], (int)m);
+      if (m >= (1u << l)) v = eadd(v, o);
+    }
+    if (valid && b == 0) pe_buf[(uint64_t)c * PE_PER + (1u << l) - 2 + a] = v;
   }
   for (uint32_t c = tid; c <= n_claims; c += SERIAL_THREADS) s_pref[c] = eq1_pref[c];
   for (uint32_t c = tid; c < n_claims; c += SERIAL_THREADS) {
@@ -360,17 +371,25 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t*
           else term = esub(eadd(edbl(y), y), eone());
           exy = emul(exy, term);
         }
-        // <gpe[l], eq(pt)>; eq(pt)[a] = prod_j (a_j ? pt_j : 1 - pt_j), small integers
+        // <gpe[l], eq(pt)>; eq(pt)[a] = prod_j (a_j ? pt_j : 1 - pt_j) is 0 unless
+        // a_j = pt_j wherever pt_j < 2; a position with pt_j = 2 gives 2 (a_j = 1)
+        // or -1 (a_j = 0), so the terms are +-2^e gp[a] over subsets of those
         const Ext* gp = gpe_buf + (uint64_t)g * PE_PER + (1u << l) - 2;
-        Ext dot = ezero();
-        for (uint32_t a = 0; a < (1u << l); ++a) {
-          int32_t coef = 1;
-          for (uint32_t j = 0; j < l; ++j) coef *= ((a >> j) & 1) ? (int32_t)pt[j] : 1 - (int32_t)pt[j];
-          if (coef == 0) continue;
-          const uint32_t mag = from_canonical((uint32_t)(coef < 0 ? -coef : coef));
-          const Ext term = emul_base(gp[a], mag);
-          dot = coef < 0 ? esub(dot, term) : eadd(dot, term);
+        uint32_t fixed = 0, freem = 0;
+        for (uint32_t j = 0; j < l; ++j) {
+          if (pt[j] == 1) fixed |= 1u << j;
+          else if (pt[j] == 2) freem |= 1u << j;
         }
+        const uint32_t n_free = (uint32_t)__builtin_popcount(freem);
+        Ext dot = ezero();
+        uint32_t sub = 0;
+        do {
+          Ext term = gp[fixed | sub];
+          const uint32_t ones = (uint32_t)__builtin_popcount(sub);
+          for (uint32_t e = 0; e < ones; ++e) term = edbl(term);
+          dot = ((n_free - ones) & 1) ? esub(dot, term) : eadd(dot, term);
+          sub = (sub - freem) & freem;
+        } while (sub != 0);
         total = eadd(total, emul(exy, dot));
       }
     }
@@ -440,7 +459,10 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t*
   // (still unscaled) table, then every first claim scales its own in place.
   // Four loads in flight per thread; claim offsets come from LDS.
   const uint32_t total = s_pref[n_claims];
-  for (int phase = 0; phase < 2; ++phase) {
+  int copies = 0;
+  for (uint32_t c = tid; c < n_claims; c += SERIAL_THREADS) copies |= s_e1[c] != s_le[c];
+  copies = __syncthreads_or(copies);
+  for (int phase = copies ? 0 : 1; phase < 2; ++phase) {
     for (uint32_t it0 = tid; it0 < total; it0 += 4 * SERIAL_THREADS) {
       Ext v[4];
       uint32_t dst[4], cl[4];
@@ -573,14 +595,30 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_begin_kernel(DevTx* t, ui
     *a.count_out = count;
     js->stir_count = count;
   }
-  const uint64_t n0 = 1ull << a.m0, n1 = 1ull << a.m1;
-  for (uint64_t it = tid; it < (uint64_t)a.n_eq * n1; it += SERIAL_THREADS) {
-    const uint64_t c = it / n1, x = it % n1;
-    if (c == 0) continue;
-    Ext* p = a.eq_pool + c * a.eq_stride + n0 + x;
-    st_ext(p, emul(ld_ext(p), s_apow[c]));
+  // query scales: thread t takes q = t + 256 i as alpha^(n_eq + t) alpha^(256 i)
+  Ext sq = epow(s_alpha, (uint64_t)a.n_eq + tid), step = s_alpha;
+  for (int i = 0; i < 8; ++i) step = esqr(step);  // alpha^256 (SERIAL_THREADS)
+  for (uint32_t q = tid; q < count; q += SERIAL_THREADS) {
+    a.qa.scale[q] = sq;
+    sq = emul(sq, step);
   }
-  for (uint32_t q = tid; q < count; q += SERIAL_THREADS) a.qa.scale[q] = epow(s_alpha, (uint64_t)a.n_eq + q);
+  // eq1 of OOD claim c >= 1 times alpha^c, four loads in flight
+  const uint64_t n0 = 1ull << a.m0, n1 = 1ull << a.m1;
+  const uint64_t n_sc = a.n_eq > 1 ? (uint64_t)(a.n_eq - 1) * n1 : 0;
+  for (uint64_t it0 = tid; it0 < n_sc; it0 += 4 * SERIAL_THREADS) {
+    Ext v[4];
+    Ext* p[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint64_t it = it0 + (uint64_t)u * SERIAL_THREADS;
+      const uint64_t c = 1 + it / n1, x = it % n1;
+      p[u] = it < n_sc ? a.eq_pool + c * a.eq_stride + n0 + x : nullptr;
+      if (p[u]) v[u] = ld_ext(p[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (p[u]) st_ext(p[u], emul(v[u], s_apow[1 + (it0 + (uint64_t)u * SERIAL_THREADS) / n1]));
+  }
 }
 
 // Word w of query `index`'s opening: the row (row_words words), then the
