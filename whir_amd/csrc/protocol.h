@@ -779,35 +779,4 @@ __global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(
   }
 }
 
-// transcript.write_many(final poly) (whir.rs:317).
-__global__ void __launch_bounds__(64) write_final_kernel(DevTx* t, uint32_t* log, JobScalars* js, const Ext* poly,
-                                                         uint32_t n) {
-  WaveTx tx;
-  tx_begin(tx, t, log, js);
-  // 16 extension elements per wave load (one word per lane), then read back
-  // lane by lane, so the duplex chain does not wait on memory
-  const uint32_t* words = reinterpret_cast<const uint32_t*>(poly);
-  const uint32_t lane = threadIdx.x & 63;
-  for (uint32_t i0 = 0; i0 < n; i0 += 16) {
-    const uint32_t v = i0 * 4 + lane < n * 4 ? words[i0 * 4 + lane] : 0u;
-    const uint32_t m = n - i0 < 16 ? n - i0 : 16u;
-    for (uint32_t j = 0; j < m; ++j) {
-      Ext e;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) e.c[c] = (uint32_t)__builtin_amdgcn_readlane((int)v, (int)(4 * j + c));
-      tx.write_e(e);
-    }
-  }
-  tx_end(tx, t, js);
-}
-
-// The test's checkpoint draw (test.rs:233) after the last openings.
-__global__ void __launch_bounds__(64) checkpoint_kernel(DevTx* t, uint32_t* log, JobScalars* js) {
-  WaveTx tx;
-  tx_begin(tx, t, log, js);
-  const uint32_t v = tx.sample();
-  tx_end(tx, t, js);
-  if (threadIdx.x == 0) js->checkpoint = v;
-}
-
 }  // namespace kb
