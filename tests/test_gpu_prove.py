@@ -22,6 +22,8 @@ SMALL = [
     (16, 4, 1, 1, 1, 123, 1),
     (18, 4, 1, 3, 1, 123, 1),   # the reference's CPU-runnable configuration
     (17, 5, 1, 3, 1, 123, 1),   # folding 5 (whir_bench shape, smaller k)
+    (18, 3, 1, 3, 1, 123, 2),   # a later round of 2^12 coefficients (above the one-workgroup rounds)
+    (17, 3, 2, 2, 1, 123, 1),   # a later round of 2^11 coefficients, rate 1/4
 ]
 
 
