@@ -700,6 +700,7 @@ __global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(
     const uint32_t* tw_hi) {
   __shared__ Ext sh[SERIAL_THREADS];
   __shared__ Ext s_r;
+  __shared__ uint32_t s_index[MAX_STIR];
   const uint32_t tid = threadIdx.x;
   WaveTx tx;
   if (is_wave0()) tx_begin(tx, t, log, js);
@@ -752,6 +753,7 @@ __global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(
   const Ext r = s_r;
   for (uint32_t q = tid; q < count; q += SERIAL_THREADS) {
     const uint32_t index = qa.index[q];
+    s_index[q] = index;
     const uint32_t ws = qa.ws[q];
     // s *= 1 + r (w - 1)
     qa.scale[q] = emul(qa.scale[q], eadd(eone(), emul(r, esub(efrom(ws), eone()))));
@@ -759,22 +761,40 @@ __global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(
     uint32_t ts = sqr(qa.ts[q]);
     if (rho + 2 <= k_data && ((index >> (k_data - rho - 2)) & 1)) ts = neg(ts);
     qa.ts[q] = ts;
-    // G_{rho+1}[tt] = fold(G[tt], G[tt + h], omega_{2^(k_data-rho)}^{-i}, r), i = b + tt*M
-    if (rho + 1 < d) {
-      const uint32_t h = 1u << (d - rho - 1);
-      const uint32_t M = 1u << (k_data - d);
-      const uint32_t b = index & (M - 1);
-      const uint32_t lg = k_data - rho;
-      for (uint32_t tt = 0; tt < h; ++tt) {
-        const uint32_t i = b + tt * M;
+  }
+  if (rho + 1 >= d) return;
+  __syncthreads();
+  // G_{rho+1}[tt] = fold(G[tt], G[tt + h], omega_{2^(k_data-rho)}^{-i}, r), i = b + tt*M,
+  // one (query, tt) item per thread, four items' loads in flight
+  const uint32_t h = 1u << (d - rho - 1);
+  const uint32_t M = 1u << (k_data - d);
+  const uint32_t lg = k_data - rho;
+  const uint32_t lh = d - rho - 1;  // log2 h
+  const uint32_t n_items = count << lh;
+  for (uint32_t it0 = tid; it0 < n_items; it0 += 4 * SERIAL_THREADS) {
+    Ext g_lo[4], g_hi[4];
+    uint32_t tw[4], gi[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t it = it0 + (uint32_t)u * SERIAL_THREADS;
+      gi[u] = 0xFFFFFFFFu;
+      if (it < n_items) {
+        const uint32_t q = it >> lh, tt = it & (h - 1);
+        const uint32_t i = (s_index[q] & (M - 1)) + tt * M;
         const uint32_t e = ((1u << lg) - i) & ((1u << lg) - 1);
         const uint32_t e24 = e << (24 - lg);
-        const uint32_t tw = mul(tw_lo[e24 & 4095u], tw_hi[e24 >> 12]);
-        const Ext g_lo = qa.G[q * GS + tt], g_hi = qa.G[q * GS + tt + h];
-        const Ext even = emul_base(eadd(g_lo, g_hi), HALF);
-        const Ext odd = emul_base(emul_base(esub(g_lo, g_hi), HALF), tw);
-        qa.G[q * GS + tt] = eadd(even, emul(r, esub(odd, even)));
+        gi[u] = q * GS + tt;
+        tw[u] = mul(tw_lo[e24 & 4095u], tw_hi[e24 >> 12]);
+        g_lo[u] = qa.G[gi[u]];
+        g_hi[u] = qa.G[gi[u] + h];
       }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (gi[u] == 0xFFFFFFFFu) continue;
+      const Ext even = emul_base(eadd(g_lo[u], g_hi[u]), HALF);
+      const Ext odd = emul_base(emul_base(esub(g_lo[u], g_hi[u]), HALF), tw[u]);
+      qa.G[gi[u]] = eadd(even, emul(r, esub(odd, even)));
     }
   }
 }
