@@ -297,26 +297,31 @@ __global__ void __launch_bounds__(SC_THREADS) expr_build_kernel(const uint32_t* 
 // ---- claim combination (combine_claims, combine.rs:37-64,178-203,407-468) ----
 // w[x] += sum_c eq0_c[x_lo] * eq1s_c[x_hi]  +  sum_j powR[j][x_hi] * powL[j][x_lo]
 // eq1s_c carries alpha^c; powL[j][e] = var_j^e (e < 2^m0), powR[j][e] = scale_j * var_j^(e*2^m0).
-// COMBINE_SPLIT adjacent lanes share an output and split the terms; the pow
-// terms are Montgomery products taken two per REDC (2 p^2 < p 2^32) and summed
-// unreduced in 64 bits, so a term costs ~5 VALU instructions instead of an
-// extension-by-base multiply plus modular adds.
-constexpr int COMBINE_SPLIT = 4;
-
+// A workgroup takes 256 / S consecutive outputs; its S wavefront groups split
+// the terms (term j to group j mod S) and meet in LDS. Each wavefront covers 64
+// consecutive outputs with one term per step, so for m0 >= 6 (UNI) the term's
+// eq1 / powR entry is the same for the whole wavefront and is read as a scalar
+// load; only the eq0 / powL row is a (coalesced) vector load. The pow terms are
+// Montgomery products taken two per REDC (2 p^2 < p 2^32) and summed unreduced
+// in 64 bits, so a term costs ~5 VALU instructions per coefficient.
+template <int S, bool UNI>
 __global__ void __launch_bounds__(SC_THREADS) combine_kernel(Ext* __restrict__ w, uint64_t n, uint32_t m0,
                                                          const Ext* __restrict__ eq_pool, uint32_t n_eq,
                                                          uint64_t eq_stride, const uint32_t* __restrict__ powL,
                                                          const Ext* __restrict__ powR, const uint32_t* n_pow_ptr,
                                                          uint32_t m1) {
-  constexpr uint32_t S = COMBINE_SPLIT;
-  const uint64_t gid = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
-  const uint64_t x = gid / S;
-  const uint32_t sub = (uint32_t)(gid % S);
-  const bool valid = x < n;  // every lane takes part in the reduction below
+  constexpr uint32_t XB = SC_THREADS / S;  // outputs per workgroup, a multiple of 64
+  __shared__ Ext s_part[S > 1 ? SC_THREADS : 1];
+  const uint32_t sub = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / XB));
+  const uint32_t xl = threadIdx.x % XB;
+  const uint64_t x = (uint64_t)blockIdx.x * XB + xl;
+  const bool valid = x < n;  // every lane takes part in the LDS reduction below
   const uint64_t xx = valid ? x : 0;
   const uint32_t n_pow = n_pow_ptr ? *n_pow_ptr : 0u;
   const uint64_t n0 = 1ull << m0, n1 = 1ull << m1;
-  const uint64_t lo = xx & (n0 - 1), hi = xx >> m0;
+  const uint64_t lo = xx & (n0 - 1);
+  uint64_t hi = xx >> m0;
+  if constexpr (UNI) hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)hi);
   uint64_t a[4] = {0, 0, 0, 0};
   uint32_t j = sub;
 #pragma unroll 2
@@ -339,12 +344,40 @@ __global__ void __launch_bounds__(SC_THREADS) combine_kernel(Ext* __restrict__ w
     const Ext* base = eq_pool + c * eq_stride;
     part = eadd(part, emul(ld_ext(base + lo), ld_ext(base + n0 + hi)));
   }
+  if constexpr (S > 1) {
+    s_part[threadIdx.x] = part;
+    __syncthreads();
+    if (sub != 0) return;
 #pragma unroll
-  for (uint32_t m = 1; m < S; m <<= 1) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) part.c[c] = add(part.c[c], (uint32_t)__shfl_xor((int)part.c[c], (int)m));
+    for (int k = 1; k < S; ++k) part = eadd(part, s_part[xl + k * XB]);
   }
-  if (valid && sub == 0) st_ext(w + x, eadd(ld_ext(w + x), part));
+  if (valid) st_ext(w + x, eadd(ld_ext(w + x), part));
+}
+
+// Host: launch combine_kernel for 2^(m0 + m1) = n outputs; S = 4 when there
+// are pow terms (n_pow_ptr set) or more than four eq terms, one lane per output
+// otherwise.
+inline void launch_combine(hipStream_t s, Ext* w, uint64_t n, uint32_t m0, const Ext* eq_pool, uint32_t n_eq,
+                           uint64_t eq_stride, const uint32_t* powL, const Ext* powR, const uint32_t* n_pow_ptr,
+                           uint32_t m1) {
+  const bool uni = m0 >= 6;
+  const unsigned S = n_pow_ptr || n_eq > 4 ? 4 : 1;
+  const unsigned grid = (unsigned)((n + SC_THREADS / S - 1) / (SC_THREADS / S));
+  if (S == 4) {
+    if (uni)
+      hipLaunchKernelGGL((combine_kernel<4, true>), dim3(grid), dim3(SC_THREADS), 0, s, w, n, m0, eq_pool, n_eq,
+                         eq_stride, powL, powR, n_pow_ptr, m1);
+    else
+      hipLaunchKernelGGL((combine_kernel<4, false>), dim3(grid), dim3(SC_THREADS), 0, s, w, n, m0, eq_pool, n_eq,
+                         eq_stride, powL, powR, n_pow_ptr, m1);
+  } else {
+    if (uni)
+      hipLaunchKernelGGL((combine_kernel<1, true>), dim3(grid), dim3(SC_THREADS), 0, s, w, n, m0, eq_pool, n_eq,
+                         eq_stride, powL, powR, n_pow_ptr, m1);
+    else
+      hipLaunchKernelGGL((combine_kernel<1, false>), dim3(grid), dim3(SC_THREADS), 0, s, w, n, m0, eq_pool, n_eq,
+                         eq_stride, powL, powR, n_pow_ptr, m1);
+  }
 }
 
 // powL[j][e] = var_j^e for e < 2^m0; powR[j][e] = scale_j * var_j^(e * 2^m0) for e < 2^m1.

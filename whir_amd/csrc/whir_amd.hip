@@ -927,8 +927,8 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
                          P->stir_idx, P->stir_count, P->cw[(r - 1) & 1], 4 * W, P->tree[(r - 1) & 1],
                          (uint32_t)rp.kfd, 2, (uint32_t)f, P->qa, om0, om1, P->powL, P->powR);
     }
-    hipLaunchKernelGGL(combine_kernel, dim3(blocks_for(n * COMBINE_SPLIT)), dim3(SC_THREADS), 0, s, P->ew[cur], n, om0, P->ood_pool,
-                       (uint32_t)rp.n_ood, P->ood_stride, P->powL, P->powR, r > 0 ? P->stir_count : nullptr, om1);
+    launch_combine(s, P->ew[cur], n, om0, P->ood_pool, (uint32_t)rp.n_ood, P->ood_stride, P->powL, P->powR,
+                   r > 0 ? P->stir_count : nullptr, om1);
     for (int t = 0; t < f; ++t) {
       unsigned g;
       if (t == 0) {
@@ -956,8 +956,7 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
       const uint32_t pm0 = mm / 2, pm1 = mm - mm / 2;
       hipLaunchKernelGGL(pow_tables_kernel, dim3(blocks_for((uint64_t)rp.n_stir * ((1ull << pm0) + (1ull << pm1)))),
                          dim3(SC_THREADS), 0, s, P->qa.ws, P->qa.scale, P->stir_count, pm0, pm1, P->powL, P->powR);
-      hipLaunchKernelGGL(combine_kernel, dim3(blocks_for(n * COMBINE_SPLIT)), dim3(SC_THREADS), 0, s, P->ew[cur], n, pm0,
-                         (const Ext*)nullptr, 0u, (uint64_t)0, P->powL, P->powR, P->stir_count, pm1);
+      launch_combine(s, P->ew[cur], n, pm0, nullptr, 0u, 0, P->powL, P->powR, P->stir_count, pm1);
     }
   }
   // ---- final polynomial, queries and the test's checkpoint (whir.rs:314-355, test.rs:233) ----
