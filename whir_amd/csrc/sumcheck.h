@@ -276,8 +276,15 @@ __global__ void __launch_bounds__(SC_THREADS) expr_build_kernel(const uint32_t* 
   st_ext(epoly + j, acc);
   const uint64_t lo = j & ((1ull << m0) - 1), hi = j >> m0;
   Ext w = ezero();
-  for (uint32_t c = full0; c < full0 + n_full; ++c)
-    w = eadd(w, emul(ld_ext(pool + desc[c].eq0_off + lo), ld_ext(pool + desc[c].eq1_off + hi)));
+  if (m0 >= 6) {
+    // a wavefront's 64 consecutive j share hi: eq1 entries as scalar loads
+    const uint64_t hu = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)hi);
+    for (uint32_t c = full0; c < full0 + n_full; ++c)
+      w = eadd(w, emul(ld_ext(pool + desc[c].eq0_off + lo), ld_ext(pool + desc[c].eq1_off + hu)));
+  } else {
+    for (uint32_t c = full0; c < full0 + n_full; ++c)
+      w = eadd(w, emul(ld_ext(pool + desc[c].eq0_off + lo), ld_ext(pool + desc[c].eq1_off + hi)));
+  }
   if (sb.n_class) {
     uint32_t cl = 0;
     while (cl < sb.n_class && j >= sb.start[cl + 1]) ++cl;
