@@ -71,7 +71,7 @@ def test_sumcheck_coeffs_and_fold(gpu_ctx, oracle, k):
     assert np.array_equal(whir_amd.api.fix_lo_var(f, r, ctx=gpu_ctx), oracle.fix_lo_var(f, r))
 
 
-@pytest.mark.parametrize("k,n", [(1, 1), (6, 3), (11, 2), (16, 5)])
+@pytest.mark.parametrize("k,n", [(1, 1), (6, 3), (11, 2), (16, 5), (13, 9), (12, 1)])
 def test_combine_eqs(gpu_ctx, oracle, k, n):
     import whir_amd
 

@@ -149,12 +149,11 @@ __global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_pair_kernel(const
 constexpr int COOP_THREADS = 256;
 constexpr uint32_t COOP_MAX_NODES = 32;  // n0 / G
 
-__global__ void __launch_bounds__(COOP_THREADS) merkle_coop_subtrees_kernel(uint32_t* layer, uint32_t n0, uint32_t G) {
-  __shared__ uint32_t s_node[COOP_MAX_NODES * 8];
-  const uint32_t w = blockIdx.x;
+// Workgroup w's part (COOP_THREADS threads); on return s_node[0..8) holds its
+// node of the G-node layer and every thread has passed the last barrier.
+__device__ __forceinline__ void coop_subtrees_body(uint32_t* layer, uint32_t n0, uint32_t G, uint32_t w,
+                                                   const CoopP16& K, uint32_t* s_node) {
   const uint32_t slot = threadIdx.x >> 4, li = threadIdx.x & 15;
-  CoopP16 K;
-  K.load();
   uint32_t kk = n0 / G;  // nodes of this residue class in the current layer
   for (uint32_t e = threadIdx.x; e < kk * 8; e += COOP_THREADS)
     s_node[e] = layer[(uint64_t)(w + (e >> 3) * G) * 8 + (e & 7)];
@@ -178,5 +177,12 @@ __global__ void __launch_bounds__(COOP_THREADS) merkle_coop_subtrees_kernel(uint
     next += (uint64_t)n * 8;
     kk = h;
   }
+}
+
+__global__ void __launch_bounds__(COOP_THREADS) merkle_coop_subtrees_kernel(uint32_t* layer, uint32_t n0, uint32_t G) {
+  __shared__ uint32_t s_node[COOP_MAX_NODES * 8];
+  CoopP16 K;
+  K.load();
+  coop_subtrees_body(layer, n0, G, blockIdx.x, K, s_node);
 }
 }  // namespace kb

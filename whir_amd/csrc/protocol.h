@@ -9,6 +9,7 @@
 #include "field.h"
 #include "tx.h"
 #include "sumcheck.h"
+#include "merkle.h"
 
 namespace kb {
 
@@ -123,24 +124,27 @@ __device__ __forceinline__ void write_root_body(WaveTx& tx, const uint32_t* root
   for (int j = 0; j < 8; ++j) tx.observe((uint32_t)__shfl((int)v, j, 8));
 }
 
-// transcript.write(root) alone (a commitment with no claim drawn after it).
-__global__ void __launch_bounds__(64) write_root_kernel(DevTx* t, uint32_t* log, JobScalars* js, const uint32_t* root) {
-  WaveTx tx;
-  tx_begin(tx, t, log, js);
-  write_root_body(tx, root);
-  tx_end(tx, t, js);
-}
-
-// [transcript.write(root) if root != nullptr;] begin a claim.
-__global__ void __launch_bounds__(SERIAL_THREADS) claim_begin_kernel(DevTx* t, uint32_t* log, JobScalars* js,
-                                                                     const uint32_t* root, ClaimBegin cb) {
+// The last cooperative launch of a commitment's tree (the top n0 <= 32 nodes
+// down to the root, merkle_coop_subtrees_kernel with G = 1), then its
+// transcript step in the same launch: transcript.write(root) and, when cb.slot
+// is set, the first claim's draw (claim_begin_body).
+static_assert(COOP_THREADS == SERIAL_THREADS, "one workgroup shape for the tree top and the claim");
+__global__ void __launch_bounds__(SERIAL_THREADS) tree_top_claim_kernel(uint32_t* layer, uint32_t n0, DevTx* t,
+                                                                        uint32_t* log, JobScalars* js,
+                                                                        ClaimBegin cb) {
+  __shared__ uint32_t s_node[COOP_MAX_NODES * 8];
   __shared__ ClaimScratch sc;
+  {
+    CoopP16 K;
+    K.load();
+    coop_subtrees_body(layer, n0, 1, 0, K, s_node);
+  }
   WaveTx tx;
   if (is_wave0()) {
     tx_begin(tx, t, log, js);
-    if (root) write_root_body(tx, root);
+    write_root_body(tx, s_node);
   }
-  claim_begin_body(tx, cb, sc);
+  if (cb.slot) claim_begin_body(tx, cb, sc);
   if (is_wave0()) tx_end(tx, t, js);
 }
 

@@ -305,8 +305,11 @@ static int leaves_enqueue(const uint32_t* rows, uint64_t n_rows, uint32_t row_wo
 // MERKLE_COOP_FROM nodes down, cooperative subtrees (latency-bound part).
 constexpr uint64_t MERKLE_COOP_FROM = 1ull << 15;
 
+// With `top` set, the last cooperative launch (the top <= COOP_MAX_NODES nodes
+// down to the root) is left to the caller: *top / *top_n = that layer and its
+// node count (tree_top_claim_kernel runs it with the transcript step).
 static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, uint32_t* layers, hipStream_t s,
-                          hipEvent_t* leaf_ev = nullptr) {
+                          hipEvent_t* leaf_ev = nullptr, uint32_t** top = nullptr, uint32_t* top_n = nullptr) {
   const uint64_t H = 1ull << log_h;
   if (leaf_ev) HIP_TRY(hipEventRecord(leaf_ev[0], s));
   int rc = leaves_enqueue(rows, H, row_words, layers, s);
@@ -330,11 +333,16 @@ static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, u
   while (n > 1) {
     const uint64_t kk = std::min<uint64_t>(n, COOP_MAX_NODES);
     const uint32_t G = (uint32_t)(n / kk);
+    if (G == 1 && top) break;
     hipLaunchKernelGGL(merkle_coop_subtrees_kernel, dim3(G), dim3(COOP_THREADS), 0, s, cur, (uint32_t)n, G);
     while (n > G) {  // advance to the G-node layer
       cur += n * 8;
       n /= 2;
     }
+  }
+  if (top) {
+    *top = cur;
+    *top_n = (uint32_t)n;
   }
   return WHIR_OK;
 }
@@ -801,11 +809,12 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
   if (P->timers) HIP_TRY(hipEventRecord(P->ev[WHIR_TIMER_DFT_BASE][0], s));
   if ((rc = ntt_enqueue(c, poly, 1ull << (k - f), log_h0, f, P->cw0, P->ntt_tmp, s))) return rc;
   if (P->timers) HIP_TRY(hipEventRecord(P->ev[WHIR_TIMER_DFT_BASE][1], s));
-  if ((rc = merkle_enqueue(P->cw0, log_h0, W, P->tree0, s, P->timers ? P->ev[WHIR_TIMER_LEAVES_BASE] : nullptr)))
+  uint32_t* top = nullptr;
+  uint32_t top_n = 0;
+  if ((rc = merkle_enqueue(P->cw0, log_h0, W, P->tree0, s, P->timers ? P->ev[WHIR_TIMER_LEAVES_BASE] : nullptr, &top,
+                           &top_n)))
     return rc;
   P->timer_pending = P->timers;
-  const uint64_t H0 = 1ull << log_h0;
-  const uint32_t* root0 = P->tree0 + (2 * H0 - 2) * 8;
 
   // ---- evaluation claims + OOD virtual claims (prover.rs:64-90, 231-277) ----
   // Claims drawn at one point form a group (one compress launch over its
@@ -829,7 +838,8 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
     }
     return cb;
   };
-  hipLaunchKernelGGL(claim_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, root0,
+  // the tree's top levels, transcript.write(root) and the first group's draw
+  hipLaunchKernelGGL(tree_top_claim_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, top, top_n, P->tx, P->log, P->js,
                      group_begin(0));
   for (int g = 0; g < n_groups; ++g) {
     const ClaimBegin cb = group_begin(g);
@@ -856,10 +866,11 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
     // commit_ext (whir.rs:38-61): 2^(kc-f) rows of 2^f extension elements
     if ((rc = ntt_enqueue(c, (const uint32_t*)P->ep[cur], 1ull << (rp.kc - f), rp.log_rows, f + 2, cw, P->ntt_tmp, s)))
       return rc;
-    if ((rc = merkle_enqueue(cw, rp.log_rows, 4 * W, tree, s))) return rc;
+    const bool small = r > 0 && small_round_fits(rp.kc, rp.n_ood, rp.n_stir);
+    if ((rc = merkle_enqueue(cw, rp.log_rows, 4 * W, tree, s, nullptr, small ? nullptr : &top, &top_n))) return rc;
     const uint64_t Hr = 1ull << rp.log_rows;
     const uint32_t* root = tree + (2 * Hr - 2) * 8;
-    if (r > 0 && small_round_fits(rp.kc, rp.n_ood, rp.n_stir)) {
+    if (small) {
       // the whole round in one workgroup (round_small.h)
       SmallRoundArgs sa{root, P->ep[cur], P->ew[cur], P->ep[cur ^ 1], P->ew[cur ^ 1], (uint32_t)rp.kc,
                         (uint32_t)rp.n_ood, (uint32_t)rp.n_stir, (uint32_t)rp.kfd, (uint32_t)f, rs_idx, 4 * W,
@@ -886,11 +897,9 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
       }
       return cb;
     };
-    if (rp.n_ood > 0)
-      hipLaunchKernelGGL(claim_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, root,
-                         ood_claim(0));
-    else
-      hipLaunchKernelGGL(write_root_kernel, dim3(1), dim3(64), 0, s, P->tx, P->log, P->js, root);
+    // the tree's top levels, transcript.write(root) and the first OOD draw
+    hipLaunchKernelGGL(tree_top_claim_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, top, top_n, P->tx, P->log, P->js,
+                       rp.n_ood > 0 ? ood_claim(0) : ClaimBegin{});
     for (int o = 0; o < rp.n_ood; ++o) {
       const ClaimBegin cb = ood_claim(o);
       hipLaunchKernelGGL(eval_ext_kernel, dim3(1u << om1), dim3(SC_THREADS), 0, s, P->ep[cur], cb.eq0, cb.eq1, om0,
