@@ -302,7 +302,14 @@ __global__ void __launch_bounds__(SERIAL_THREADS) final_phase_kernel(DevTx* t, u
     tx.log_words += count * per;
     if (tx.log_words > tx.log_cap) tx.overflow = 1;
     const uint32_t v = tx.sample();  // the test's checkpoint draw
-    if (tid == 0) js->checkpoint = v;
+    if (tid == 0) {
+      js->checkpoint = v;
+      // the proof's status after the log's capacity, so one copy brings back
+      // the proof and its length, overflow flag and checkpoint
+      log[tx.log_cap] = tx.log_words;
+      log[tx.log_cap + 1] = tx.overflow;
+      log[tx.log_cap + 2] = v;
+    }
     tx_end(tx, t, js);
   }
   __syncthreads();

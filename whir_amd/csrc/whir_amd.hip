@@ -399,6 +399,10 @@ extern "C" int whir_poseidon2_permute(whir_ctx* c, int width, uint32_t* d_states
 
 // ============================ prover ============================
 
+// Status words after a proof log's capacity (written by final_phase_kernel):
+// words used, overflow flag, checkpoint draw, spare.
+constexpr uint32_t LOG_TAIL = 4;
+
 struct whir_prover {
   whir_ctx* ctx = nullptr;
   Plan plan;
@@ -445,9 +449,7 @@ struct whir_prover {
   std::vector<uint32_t> h_group_start, h_eq1_pref, h_seg_start, h_seg_claims;
   uint32_t full0 = 0, n_full = 0;  // claims spanning the whole stacked vector
   StackBlocks sb{};
-  uint32_t* host_log = nullptr;  // pinned
-  JobScalars* host_js = nullptr;
-  DevTx* host_tx = nullptr;
+  uint32_t* host_log = nullptr;  // pinned: the log, then LOG_TAIL status words
   // optional HIP-event timers around kernel classes (whir_prover_set_timers)
   bool timers = false;
   hipEvent_t ev[WHIR_TIMER_COUNT][2] = {};
@@ -605,7 +607,7 @@ static int prover_layout(whir_prover* P, const whir_params* prm, const whir_stac
   items = {
       {(void**)&P->tx, sizeof(DevTx)},
       {(void**)&P->js, sizeof(JobScalars)},
-      {(void**)&P->log, pl.log_words * 4},
+      {(void**)&P->log, (pl.log_words + LOG_TAIL) * 4},
       {(void**)&P->input, (1ull << k) * 4},
       {(void**)&P->cw0, cw0_words * 4},
       {(void**)&P->tree0, (2 * H0 - 1) * 32},
@@ -680,9 +682,7 @@ static int prover_create(whir_ctx* c, const whir_params* prm, const whir_stack* 
     p += align_up(it.bytes);
   }
   const Plan& pl = P->plan;
-  if (hipHostMalloc((void**)&P->host_log, pl.log_words * 4, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&P->host_js, sizeof(JobScalars), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&P->host_tx, sizeof(DevTx), hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void**)&P->host_log, (pl.log_words + LOG_TAIL) * 4, hipHostMallocDefault) != hipSuccess) {
     whir_prover_destroy(P);
     return WHIR_EOOM;
   }
@@ -728,8 +728,6 @@ extern "C" int whir_prover_destroy(whir_prover* P) {
   if (!P) return WHIR_OK;
   if (P->arena) hipFree(P->arena);
   if (P->host_log) hipHostFree(P->host_log);
-  if (P->host_js) hipHostFree(P->host_js);
-  if (P->host_tx) hipHostFree(P->host_tx);
   for (int i = 0; i < WHIR_TIMER_COUNT; ++i)
     for (int j = 0; j < 2; ++j)
       if (P->ev[i][j]) (void)hipEventDestroy(P->ev[i][j]);
@@ -983,9 +981,8 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
 static int enqueue_with_copies(whir_prover* P, const uint32_t* d_poly, hipStream_t s) {
   int rc = prover_enqueue(P, d_poly, s);
   if (rc) return rc;
-  HIP_TRY(hipMemcpyAsync(P->host_tx, P->tx, sizeof(DevTx), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(P->host_js, P->js, sizeof(JobScalars), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(P->host_log, P->log, P->plan.log_words * 4, hipMemcpyDeviceToHost, s));
+  // the log and the status words final_phase_kernel leaves after it
+  HIP_TRY(hipMemcpyAsync(P->host_log, P->log, (P->plan.log_words + LOG_TAIL) * 4, hipMemcpyDeviceToHost, s));
   return WHIR_OK;
 }
 
@@ -1074,11 +1071,11 @@ extern "C" int whir_prover_result(whir_prover* P, uint8_t* proof, size_t cap, si
     }
     P->timer_pending = false;
   }
-  const DevTx& t = *P->host_tx;
-  if (t.overflow || t.log_words > t.log_cap) return WHIR_ETRANSCRIPT;
-  const size_t bytes = (size_t)t.log_words * 4;
+  const uint32_t* tail = P->host_log + P->plan.log_words;  // log_words, overflow, checkpoint
+  if (tail[1] || tail[0] > P->plan.log_words) return WHIR_ETRANSCRIPT;
+  const size_t bytes = (size_t)tail[0] * 4;
   if (len) *len = bytes;
-  if (checkpoint) *checkpoint = P->host_js->checkpoint;
+  if (checkpoint) *checkpoint = tail[2];
   if (root8) memcpy(root8, P->host_log, 32);  // the first transcript write is the root
   if (proof) {
     if (bytes > cap) return WHIR_ETRANSCRIPT;
