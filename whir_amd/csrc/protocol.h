@@ -69,19 +69,6 @@ __device__ __forceinline__ void lagrange012(const Ext& r, Ext (&l)[3]) {
   l[2] = emul_base(emul(r, esub(r, one)), HALF);
 }
 
-__global__ void tx_init_kernel(DevTx* t, JobScalars* js, uint32_t cap) {
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < 16; ++i) t->state[i] = 0;
-    for (int i = 0; i < 8; ++i) t->in_buf[i] = t->out_buf[i] = 0;
-    t->in_len = t->out_len = 0;
-    t->log_words = 0;
-    t->log_cap = cap;
-    t->overflow = 0;
-    js->pending_words = 0;
-    js->stir_count = 0;
-  }
-}
-
 // Where the next claim's draw goes (a claim kernel can finish one claim and
 // begin the next in the same launch).
 struct ClaimBegin {
@@ -127,11 +114,12 @@ __device__ __forceinline__ void write_root_body(WaveTx& tx, const uint32_t* root
 // The last cooperative launch of a commitment's tree (the top n0 <= 32 nodes
 // down to the root, merkle_coop_subtrees_kernel with G = 1), then its
 // transcript step in the same launch: transcript.write(root) and, when cb.slot
-// is set, the first claim's draw (claim_begin_body).
+// is set, the first claim's draw (claim_begin_body). fresh_cap != 0 starts a
+// new transcript with a log of that many words (the proof's first commitment).
 static_assert(COOP_THREADS == SERIAL_THREADS, "one workgroup shape for the tree top and the claim");
 __global__ void __launch_bounds__(SERIAL_THREADS) tree_top_claim_kernel(uint32_t* layer, uint32_t n0, DevTx* t,
                                                                         uint32_t* log, JobScalars* js,
-                                                                        ClaimBegin cb) {
+                                                                        ClaimBegin cb, uint32_t fresh_cap) {
   __shared__ uint32_t s_node[COOP_MAX_NODES * 8];
   __shared__ ClaimScratch sc;
   {
@@ -141,7 +129,8 @@ __global__ void __launch_bounds__(SERIAL_THREADS) tree_top_claim_kernel(uint32_t
   }
   WaveTx tx;
   if (is_wave0()) {
-    tx_begin(tx, t, log, js);
+    if (fresh_cap) tx.fresh(fresh_cap, log);
+    else tx_begin(tx, t, log, js);
     write_root_body(tx, s_node);
   }
   if (cb.slot) claim_begin_body(tx, cb, sc);

@@ -139,6 +139,18 @@ struct WaveTx {
   uint32_t* log;
   CoopP16 k;
 
+  // A fresh transcript: zero sponge state, empty buffers, an empty log of
+  // `cap` words.
+  __device__ __forceinline__ void fresh(uint32_t cap, uint32_t* log_ptr) {
+    s = 0u;
+    pin = 0u;
+    in_len = out_len = 0;
+    log_words = 0;
+    log_cap = cap;
+    overflow = 0;
+    log = log_ptr;
+    k.load();
+  }
   __device__ __forceinline__ void load(const DevTx* t, uint32_t* log_ptr) {
     const int lane = threadIdx.x & 63;
     s = t->state[lane & 15];

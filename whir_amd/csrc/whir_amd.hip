@@ -799,8 +799,7 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
   const Plan& pl = P->plan;
   const int k = pl.k, f = pl.f;
   const uint32_t W = 1u << f;
-  int rc;
-  hipLaunchKernelGGL(tx_init_kernel, dim3(1), dim3(64), 0, s, P->tx, P->js, (uint32_t)pl.log_words);
+  int rc;  // the transcript starts fresh in the base commitment's tree_top_claim_kernel
 
   // ---- commit: RS encode + Merkle (whir.rs:19-36) ----
   const int log_h0 = k + pl.rate - f;
@@ -838,7 +837,7 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
   };
   // the tree's top levels, transcript.write(root) and the first group's draw
   hipLaunchKernelGGL(tree_top_claim_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, top, top_n, P->tx, P->log, P->js,
-                     group_begin(0));
+                     group_begin(0), (uint32_t)pl.log_words);
   for (int g = 0; g < n_groups; ++g) {
     const ClaimBegin cb = group_begin(g);
     const whir_prover::Group& G = P->groups[g];
@@ -897,7 +896,7 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
     };
     // the tree's top levels, transcript.write(root) and the first OOD draw
     hipLaunchKernelGGL(tree_top_claim_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, top, top_n, P->tx, P->log, P->js,
-                       rp.n_ood > 0 ? ood_claim(0) : ClaimBegin{});
+                       rp.n_ood > 0 ? ood_claim(0) : ClaimBegin{}, 0u);
     for (int o = 0; o < rp.n_ood; ++o) {
       const ClaimBegin cb = ood_claim(o);
       hipLaunchKernelGGL(eval_ext_kernel, dim3(1u << om1), dim3(SC_THREADS), 0, s, P->ep[cur], cb.eq0, cb.eq1, om0,
