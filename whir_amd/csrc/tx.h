@@ -28,9 +28,13 @@ struct DevTx {
 // Cross-lane moves inside each 16-lane row via DPP (VALU latency, no LDS):
 //   quad_perm(p0,p1,p2,p3): lane i of each quad reads lane p_i of that quad;
 //   row_ror:n             : lane i of each 16-lane row reads lane (i - n) mod 16.
+// Every lane's source is a valid lane under these controls, so bound_ctrl (0
+// for an invalid source) changes no result; it lets the compiler drop the
+// "old" operand, which would otherwise be materialised before each move
+// (cooperative P16: 971 -> 904 VALU instructions, 1.72 -> 1.56 us).
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, true);
 }
 constexpr int DPP_QUAD_NEXT = 1 | (2 << 2) | (3 << 4) | (0 << 6);  // lane a <- a+1 (mod 4)
 constexpr int DPP_QUAD_XOR1 = 1 | (0 << 2) | (3 << 4) | (2 << 6);
