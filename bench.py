@@ -293,7 +293,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--slots", type=int, default=4, help="proof-server streams (batched and serving legs)")
+    ap.add_argument("--slots", type=int, default=6,
+                    help="proof-server streams (batched and serving legs); 6 measured best on MI355X")
     ap.add_argument("--serve-jobs", type=int, default=240)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this size (for HBM counter calibration)")

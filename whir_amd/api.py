@@ -389,7 +389,7 @@ def prover_bytes_for(p):
 class Server:
     """whir_server: continuous batching of independent proofs (see include/whir_gpu.h)."""
 
-    def __init__(self, max_slots=4, hbm_budget_bytes=64 << 30, ctx=None):
+    def __init__(self, max_slots=6, hbm_budget_bytes=64 << 30, ctx=None):
         self.ctx = ctx or default_context()
         p = ctypes.c_void_p()
         check(lib().whir_server_create(self.ctx.ptr, max_slots, hbm_budget_bytes, ctypes.byref(p)),
