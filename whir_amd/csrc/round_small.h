@@ -18,11 +18,11 @@
 
 namespace kb {
 
-// Measured on MI355X (k = 24): the round of 2^8 elements takes 84 us here
-// against 133 us as a kernel sequence; at 2^12 elements one workgroup is too
-// small for the claim combination (34 us against 10 us on the grid), hence
-// the 2^10 limit.
-constexpr int SR_THREADS = 256;
+// Measured on MI355X (k = 24): the round of 2^8 elements takes 86 us here
+// (91 us with 256 threads) against 133 us as a kernel sequence; at 2^12
+// elements one workgroup is too small for the claim combination (34 us against
+// 10 us on the grid), hence the 2^10 limit.
+constexpr int SR_THREADS = 512;
 constexpr int SR_WAVES = SR_THREADS / 64;
 constexpr uint32_t SR_MAX_LOG_N = 10;
 constexpr int SR_MAX_OOD = 8;
