@@ -154,6 +154,103 @@ KB_HD uint32_t canon2(uint32_t t) {  // [0, 2p) -> [0, p)
   return t < u ? t : u;
 }
 
+// internal_lazy's sum without x[0] as two interleaved chains, sa = x1 + x3 +
+// ... + x[W-1] and sb = x2 + x4 + ..., in one asm block on the device: the
+// same v_mad_u64_u32 sequence as separate mad_k<1> calls, without the wait
+// state the compiler puts after every inline asm statement (consecutive
+// dependent v_mad_u64_u32 need none: compiled code issues them back to back).
+template <int W>
+KB_HD void sum_chains(const uint32_t (&x)[W], uint64_t& sa, uint64_t& sb) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (W == 16) {
+    asm("v_mad_u64_u32 %0, vcc, %2, 1, 0\n"
+        "v_mad_u64_u32 %1, vcc, %3, 1, 0\n"
+        "v_mad_u64_u32 %0, vcc, %4, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %5, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %6, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %7, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %8, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %9, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %10, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %11, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %12, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %13, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %14, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %15, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %16, 1, %0"
+        : "=&v"(sa), "=&v"(sb)
+        : "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(x[4]), "v"(x[5]), "v"(x[6]), "v"(x[7]), "v"(x[8]), "v"(x[9]), "v"(x[10]), "v"(x[11]), "v"(x[12]), "v"(x[13]), "v"(x[14]), "v"(x[15])
+        : "vcc");
+    return;
+  }
+  else if constexpr (W == 24) {
+    asm("v_mad_u64_u32 %0, vcc, %2, 1, 0\n"
+        "v_mad_u64_u32 %1, vcc, %3, 1, 0\n"
+        "v_mad_u64_u32 %0, vcc, %4, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %5, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %6, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %7, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %8, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %9, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %10, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %11, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %12, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %13, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %14, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %15, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %16, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %17, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %18, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %19, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %20, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %21, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %22, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %23, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %24, 1, %0"
+        : "=&v"(sa), "=&v"(sb)
+        : "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(x[4]), "v"(x[5]), "v"(x[6]), "v"(x[7]), "v"(x[8]), "v"(x[9]), "v"(x[10]), "v"(x[11]), "v"(x[12]), "v"(x[13]), "v"(x[14]), "v"(x[15]), "v"(x[16]), "v"(x[17]), "v"(x[18]), "v"(x[19]), "v"(x[20]), "v"(x[21]), "v"(x[22]), "v"(x[23])
+        : "vcc");
+    return;
+  }
+#endif
+  sa = x[1];
+  sb = x[2];
+  for (int j = 3; j + 1 < W; j += 2) {
+    sa += x[j];
+    sb += x[j + 1];
+  }
+  sa += x[W - 1];
+}
+
+// One 4-chunk of MDS-light, y_a = S + x_a + 2 x_{a+1} (S the chunk sum), as
+// one asm block of twelve v_mad_u64_u32 (the mad_k chain of mds_lazy).
+KB_HD void m4_chunk(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3, uint64_t (&y)[4]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t s;
+  asm("v_mad_u64_u32 %4, vcc, %5, 1, 0\n"
+      "v_mad_u64_u32 %4, vcc, %6, 1, %4\n"
+      "v_mad_u64_u32 %4, vcc, %7, 1, %4\n"
+      "v_mad_u64_u32 %4, vcc, %8, 1, %4\n"
+      "v_mad_u64_u32 %0, vcc, %5, 1, %4\n"
+      "v_mad_u64_u32 %1, vcc, %6, 1, %4\n"
+      "v_mad_u64_u32 %2, vcc, %7, 1, %4\n"
+      "v_mad_u64_u32 %3, vcc, %8, 1, %4\n"
+      "v_mad_u64_u32 %0, vcc, %6, 2, %0\n"
+      "v_mad_u64_u32 %1, vcc, %7, 2, %1\n"
+      "v_mad_u64_u32 %2, vcc, %8, 2, %2\n"
+      "v_mad_u64_u32 %3, vcc, %5, 2, %3"
+      : "=&v"(y[0]), "=&v"(y[1]), "=&v"(y[2]), "=&v"(y[3]), "=&v"(s)
+      : "v"(x0), "v"(x1), "v"(x2), "v"(x3)
+      : "vcc");
+#else
+  const uint64_t s = (uint64_t)x0 + x1 + x2 + x3;
+  y[0] = s + x0 + 2ull * x1;
+  y[1] = s + x1 + 2ull * x2;
+  y[2] = s + x2 + 2ull * x3;
+  y[3] = s + x3 + 2ull * x0;
+#endif
+}
+
 enum { LAZY_ADD_NONE = 0, LAZY_ADD_FIRST = 1, LAZY_ADD_ALL = 2 };
 
 // x <- REDC(MDS-light(x) + add), x_i < 2p on entry, < p + 2^7 on exit.
@@ -164,12 +261,10 @@ KB_HD void mds_lazy(uint32_t (&x)[W], const uint32_t* add) {
   uint64_t y[W];
 #pragma unroll
   for (int c = 0; c < W; c += 4) {
-    uint64_t s = mul_k<1>(x[c]);
-    s = mad_k<1>(x[c + 1], s);
-    s = mad_k<1>(x[c + 2], s);
-    s = mad_k<1>(x[c + 3], s);
+    uint64_t yc[4];
+    m4_chunk(x[c], x[c + 1], x[c + 2], x[c + 3], yc);
 #pragma unroll
-    for (int a = 0; a < 4; ++a) y[c + a] = mad_k<2>(x[c + ((a + 1) & 3)], mad_k<1>(x[c + a], s));
+    for (int a = 0; a < 4; ++a) y[c + a] = yc[a];
   }
   uint64_t sums[4];
 #pragma unroll
@@ -205,14 +300,9 @@ KB_HD void external_lazy(uint32_t (&x)[W], const uint32_t* add) {
 template <int W, bool LAST>
 KB_HD void internal_lazy(uint32_t (&x)[W], int r, const Poseidon2Lazy<W>& C) {
   x[0] = mulr(mulr(mulr(x[0], x[0]), x[0]), C.kappa[r]);
-  // two interleaved accumulation chains (the S-box of lane 0 overlaps the first)
-  uint64_t sa = mul_k<1>(x[1]), sb = mul_k<1>(x[2]);
-#pragma unroll
-  for (int j = 3; j + 1 < W; j += 2) {
-    sa = mad_k<1>(x[j], sa);
-    sb = mad_k<1>(x[j + 1], sb);
-  }
-  sa = mad_k<1>(x[W - 1], sa);
+  // two interleaved accumulation chains over x[1..W)
+  uint64_t sa, sb;
+  sum_chains<W>(x, sa, sb);
   const uint64_t sum = mad_k<1>(x[0], sa + sb);
   uint64_t t0 = add_u(mad_k<2>(2u * P - x[0], sum), LAST ? C.lin_add[5][0] : C.int_add[r + 1]);
   x[0] = LAST ? canon2(redc(t0)) : redc(t0);
