@@ -35,6 +35,50 @@ struct PairLane {
   }
 };
 
+// The two interleaved sum chains of a lane's half state, sa = x0 + x2 + ...
+// and sb = x1 + x3 + ..., as one asm block on the device (see sum_chains).
+template <int H>
+__device__ __forceinline__ void pair_sum_chains(const uint32_t (&x)[H], uint64_t& sa, uint64_t& sb) {
+  if constexpr (H == 8) {
+    asm("v_mad_u64_u32 %0, vcc, %2, 1, 0\n"
+        "v_mad_u64_u32 %1, vcc, %3, 1, 0\n"
+        "v_mad_u64_u32 %0, vcc, %4, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %5, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %6, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %7, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %8, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %9, 1, %1"
+        : "=&v"(sa), "=&v"(sb)
+        : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(x[4]), "v"(x[5]), "v"(x[6]), "v"(x[7])
+        : "vcc");
+    return;
+  }
+  else if constexpr (H == 12) {
+    asm("v_mad_u64_u32 %0, vcc, %2, 1, 0\n"
+        "v_mad_u64_u32 %1, vcc, %3, 1, 0\n"
+        "v_mad_u64_u32 %0, vcc, %4, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %5, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %6, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %7, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %8, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %9, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %10, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %11, 1, %1\n"
+        "v_mad_u64_u32 %0, vcc, %12, 1, %0\n"
+        "v_mad_u64_u32 %1, vcc, %13, 1, %1"
+        : "=&v"(sa), "=&v"(sb)
+        : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(x[4]), "v"(x[5]), "v"(x[6]), "v"(x[7]), "v"(x[8]), "v"(x[9]), "v"(x[10]), "v"(x[11])
+        : "vcc");
+    return;
+  }
+  sa = x[0];
+  sb = x[1];
+  for (int j = 2; j < H; j += 2) {
+    sa += x[j];
+    sb += x[j + 1];
+  }
+}
+
 // x <- REDC(MDS-light(x) + add) as mds_lazy, on the lane's half of the state.
 template <int W, int ADD>
 __device__ __forceinline__ void mds_pair(uint32_t (&x)[W / 2], const uint32_t* add, const PairLane<W>& L) {
@@ -42,12 +86,10 @@ __device__ __forceinline__ void mds_pair(uint32_t (&x)[W / 2], const uint32_t* a
   uint64_t y[H];
 #pragma unroll
   for (int c = 0; c < H; c += 4) {
-    uint64_t s = mul_k<1>(x[c]);
-    s = mad_k<1>(x[c + 1], s);
-    s = mad_k<1>(x[c + 2], s);
-    s = mad_k<1>(x[c + 3], s);
+    uint64_t yc[4];
+    m4_chunk(x[c], x[c + 1], x[c + 2], x[c + 3], yc);
 #pragma unroll
-    for (int a = 0; a < 4; ++a) y[c + a] = mad_k<2>(x[c + ((a + 1) & 3)], mad_k<1>(x[c + a], s));
+    for (int a = 0; a < 4; ++a) y[c + a] = yc[a];
   }
   uint64_t cs[4];
 #pragma unroll
@@ -83,12 +125,8 @@ __device__ __forceinline__ void internal_pair(uint32_t (&x)[W / 2], int r, const
   constexpr int H = W / 2;
   const uint32_t c0 = mulr(mulr(mulr(x[0], x[0]), x[0]), C.kappa[r]);
   x[0] = L.h ? x[0] : c0;
-  uint64_t sa = mul_k<1>(x[0]), sb = mul_k<1>(x[1]);
-#pragma unroll
-  for (int j = 2; j < H; j += 2) {
-    sa = mad_k<1>(x[j], sa);
-    sb = mad_k<1>(x[j + 1], sb);
-  }
+  uint64_t sa, sb;
+  pair_sum_chains<H>(x, sa, sb);
   uint64_t sum = sa + sb;
   sum += pair_swap64(sum);
   const uint32_t y0 = L.h ? x[0] : 2u * P - x[0];
