@@ -251,6 +251,49 @@ KB_HD void m4_chunk(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3, uint64_t
 #endif
 }
 
+// t[k] = x[I0 + k] * V_{I0 + k} + sum for k < K (K <= 8), the internal
+// layer's diagonal terms, as one asm block on the device (see sum_chains).
+template <int W, int I0, int K>
+KB_HD void diag_mads(const uint32_t (&x)[W], uint64_t sum, uint64_t (&t)[K]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (K == 8) {
+    asm("v_mad_u64_u32 %0, vcc, %8, %16, %24\n"
+        "v_mad_u64_u32 %1, vcc, %9, %17, %24\n"
+        "v_mad_u64_u32 %2, vcc, %10, %18, %24\n"
+        "v_mad_u64_u32 %3, vcc, %11, %19, %24\n"
+        "v_mad_u64_u32 %4, vcc, %12, %20, %24\n"
+        "v_mad_u64_u32 %5, vcc, %13, %21, %24\n"
+        "v_mad_u64_u32 %6, vcc, %14, %22, %24\n"
+        "v_mad_u64_u32 %7, vcc, %15, %23, %24"
+        : "=&v"(t[0]), "=&v"(t[1]), "=&v"(t[2]), "=&v"(t[3]), "=&v"(t[4]), "=&v"(t[5]), "=&v"(t[6]), "=&v"(t[7])
+        : "v"(x[I0]), "v"(x[I0 + 1]), "v"(x[I0 + 2]), "v"(x[I0 + 3]), "v"(x[I0 + 4]), "v"(x[I0 + 5]),
+          "v"(x[I0 + 6]), "v"(x[I0 + 7]), "s"(Poseidon2Lazy<W>::diag(I0)), "s"(Poseidon2Lazy<W>::diag(I0 + 1)),
+          "s"(Poseidon2Lazy<W>::diag(I0 + 2)), "s"(Poseidon2Lazy<W>::diag(I0 + 3)),
+          "s"(Poseidon2Lazy<W>::diag(I0 + 4)), "s"(Poseidon2Lazy<W>::diag(I0 + 5)),
+          "s"(Poseidon2Lazy<W>::diag(I0 + 6)), "s"(Poseidon2Lazy<W>::diag(I0 + 7)), "v"(sum)
+        : "vcc");
+    return;
+  } else if constexpr (K == 7) {
+    asm("v_mad_u64_u32 %0, vcc, %7, %14, %21\n"
+        "v_mad_u64_u32 %1, vcc, %8, %15, %21\n"
+        "v_mad_u64_u32 %2, vcc, %9, %16, %21\n"
+        "v_mad_u64_u32 %3, vcc, %10, %17, %21\n"
+        "v_mad_u64_u32 %4, vcc, %11, %18, %21\n"
+        "v_mad_u64_u32 %5, vcc, %12, %19, %21\n"
+        "v_mad_u64_u32 %6, vcc, %13, %20, %21"
+        : "=&v"(t[0]), "=&v"(t[1]), "=&v"(t[2]), "=&v"(t[3]), "=&v"(t[4]), "=&v"(t[5]), "=&v"(t[6])
+        : "v"(x[I0]), "v"(x[I0 + 1]), "v"(x[I0 + 2]), "v"(x[I0 + 3]), "v"(x[I0 + 4]), "v"(x[I0 + 5]),
+          "v"(x[I0 + 6]), "s"(Poseidon2Lazy<W>::diag(I0)), "s"(Poseidon2Lazy<W>::diag(I0 + 1)),
+          "s"(Poseidon2Lazy<W>::diag(I0 + 2)), "s"(Poseidon2Lazy<W>::diag(I0 + 3)),
+          "s"(Poseidon2Lazy<W>::diag(I0 + 4)), "s"(Poseidon2Lazy<W>::diag(I0 + 5)),
+          "s"(Poseidon2Lazy<W>::diag(I0 + 6)), "v"(sum)
+        : "vcc");
+    return;
+  }
+#endif
+  for (int k = 0; k < K; ++k) t[k] = (uint64_t)x[I0 + k] * Poseidon2Lazy<W>::diag(I0 + k) + sum;
+}
+
 enum { LAZY_ADD_NONE = 0, LAZY_ADD_FIRST = 1, LAZY_ADD_ALL = 2 };
 
 // x <- REDC(MDS-light(x) + add), x_i < 2p on entry, < p + 2^7 on exit.
@@ -306,14 +349,32 @@ KB_HD void internal_lazy(uint32_t (&x)[W], int r, const Poseidon2Lazy<W>& C) {
   const uint64_t sum = mad_k<1>(x[0], sa + sb);
   uint64_t t0 = add_u(mad_k<2>(2u * P - x[0], sum), LAST ? C.lin_add[5][0] : C.int_add[r + 1]);
   x[0] = LAST ? canon2(redc(t0)) : redc(t0);
+  // words 1..W-1 in blocks of 8, 8 (W = 24) and 7
+  uint64_t t[W];
+  {
+    uint64_t b[8];
+    diag_mads<W, 1, 8>(x, sum, b);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t[1 + k] = b[k];
+  }
+  if constexpr (W == 24) {
+    uint64_t b[8];
+    diag_mads<W, 9, 8>(x, sum, b);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t[9 + k] = b[k];
+  }
+  {
+    uint64_t b[7];
+    diag_mads<W, W - 7, 7>(x, sum, b);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) t[W - 7 + k] = b[k];
+  }
 #pragma unroll
   for (int i = 1; i < W; ++i) {
-    uint64_t t = mad_u(x[i], Poseidon2Lazy<W>::diag(i), sum);
     if constexpr (LAST) {
-      t = add_u(t, C.lin_add[5][i]);
-      x[i] = canon2(redc(t));
+      x[i] = canon2(redc(add_u(t[i], C.lin_add[5][i])));
     } else {
-      x[i] = redc(t);
+      x[i] = redc(t[i]);
     }
   }
 }
