@@ -354,8 +354,11 @@ def main():
     whir_amd.lib().whir_device_synchronize()
     graph_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     prover.set_graph(False)
-    with open(os.path.join(ROOT, "whir_amd", "kernel_stats.json")) as fh:
-        kstats = json.load(fh)
+    try:  # written by the build from the library's own code object
+        with open(os.path.join(ROOT, "whir_amd", "kernel_stats.json")) as fh:
+            kstats = json.load(fh)
+    except OSError:
+        kstats = {}
     instr = kstats.get("leaves16_valu_per_lane")
     leaf_s = leaf_ms / leaf_n / 1e3
     achieved = instr * perms / leaf_s / 1e12 if instr else None
