@@ -310,8 +310,9 @@ def main():
         print(json.dumps({"copy_calibration_bytes": n, "copies": 3}))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the proof does not shard across GPUs here; see DESIGN.md).", file=sys.stderr)
+        print(f"bench.py: measures one GPU only; --gpus {args.gpus} is not supported "
+              "(a proof runs on a single GPU; batched proving scales by running one "
+              "process per GPU).", file=sys.stderr)
         sys.exit(2)
 
     import whir_amd
