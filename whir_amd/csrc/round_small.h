@@ -26,6 +26,7 @@ constexpr int SR_THREADS = 512;
 constexpr int SR_WAVES = SR_THREADS / 64;
 constexpr uint32_t SR_MAX_LOG_N = 10;
 constexpr int SR_MAX_OOD = 8;
+constexpr uint32_t SC_TAIL_MAX_LOG_N = 12;
 constexpr uint32_t SR_POW_BYTES = 96 * 1024;  // pow tables: count * (4 * 2^m0 + 16 * 2^m1) bytes
 
 // Host: whether a round of 2^kc elements with these claim counts fits the
@@ -101,6 +102,50 @@ __device__ __forceinline__ void eq_small(const Ext* z, uint32_t m, Ext* out) {
       out[threadIdx.x] = esub(lo, hi);
     }
     __syncthreads();
+  }
+}
+
+// f sumcheck rounds of the whole workgroup on a polynomial of nn extension
+// elements and its weights (expr.rs:52-70, poly.rs:131-142): per round
+// c0 = sum w[2i] f[2i], c2 = sum (2w[2i+1]-w[2i])(2f[2i+1]-f[2i]), written by
+// wave 0, the challenge r drawn into js->rs[rs_idx + round], then both arrays
+// folded, a[i] = a[2i] + r (a[2i+1] - a[2i]), into the other buffer of the pair.
+// After f rounds the folded arrays are in (F, Wt) if f is even, else in
+// (Fo, Wo). Ends with a barrier.
+// Launched with SR_THREADS threads.
+__device__ __forceinline__ void sumcheck_block(WaveTx& tx, JobScalars* js, Ext* F, Ext* Wt, Ext* Fo, Ext* Wo,
+                                               uint32_t nn, uint32_t f, uint32_t rs_idx, Ext* s_wave, Ext& s_x) {
+  const uint32_t tid = threadIdx.x, T = SR_THREADS;
+  for (uint32_t r = 0; r < f; ++r) {
+    Ext c0 = ezero(), c2 = ezero();
+    for (uint32_t i = tid; i < nn / 2; i += T)
+      pair_coeffs(ld_ext(F + 2 * i), ld_ext(F + 2 * i + 1), ld_ext(Wt + 2 * i), ld_ext(Wt + 2 * i + 1), c0, c2);
+    block_sum2_to_wave0(c0, c2, s_wave);
+    if (is_wave0()) {
+      tx.write_e(c0);
+      tx.write_e(c2);
+      const Ext ch = tx.sample_ext();
+      if (tid == 0) {
+        s_x = ch;
+        js->rs[rs_idx + r] = ch;
+      }
+    }
+    __syncthreads();
+    const Ext rr = s_x;
+    for (uint32_t i = tid; i < nn / 2; i += T) {
+      const Ext f0 = ld_ext(F + 2 * i), f1 = ld_ext(F + 2 * i + 1);
+      const Ext w0 = ld_ext(Wt + 2 * i), w1 = ld_ext(Wt + 2 * i + 1);
+      st_ext(Fo + i, eadd(f0, emul(rr, esub(f1, f0))));
+      st_ext(Wo + i, eadd(w0, emul(rr, esub(w1, w0))));
+    }
+    __syncthreads();
+    Ext* tf = F;
+    F = Fo;
+    Fo = tf;
+    Ext* tw = Wt;
+    Wt = Wo;
+    Wo = tw;
+    nn /= 2;
   }
 }
 
@@ -230,39 +275,23 @@ __global__ void __launch_bounds__(SR_THREADS) small_round_kernel(DevTx* t, uint3
   __syncthreads();
 
   // ---- f sumcheck rounds, each followed by the fold (expr.rs:52-70) ----
-  Ext *F = a.f_cur, *Wt = a.w_cur, *Fo = a.f_alt, *Wo = a.w_alt;
-  uint32_t nn = n;
-  for (uint32_t r = 0; r < a.f; ++r) {
-    Ext c0 = ezero(), c2 = ezero();
-    for (uint32_t i = tid; i < nn / 2; i += SR_THREADS)
-      pair_coeffs(ld_ext(F + 2 * i), ld_ext(F + 2 * i + 1), ld_ext(Wt + 2 * i), ld_ext(Wt + 2 * i + 1), c0, c2);
-    block_sum2_to_wave0(c0, c2, s_wave);
-    if (is_wave0()) {
-      tx.write_e(c0);
-      tx.write_e(c2);
-      const Ext ch = tx.sample_ext();
-      if (tid == 0) {
-        s_x = ch;
-        js->rs[a.rs_idx + r] = ch;
-      }
-    }
-    __syncthreads();
-    const Ext rr = s_x;
-    for (uint32_t i = tid; i < nn / 2; i += SR_THREADS) {
-      const Ext f0 = ld_ext(F + 2 * i), f1 = ld_ext(F + 2 * i + 1);
-      const Ext w0 = ld_ext(Wt + 2 * i), w1 = ld_ext(Wt + 2 * i + 1);
-      st_ext(Fo + i, eadd(f0, emul(rr, esub(f1, f0))));
-      st_ext(Wo + i, eadd(w0, emul(rr, esub(w1, w0))));
-    }
-    __syncthreads();
-    Ext* tf = F;
-    F = Fo;
-    Fo = tf;
-    Ext* tw = Wt;
-    Wt = Wo;
-    Wo = tw;
-    nn /= 2;
-  }
+  sumcheck_block(tx, js, a.f_cur, a.w_cur, a.f_alt, a.w_alt, n, a.f, a.rs_idx, s_wave, s_x);
+  if (is_wave0()) tx_end(tx, t, js);
+}
+
+// The f sumcheck rounds of a later WHIR round whose combined polynomial has
+// n <= 2^SC_TAIL_MAX_LOG_N extension elements, in one workgroup: the same
+// transcript and folded arrays as coeffs_kernel, then f times round_finalize
+// (fold_coeffs_kernel between them) and fold_kernel, in one launch instead of
+// 2f + 1 whose ~5 us floors are most of their time at this size.
+__global__ void __launch_bounds__(SR_THREADS) sumcheck_tail_kernel(DevTx* t, uint32_t* log, JobScalars* js, Ext* f,
+                                                                   Ext* w, Ext* fo, Ext* wo, uint32_t n, uint32_t nf,
+                                                                   uint32_t rs_idx) {
+  __shared__ Ext s_wave[2 * SR_WAVES];
+  __shared__ Ext s_x;
+  WaveTx tx;
+  if (is_wave0()) tx_begin(tx, t, log, js);
+  sumcheck_block(tx, js, f, w, fo, wo, n, nf, rs_idx, s_wave, s_x);
   if (is_wave0()) tx_end(tx, t, js);
 }
 

@@ -935,6 +935,15 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
     }
     launch_combine(s, P->ew[cur], n, om0, P->ood_pool, (uint32_t)rp.n_ood, P->ood_stride, P->powL, P->powR,
                    r > 0 ? P->stir_count : nullptr, om1);
+    if (r > 0 && n <= (1ull << SC_TAIL_MAX_LOG_N)) {
+      // the round's f sumcheck rounds and folds in one workgroup (round_small.h)
+      hipLaunchKernelGGL(sumcheck_tail_kernel, dim3(1), dim3(SR_THREADS), 0, s, P->tx, P->log, P->js, P->ep[cur],
+                         P->ew[cur], P->ep[cur ^ 1], P->ew[cur ^ 1], (uint32_t)n, (uint32_t)f, rs_idx);
+      n >>= f;
+      cur ^= f & 1;
+      rs_idx += f;
+      continue;
+    }
     for (int t = 0; t < f; ++t) {
       unsigned g;
       if (t == 0) {
