@@ -129,6 +129,45 @@ __device__ __forceinline__ void permute16_coop(uint32_t& s, const CoopP16& K) {
   s = canon2(mulr(x, K.fix));
 }
 
+// The transcript's permutation, out of line. A transcript kernel is one
+// workgroup running straight-line code once, so its time goes to fetching
+// instructions into a cold instruction cache as much as to executing them; with
+// the permutation (~900 instructions) inlined at every observe/sample site, each
+// site is a copy of its own and each executed copy is fetched cold (measured on
+// MI355X: a claim's draw, eq tables and value took ~14 us in one workgroup, 3-5x
+// their instruction count). One shared copy is fetched once per kernel (and
+// stays in L2 between kernels). Its arguments are the lane's state word and
+// lane constants; the wave-uniform constants come from constant memory.
+struct CoopLane {
+  uint32_t la[9];  // lin_add[k][lane]
+  uint32_t d;      // V_lane as an integer; 2 on lane 0
+
+  __device__ __forceinline__ void load() {
+    CoopP16 K;
+    K.load();
+#pragma unroll
+    for (int k = 0; k < 9; ++k) la[k] = K.la[k];
+    d = K.d;
+  }
+};
+
+__device__ __noinline__ uint32_t permute16_coop_shared(uint32_t s, CoopLane L) {
+  const Poseidon2Lazy<16>& C = c_p2l.w16;
+  const bool lane0 = (threadIdx.x & 15) == 0;
+  CoopP16 K;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) K.la[k] = L.la[k];
+  K.d = L.d;
+  K.ia[0] = 0;
+#pragma unroll
+  for (int r = 1; r < 20; ++r) K.ia[r] = lane0 ? C.int_add[r] : 0u;
+#pragma unroll
+  for (int r = 0; r < 20; ++r) K.kappa[r] = C.kappa[r];
+  K.fix = C.fix;
+  permute16_coop(s, K);
+  return s;
+}
+
 // One wavefront runs the transcript in uniform control flow. The sponge state
 // and both rate buffers are lane-distributed so nothing is indexed dynamically:
 //   s   : lane i holds state[i & 15];
@@ -141,7 +180,7 @@ struct WaveTx {
   uint32_t in_len, out_len;
   uint32_t log_words, log_cap, overflow;
   uint32_t* log;
-  CoopP16 k;
+  CoopLane k;
 
   // A fresh transcript: zero sponge state, empty buffers, an empty log of
   // `cap` words.
@@ -186,7 +225,7 @@ struct WaveTx {
     const uint32_t i = threadIdx.x & 15;
     if (i < in_len) s = pin;
     in_len = 0;
-    permute16_coop(s, k);
+    s = permute16_coop_shared(s, k);
     out_len = 8;
   }
   __device__ __forceinline__ void observe(uint32_t v) {
