@@ -111,67 +111,17 @@ __device__ __forceinline__ void write_root_body(WaveTx& tx, const uint32_t* root
   for (int j = 0; j < 8; ++j) tx.observe((uint32_t)__shfl((int)v, j, 8));
 }
 
-// OOD claims of a round polynomial small enough to evaluate inside the claim
-// kernel (2^kc <= 2^OOD_INLINE_MAX_LOG_N extension elements): claim o uses
-// slot cb.slot + o and the eq tables at cb.eq0 / cb.eq1 + o * stride.
-constexpr uint32_t OOD_INLINE_MAX_LOG_N = 12;
-struct OodInline {
-  const Ext* f;    // the round polynomial; nullptr: no inline claims
-  uint32_t count;  // number of OOD claims
-  uint64_t stride;
-};
-
-// Thread tid's elements x = tid + SERIAL_THREADS i (i < OOD_PER_THREAD, x < n)
-// of a round polynomial of n <= 2^OOD_INLINE_MAX_LOG_N elements, all loads
-// issued at once (zero past n).
-constexpr uint32_t OOD_PER_THREAD = (1u << OOD_INLINE_MAX_LOG_N) / SERIAL_THREADS;
-__device__ __forceinline__ void ood_load(const Ext* f, uint32_t n, Ext (&v)[OOD_PER_THREAD]) {
-#pragma unroll
-  for (uint32_t i = 0; i < OOD_PER_THREAD; ++i) {
-    const uint32_t x = threadIdx.x + i * SERIAL_THREADS;
-    v[i] = x < n ? ld_ext(f + x) : ezero();
-  }
-}
-
-// <f, eq0 (x) eq1> over n = 2^(m0 + m1) elements, m0 <= 6 and m1 <= EQ_LDS_LOG,
-// from the values ood_load left in v and the tables as build_eq_pair leaves
-// them in LDS (lo[u][x], hi[u][0] = 1), returned to every thread. With
-// SERIAL_THREADS a multiple of 2^m0, all of thread tid's elements have
-// x mod 2^m0 = tid mod 2^m0, so eq0 is applied once per thread.
-// `sh`: SERIAL_THREADS / 64 entries.
-__device__ __forceinline__ Ext ood_eval_block(const Ext (&v)[OOD_PER_THREAD], uint32_t m0, uint32_t m1,
-                                              const EqPairScratch& eq, Ext* sh) {
-  const uint32_t tid = threadIdx.x, n = 1u << (m0 + m1);
-  Ext acc = ezero();
-#pragma unroll
-  for (uint32_t i = 0; i < OOD_PER_THREAD; ++i) {
-    const uint32_t x = tid + i * SERIAL_THREADS;
-    if (x < n) acc = eadd(acc, emul(v[i], eq.lo[1][x >> m0]));
-  }
-  acc = emul(acc, eq.lo[0][tid & ((1u << m0) - 1)]);
-  return block_reduce_ext<SERIAL_THREADS>(acc, sh);
-}
-
 // The last cooperative launch of a commitment's tree (the top n0 <= 32 nodes
 // down to the root, merkle_coop_subtrees_kernel with G = 1), then its
 // transcript step in the same launch: transcript.write(root) and, when cb.slot
 // is set, the first claim's draw (claim_begin_body). fresh_cap != 0 starts a
 // new transcript with a log of that many words (the proof's first commitment).
-// With ood.f set, every OOD claim of the round runs here: draw, eq tables,
-// value <f, eq(point)> written to the transcript and the slot (whir.rs:224-235),
-// which is what claim_begin_body, eval_ext_kernel and claim_end_kernel do
-// over three launches per claim.
 static_assert(COOP_THREADS == SERIAL_THREADS, "one workgroup shape for the tree top and the claim");
-static_assert(SERIAL_THREADS % 64 == 0, "ood_eval_block: 2^m0 <= 64 divides the thread count");
 __global__ void __launch_bounds__(SERIAL_THREADS) tree_top_claim_kernel(uint32_t* layer, uint32_t n0, DevTx* t,
                                                                         uint32_t* log, JobScalars* js,
-                                                                        ClaimBegin cb, uint32_t fresh_cap,
-                                                                        OodInline ood) {
+                                                                        ClaimBegin cb, uint32_t fresh_cap) {
   __shared__ uint32_t s_node[COOP_MAX_NODES * 8];
   __shared__ ClaimScratch sc;
-  __shared__ Ext s_red[SERIAL_THREADS / 64];
-  Ext fv[OOD_PER_THREAD];  // loaded first: the latency overlaps the tree top and the draws
-  if (ood.f) ood_load(ood.f, 1u << (cb.m0 + cb.m1), fv);
   {
     CoopP16 K;
     K.load();
@@ -183,25 +133,7 @@ __global__ void __launch_bounds__(SERIAL_THREADS) tree_top_claim_kernel(uint32_t
     else tx_begin(tx, t, log, js);
     write_root_body(tx, s_node);
   }
-  if (ood.f) {
-    for (uint32_t o = 0; o < ood.count; ++o) {
-      ClaimBegin c = cb;
-      c.slot = cb.slot + o;
-      c.eq0 = cb.eq0 + o * ood.stride;
-      c.eq1 = cb.eq1 + o * ood.stride;
-      claim_begin_body(tx, c, sc);
-      __syncthreads();  // the LDS eq tables are complete
-      const Ext v = ood_eval_block(fv, c.m0, c.m1, sc.eq, s_red);
-      if (is_wave0()) tx.write_e(v);
-      if (threadIdx.x == 0) {
-        c.slot->compressed[0] = v;
-        c.slot->eval = v;
-      }
-      __syncthreads();  // s_red and the scratch are reused by the next claim
-    }
-  } else if (cb.slot) {
-    claim_begin_body(tx, cb, sc);
-  }
+  if (cb.slot) claim_begin_body(tx, cb, sc);
   if (is_wave0()) tx_end(tx, t, js);
 }
 

@@ -837,7 +837,7 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
   };
   // the tree's top levels, transcript.write(root) and the first group's draw
   hipLaunchKernelGGL(tree_top_claim_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, top, top_n, P->tx, P->log, P->js,
-                     group_begin(0), (uint32_t)pl.log_words, OodInline{});
+                     group_begin(0), (uint32_t)pl.log_words);
   for (int g = 0; g < n_groups; ++g) {
     const ClaimBegin cb = group_begin(g);
     const whir_prover::Group& G = P->groups[g];
@@ -894,13 +894,10 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
       }
       return cb;
     };
-    // the tree's top levels, transcript.write(root) and the first OOD draw;
-    // for a round polynomial of <= 2^OOD_INLINE_MAX_LOG_N elements every OOD claim
-    const bool ood_inline = rp.n_ood > 0 && rp.kc <= (int)OOD_INLINE_MAX_LOG_N;
+    // the tree's top levels, transcript.write(root) and the first OOD draw
     hipLaunchKernelGGL(tree_top_claim_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, top, top_n, P->tx, P->log, P->js,
-                       rp.n_ood > 0 ? ood_claim(0) : ClaimBegin{}, 0u,
-                       ood_inline ? OodInline{P->ep[cur], (uint32_t)rp.n_ood, P->ood_stride} : OodInline{});
-    for (int o = 0; o < (ood_inline ? 0 : rp.n_ood); ++o) {
+                       rp.n_ood > 0 ? ood_claim(0) : ClaimBegin{}, 0u);
+    for (int o = 0; o < rp.n_ood; ++o) {
       const ClaimBegin cb = ood_claim(o);
       hipLaunchKernelGGL(eval_ext_kernel, dim3(1u << om1), dim3(SC_THREADS), 0, s, P->ep[cur], cb.eq0, cb.eq1, om0,
                          P->partials);
@@ -911,7 +908,7 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
     // STIR draws (whir.rs:237-241) and the fold's alpha (prover.rs:377-384),
     // after the last OOD claim's value
     StirBegin sb{};
-    sb.end_slot = rp.n_ood > 0 && !ood_inline ? P->slots + P->n_svo_claims + (rp.n_ood - 1) : nullptr;
+    sb.end_slot = rp.n_ood > 0 ? P->slots + P->n_svo_claims + (rp.n_ood - 1) : nullptr;
     sb.partials = P->partials;
     sb.n_part = 1u << om1;
     sb.n = (uint32_t)rp.n_stir;
