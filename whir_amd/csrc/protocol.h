@@ -84,17 +84,45 @@ struct ClaimScratch {
   EqPairScratch eq;
 };
 
+// Extension squaring spread over the four lanes of a quad: lane q holds and
+// returns coefficient q. With x^4 = W = 3, coefficient q of a^2 is
+//   sum_{i <= q} a_i a_(q-i)  +  W sum_{i > q} a_i a_(q-i+4),
+// each sum below 4 p^2, so one reduce_sum4 each; the factors come from DPP
+// quad broadcasts and rotations (no memory). The same canonical words as esqr
+// with half its instructions and two short chains instead of one long one
+// (esqr on one lane is ~260 ns, and a claim point takes k in a row).
+constexpr int DPP_QUAD_BCAST(int i) { return i | (i << 2) | (i << 4) | (i << 6); }
+constexpr int DPP_QUAD_PREV = 3 | (0 << 2) | (1 << 4) | (2 << 6);  // lane a <- a-1 (mod 4)
+__device__ __forceinline__ uint32_t esqr_quad(uint32_t a, uint32_t q) {
+  const uint32_t a0 = dpp<DPP_QUAD_BCAST(0)>(a), a1 = dpp<DPP_QUAD_BCAST(1)>(a);
+  const uint32_t a2 = dpp<DPP_QUAD_BCAST(2)>(a), a3 = dpp<DPP_QUAD_BCAST(3)>(a);
+  // b_i = a_((q - i) mod 4)
+  const uint32_t b1 = dpp<DPP_QUAD_PREV>(a), b2 = dpp<DPP_QUAD_XOR2>(a), b3 = dpp<DPP_QUAD_NEXT>(a);
+  const uint32_t u1 = q >= 1 ? b1 : 0u, u2 = q >= 2 ? b2 : 0u, u3 = q >= 3 ? b3 : 0u;
+  const uint32_t v1 = b1 - u1, v2 = b2 - u2, v3 = b3 - u3;  // the wrapped factors, 0 when unwrapped
+  const uint64_t U = (uint64_t)a0 * a + (uint64_t)a1 * u1 + (uint64_t)a2 * u2 + (uint64_t)a3 * u3;
+  const uint64_t V = (uint64_t)a1 * v1 + (uint64_t)a2 * v2 + (uint64_t)a3 * v3;
+  const uint32_t u = reduce_sum4(U), v = reduce_sum4(V);
+  return add(u, add(dbl(v), v));
+}
+
+// pt[i] = z^(2^i) for i < k (expand, whir.rs:227-229 and test.rs:225-226),
+// called by every lane of one wavefront with the same z; lanes 0..3 store.
+__device__ __forceinline__ void expand_point_wave(const Ext& z, uint32_t k, Ext* pt) {
+  const uint32_t q = threadIdx.x & 3;
+  uint32_t a = q == 0 ? z.c[0] : q == 1 ? z.c[1] : q == 2 ? z.c[2] : z.c[3];
+  for (uint32_t i = 0; i < k; ++i) {
+    if ((threadIdx.x & 63) < 4) pt[i].c[q] = a;
+    a = esqr_quad(a, q);
+  }
+}
+
 // Draw the claim point: point = expand(k, draw()) (test.rs:225-226, prover.rs:240,
 // whir.rs:227-229), then build the eq tables of point[fsv..k) split at m0.
 __device__ void claim_begin_body(WaveTx& tx, const ClaimBegin& cb, ClaimScratch& sc) {
   if (is_wave0()) {
-    Ext z = tx.sample_ext();
-    if (threadIdx.x == 0) {
-      for (uint32_t i = 0; i < cb.k; ++i) {
-        sc.pt[i] = z;
-        z = esqr(z);
-      }
-    }
+    const Ext z = tx.sample_ext();
+    expand_point_wave(z, cb.k, sc.pt);
   }
   __syncthreads();
   for (uint32_t it = threadIdx.x; it < cb.count * cb.k; it += blockDim.x) cb.slot[it / cb.k].point[it % cb.k] = sc.pt[it % cb.k];

@@ -174,12 +174,8 @@ __global__ void __launch_bounds__(SR_THREADS) small_round_kernel(DevTx* t, uint3
   // ---- OOD claims: z -> expand(kc, z); value <f, eq(point)> (whir.rs:224-235) ----
   for (uint32_t o = 0; o < a.n_ood; ++o) {
     if (is_wave0()) {
-      Ext z = tx.sample_ext();
-      if (tid == 0)
-        for (uint32_t i = 0; i < a.kc; ++i) {
-          s_pt[i] = z;
-          z = esqr(z);
-        }
+      const Ext z = tx.sample_ext();
+      expand_point_wave(z, a.kc, s_pt);
     }
     __syncthreads();
     eq_small(s_pt, m0, s_eq[o][0]);
