@@ -713,15 +713,13 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_query_kernel(const DevTx*
 }
 
 // Reduce the (c0, c2) partials, add the fold_with_code query terms
-// (prover.rs:509-534), write c0, c2, draw r (expr.rs:52-70), and advance the
-// per-query state (prover.rs:542-553).
+// (prover.rs:509-534), write c0, c2 and draw r (expr.rs:52-70). The per-query
+// state is advanced with r (prover.rs:542-553) by the last workgroup of the
+// next fold launch (query_fold_block), off the transcript's chain.
 __global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(
     DevTx* t, uint32_t* log, JobScalars* js, const Ext* partials, uint32_t n_part, uint32_t rs_index, int fwc,
-    QueryArrays qa, const uint32_t* count_ptr, uint32_t d, uint32_t k_data, uint32_t rho, const uint32_t* tw_lo,
-    const uint32_t* tw_hi) {
+    QueryArrays qa, const uint32_t* count_ptr, uint32_t d, uint32_t k_data, uint32_t rho) {
   __shared__ Ext sh[SERIAL_THREADS];
-  __shared__ Ext s_r;
-  __shared__ uint32_t s_index[MAX_STIR];
   const uint32_t tid = threadIdx.x;
   WaveTx tx;
   if (is_wave0()) tx_begin(tx, t, log, js);
@@ -764,19 +762,36 @@ __global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(
     tx.write_e(C2);
     const Ext r = tx.sample_ext();
     tx_end(tx, t, js);
-    if (tid == 0) {
-      s_r = r;
-      js->rs[rs_index] = r;
-    }
+    if (tid == 0) js->rs[rs_index] = r;
   }
-  __syncthreads();
-  if (!fwc) return;
-  const Ext r = s_r;
-  for (uint32_t q = tid; q < count; q += SERIAL_THREADS) {
+}
+
+// The per-query state of fold_with_code after a draw r at fold step rho
+// (prover.rs:542-553): scale *= 1 + r (w - 1), w and t squared (t negated by
+// the index bit), and G_{rho+1}[tt] = fold(G[tt], G[tt + h], omega^-i, r) for
+// every (query, tt), four items' loads in flight. QUERY_BLOCKS workgroups of
+// SERIAL_THREADS threads (qb of them here; the two loops touch disjoint state),
+// run as the last workgroups of the next fold launch: the next
+// round_finalize_kernel reads this state after that launch ends.
+struct QueryFold {
+  QueryArrays qa;
+  const uint32_t* count_ptr;
+  uint32_t d, k_data, rho;
+  const uint32_t* tw_lo;
+  const uint32_t* tw_hi;
+};
+
+constexpr uint32_t QUERY_BLOCKS = 4;
+static_assert(SC_THREADS == SERIAL_THREADS, "the query workgroups run in a fold launch");
+__device__ void query_fold_block(const QueryFold& qf, const Ext r, uint32_t qb) {
+  const QueryArrays& qa = qf.qa;
+  const uint32_t tid = qb * SERIAL_THREADS + threadIdx.x, count = *qf.count_ptr;
+  constexpr uint32_t S = QUERY_BLOCKS * SERIAL_THREADS;
+  const uint32_t d = qf.d, k_data = qf.k_data, rho = qf.rho;
+  const uint32_t GS = 1u << MAX_F;
+  for (uint32_t q = tid; q < count; q += S) {
     const uint32_t index = qa.index[q];
-    s_index[q] = index;
     const uint32_t ws = qa.ws[q];
-    // s *= 1 + r (w - 1)
     qa.scale[q] = emul(qa.scale[q], eadd(eone(), emul(r, esub(efrom(ws), eone()))));
     qa.ws[q] = sqr(ws);
     uint32_t ts = sqr(qa.ts[q]);
@@ -784,28 +799,26 @@ __global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(
     qa.ts[q] = ts;
   }
   if (rho + 1 >= d) return;
-  __syncthreads();
-  // G_{rho+1}[tt] = fold(G[tt], G[tt + h], omega_{2^(k_data-rho)}^{-i}, r), i = b + tt*M,
-  // one (query, tt) item per thread, four items' loads in flight
+  // G_{rho+1}[tt] = fold(G[tt], G[tt + h], omega_{2^(k_data-rho)}^{-i}, r), i = b + tt*M
   const uint32_t h = 1u << (d - rho - 1);
   const uint32_t M = 1u << (k_data - d);
   const uint32_t lg = k_data - rho;
   const uint32_t lh = d - rho - 1;  // log2 h
   const uint32_t n_items = count << lh;
-  for (uint32_t it0 = tid; it0 < n_items; it0 += 4 * SERIAL_THREADS) {
+  for (uint32_t it0 = tid; it0 < n_items; it0 += 4 * S) {
     Ext g_lo[4], g_hi[4];
     uint32_t tw[4], gi[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const uint32_t it = it0 + (uint32_t)u * SERIAL_THREADS;
+      const uint32_t it = it0 + (uint32_t)u * S;
       gi[u] = 0xFFFFFFFFu;
       if (it < n_items) {
         const uint32_t q = it >> lh, tt = it & (h - 1);
-        const uint32_t i = (s_index[q] & (M - 1)) + tt * M;
+        const uint32_t i = (qa.index[q] & (M - 1)) + tt * M;
         const uint32_t e = ((1u << lg) - i) & ((1u << lg) - 1);
         const uint32_t e24 = e << (24 - lg);
         gi[u] = q * GS + tt;
-        tw[u] = mul(tw_lo[e24 & 4095u], tw_hi[e24 >> 12]);
+        tw[u] = mul(qf.tw_lo[e24 & 4095u], qf.tw_hi[e24 >> 12]);
         g_lo[u] = qa.G[gi[u]];
         g_hi[u] = qa.G[gi[u] + h];
       }
@@ -818,6 +831,27 @@ __global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(
       qa.G[gi[u]] = eadd(even, emul(r, esub(odd, even)));
     }
   }
+}
+
+// fold_coeffs_kernel / fold_kernel with QUERY_BLOCKS more workgroups, the
+// last ones, for query_fold_block with the same r.
+__global__ void __launch_bounds__(SC_THREADS) fold_coeffs_q_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
+                                                               uint64_t n, const Ext* __restrict__ r_ptr,
+                                                               Ext* __restrict__ fo, Ext* __restrict__ wo,
+                                                               Ext* __restrict__ partials, QueryFold qf) {
+  __shared__ Ext sh[SC_THREADS];
+  const Ext r = ld_ext(r_ptr);
+  const uint32_t nf = gridDim.x - QUERY_BLOCKS;
+  if (blockIdx.x >= nf) query_fold_block(qf, r, blockIdx.x - nf);
+  else fold_coeffs_body(f, w, n, r, fo, wo, partials, blockIdx.x, nf, sh);
+}
+__global__ void __launch_bounds__(SC_THREADS) fold_q_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
+                                                        uint64_t n, const Ext* __restrict__ r_ptr,
+                                                        Ext* __restrict__ fo, Ext* __restrict__ wo, QueryFold qf) {
+  const Ext r = ld_ext(r_ptr);
+  const uint32_t nf = gridDim.x - QUERY_BLOCKS;
+  if (blockIdx.x >= nf) query_fold_block(qf, r, blockIdx.x - nf);
+  else fold_body(f, w, n, r, fo, wo, blockIdx.x);
 }
 
 }  // namespace kb

@@ -434,15 +434,13 @@ __global__ void __launch_bounds__(SC_THREADS) coeffs_kernel(const Ext* __restric
 
 // Fold both arrays by r (a[i] = a[2i] + r (a[2i+1] - a[2i])) and emit the next
 // round's (c0, c2) partials computed on the folded values.
-__global__ void __launch_bounds__(SC_THREADS) fold_coeffs_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
-                                                             uint64_t n, const Ext* __restrict__ r_ptr,
-                                                             Ext* __restrict__ fo, Ext* __restrict__ wo,
-                                                             Ext* __restrict__ partials) {
-  __shared__ Ext sh[SC_THREADS];
-  const Ext r = ld_ext(r_ptr);
+// Block blk of nblk (blockDim.x = SC_THREADS); sh: SC_THREADS entries.
+__device__ __forceinline__ void fold_coeffs_body(const Ext* __restrict__ f, const Ext* __restrict__ w, uint64_t n,
+                                                 const Ext r, Ext* __restrict__ fo, Ext* __restrict__ wo,
+                                                 Ext* __restrict__ partials, uint32_t blk, uint32_t nblk, Ext* sh) {
   Ext c0 = ezero(), c2 = ezero();
   const uint64_t nq = n / 4;
-  for (uint64_t i = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; i < nq; i += (uint64_t)gridDim.x * SC_THREADS) {
+  for (uint64_t i = (uint64_t)blk * SC_THREADS + threadIdx.x; i < nq; i += (uint64_t)nblk * SC_THREADS) {
     const Ext a0 = ld_ext(f + 4 * i), a1 = ld_ext(f + 4 * i + 1), a2 = ld_ext(f + 4 * i + 2), a3 = ld_ext(f + 4 * i + 3);
     const Ext b0 = ld_ext(w + 4 * i), b1 = ld_ext(w + 4 * i + 1), b2 = ld_ext(w + 4 * i + 2), b3 = ld_ext(w + 4 * i + 3);
     const Ext F0 = eadd(a0, emul(r, esub(a1, a0)));
@@ -459,21 +457,33 @@ __global__ void __launch_bounds__(SC_THREADS) fold_coeffs_kernel(const Ext* __re
   __syncthreads();
   const Ext s2 = block_reduce_ext<SC_THREADS>(c2, sh);
   if (threadIdx.x == 0) {
-    st_ext(partials + 2 * blockIdx.x, s0);
-    st_ext(partials + 2 * blockIdx.x + 1, s2);
+    st_ext(partials + 2 * blk, s0);
+    st_ext(partials + 2 * blk + 1, s2);
   }
 }
 
-__global__ void __launch_bounds__(SC_THREADS) fold_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
-                                                      uint64_t n, const Ext* __restrict__ r_ptr, Ext* __restrict__ fo,
-                                                      Ext* __restrict__ wo) {
-  const Ext r = ld_ext(r_ptr);
-  const uint64_t i = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x;
+__global__ void __launch_bounds__(SC_THREADS) fold_coeffs_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
+                                                             uint64_t n, const Ext* __restrict__ r_ptr,
+                                                             Ext* __restrict__ fo, Ext* __restrict__ wo,
+                                                             Ext* __restrict__ partials) {
+  __shared__ Ext sh[SC_THREADS];
+  fold_coeffs_body(f, w, n, ld_ext(r_ptr), fo, wo, partials, blockIdx.x, gridDim.x, sh);
+}
+
+__device__ __forceinline__ void fold_body(const Ext* __restrict__ f, const Ext* __restrict__ w, uint64_t n,
+                                          const Ext r, Ext* __restrict__ fo, Ext* __restrict__ wo, uint32_t blk) {
+  const uint64_t i = (uint64_t)blk * SC_THREADS + threadIdx.x;
   if (i >= n / 2) return;
   const Ext a0 = ld_ext(f + 2 * i), a1 = ld_ext(f + 2 * i + 1);
   const Ext b0 = ld_ext(w + 2 * i), b1 = ld_ext(w + 2 * i + 1);
   st_ext(fo + i, eadd(a0, emul(r, esub(a1, a0))));
   st_ext(wo + i, eadd(b0, emul(r, esub(b1, b0))));
+}
+
+__global__ void __launch_bounds__(SC_THREADS) fold_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
+                                                      uint64_t n, const Ext* __restrict__ r_ptr, Ext* __restrict__ fo,
+                                                      Ext* __restrict__ wo) {
+  fold_body(f, w, n, ld_ext(r_ptr), fo, wo, blockIdx.x);
 }
 
 }  // namespace kb

@@ -944,6 +944,11 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
       rs_idx += f;
       continue;
     }
+    // round 0 (fold_with_code): the per-query state after the draw of step t
+    // advances in QUERY_BLOCKS more workgroups of the next fold launch
+    auto qfold = [&](int rho) {
+      return QueryFold{P->qa, P->stir_count, (uint32_t)f, (uint32_t)rp.kfd, (uint32_t)rho, c->tw_lo(), c->tw_hi()};
+    };
     for (int t = 0; t < f; ++t) {
       unsigned g;
       if (t == 0) {
@@ -951,18 +956,26 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
         hipLaunchKernelGGL(coeffs_kernel, dim3(g), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n, P->partials);
       } else {
         g = std::min<unsigned>(1024, blocks_for(n / 4));
-        hipLaunchKernelGGL(fold_coeffs_kernel, dim3(g), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
-                           P->js->rs + (rs_idx - 1), P->ep[cur ^ 1], P->ew[cur ^ 1], P->partials);
+        if (r == 0)
+          hipLaunchKernelGGL(fold_coeffs_q_kernel, dim3(g + QUERY_BLOCKS), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
+                             P->js->rs + (rs_idx - 1), P->ep[cur ^ 1], P->ew[cur ^ 1], P->partials, qfold(t - 1));
+        else
+          hipLaunchKernelGGL(fold_coeffs_kernel, dim3(g), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
+                             P->js->rs + (rs_idx - 1), P->ep[cur ^ 1], P->ew[cur ^ 1], P->partials);
         n /= 2;
         cur ^= 1;
       }
       hipLaunchKernelGGL(round_finalize_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js,
                          P->partials, g, rs_idx, r == 0 ? 1 : 0, P->qa, P->stir_count, (uint32_t)f,
-                         (uint32_t)rp.kfd, (uint32_t)t, c->tw_lo(), c->tw_hi());
+                         (uint32_t)rp.kfd, (uint32_t)t);
       ++rs_idx;
     }
-    hipLaunchKernelGGL(fold_kernel, dim3(blocks_for(n / 2)), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
-                       P->js->rs + (rs_idx - 1), P->ep[cur ^ 1], P->ew[cur ^ 1]);
+    if (r == 0)
+      hipLaunchKernelGGL(fold_q_kernel, dim3(blocks_for(n / 2) + QUERY_BLOCKS), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur],
+                         n, P->js->rs + (rs_idx - 1), P->ep[cur ^ 1], P->ew[cur ^ 1], qfold(f - 1));
+    else
+      hipLaunchKernelGGL(fold_kernel, dim3(blocks_for(n / 2)), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
+                         P->js->rs + (rs_idx - 1), P->ep[cur ^ 1], P->ew[cur ^ 1]);
     n /= 2;
     cur ^= 1;
     if (r == 0) {
