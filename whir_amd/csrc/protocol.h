@@ -569,8 +569,9 @@ __device__ uint32_t sort_dedup_block(uint32_t* s_v, uint32_t n, uint32_t* s_scan
 //   n draws of `bits` bits -> sorted distinct indexes (whir.rs:63-77, 238);
 //   draw alpha (prover.rs:377-384; the openings written afterwards are hints,
 //   which the sponge does not absorb, so alpha can be drawn before them);
-//   scale the eq1 table of OOD claim c >= 1 by alpha^c; query scales
-//   alpha^(n_eq + q) (combine.rs:37-64, prover.rs:419-425).
+//   with query_scales, the query scales alpha^(n_eq + q) (combine.rs:37-64,
+//   prover.rs:419-425), which the same round's openings launch reads; the
+//   rest of alpha's scaling is stir_scales_block's.
 struct StirBegin {
   ClaimSlot* end_slot;  // OOD claim whose value this kernel writes first (nullptr: none)
   const Ext* partials;  // its eval_ext partials
@@ -582,6 +583,7 @@ struct StirBegin {
   uint64_t eq_stride;
   uint32_t n_eq, m0, m1;
   QueryArrays qa;
+  uint32_t query_scales;  // 0: stir_query_kernel's extra workgroups write them
 };
 
 __global__ void __launch_bounds__(SERIAL_THREADS) stir_begin_kernel(DevTx* t, uint32_t* log, JobScalars* js,
@@ -591,7 +593,6 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_begin_kernel(DevTx* t, ui
   __shared__ uint32_t s_v[MAX_STIR];
   __shared__ uint32_t s_scan[SERIAL_THREADS];
   __shared__ Ext s_alpha;
-  __shared__ Ext s_apow[16];
   const uint32_t tid = threadIdx.x;
   WaveTx tx;
   if (is_wave0()) tx_begin(tx, t, log, js);
@@ -604,11 +605,6 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_begin_kernel(DevTx* t, ui
     if (tid == 0) {
       s_alpha = al;
       js->alpha = al;
-      Ext p = eone();
-      for (uint32_t c = 0; c < a.n_eq && c < 16; ++c) {
-        s_apow[c] = p;
-        p = emul(p, al);
-      }
     }
   }
   const uint32_t count = sort_dedup_block(s_v, a.n, s_scan, a.idx_out);  // barriers make s_alpha visible
@@ -616,6 +612,7 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_begin_kernel(DevTx* t, ui
     *a.count_out = count;
     js->stir_count = count;
   }
+  if (!a.query_scales) return;
   // query scales: thread t takes q = t + 256 i as alpha^(n_eq + t) alpha^(256 i)
   Ext sq = epow(s_alpha, (uint64_t)a.n_eq + tid), step = s_alpha;
   for (int i = 0; i < 8; ++i) step = esqr(step);  // alpha^256 (SERIAL_THREADS)
@@ -623,22 +620,42 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_begin_kernel(DevTx* t, ui
     a.qa.scale[q] = sq;
     sq = emul(sq, step);
   }
-  // eq1 of OOD claim c >= 1 times alpha^c, four loads in flight
-  const uint64_t n0 = 1ull << a.m0, n1 = 1ull << a.m1;
-  const uint64_t n_sc = a.n_eq > 1 ? (uint64_t)(a.n_eq - 1) * n1 : 0;
-  for (uint64_t it0 = tid; it0 < n_sc; it0 += 4 * SERIAL_THREADS) {
+}
+
+// What the STIR step's alpha scales and no transcript kernel reads before the
+// next claim combination: eq1 of OOD claim c >= 1 times alpha^c (combine.rs:
+// 37-64), and in round 0 the query scales alpha^(n_eq + q) (read first by
+// round_finalize_kernel). STIR_EXTRA_BLOCKS workgroups appended to the
+// openings launch do it, off the transcript's chain.
+constexpr uint32_t STIR_EXTRA_BLOCKS = 4;
+struct StirScales {
+  Ext* eq_pool;  // OOD claims' split eq tables (eq0: 2^m0 entries, then eq1)
+  uint64_t eq_stride;
+  uint32_t n_eq, m0, m1;
+  Ext* scale;    // query scales to write; nullptr: stir_begin_kernel wrote them
+};
+__device__ void stir_scales_block(const StirScales& ss, const Ext alpha, uint32_t count, uint32_t eb) {
+  constexpr uint32_t S = STIR_EXTRA_BLOCKS * SERIAL_THREADS;
+  const uint32_t tid = eb * SERIAL_THREADS + threadIdx.x;
+  if (ss.scale)
+    for (uint32_t q = tid; q < count; q += S) ss.scale[q] = epow(alpha, (uint64_t)ss.n_eq + q);
+  // four loads in flight
+  const uint64_t n0 = 1ull << ss.m0, n1 = 1ull << ss.m1;
+  const uint64_t n_sc = ss.n_eq > 1 ? (uint64_t)(ss.n_eq - 1) * n1 : 0;
+  for (uint64_t it0 = tid; it0 < n_sc; it0 += 4 * S) {
     Ext v[4];
     Ext* p[4];
+    uint32_t c[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const uint64_t it = it0 + (uint64_t)u * SERIAL_THREADS;
-      const uint64_t c = 1 + it / n1, x = it % n1;
-      p[u] = it < n_sc ? a.eq_pool + c * a.eq_stride + n0 + x : nullptr;
+      const uint64_t it = it0 + (uint64_t)u * S;
+      c[u] = (uint32_t)(1 + it / n1);
+      p[u] = it < n_sc ? ss.eq_pool + c[u] * ss.eq_stride + n0 + it % n1 : nullptr;
       if (p[u]) v[u] = ld_ext(p[u]);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (p[u]) st_ext(p[u], emul(v[u], s_apow[1 + (it0 + (uint64_t)u * SERIAL_THREADS) / n1]));
+      if (p[u]) st_ext(p[u], emul(v[u], epow(alpha, c[u])));
   }
 }
 
@@ -669,8 +686,14 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_query_kernel(const DevTx*
                                                                     const uint32_t* cw, uint32_t row_words,
                                                                     const uint32_t* tree, uint32_t depth, int mode,
                                                                     uint32_t d, QueryArrays qa, uint32_t m0,
-                                                                    uint32_t m1, uint32_t* powL, Ext* powR) {
+                                                                    uint32_t m1, uint32_t* powL, Ext* powR,
+                                                                    StirScales ss) {
   const uint32_t count = *count_ptr;
+  const uint32_t n_open = gridDim.x - STIR_EXTRA_BLOCKS;  // workgroups for openings
+  if (blockIdx.x >= n_open) {
+    stir_scales_block(ss, js->alpha, count, blockIdx.x - n_open);
+    return;
+  }
   const uint32_t q = blockIdx.x;
   const uint32_t per = row_words + 8 * depth;
   if (blockIdx.x == 0 && threadIdx.x == 0) js->pending_words = count * per;

@@ -921,17 +921,22 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
     sb.m0 = om0;
     sb.m1 = om1;
     sb.qa = P->qa;
+    sb.query_scales = r > 0 ? 1u : 0u;  // later rounds' openings launch reads them
     hipLaunchKernelGGL(stir_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, sb);
     // STIR openings (whir.rs:242-293); later rounds also build the pow tables
-    // of combine_pows_scaled (prover.rs:419-425)
+    // of combine_pows_scaled (prover.rs:419-425); STIR_EXTRA_BLOCKS more
+    // workgroups scale by alpha what the combination reads next
+    const StirScales ss{P->ood_pool, P->ood_stride, (uint32_t)rp.n_ood, (uint32_t)om0, (uint32_t)om1,
+                        r == 0 ? P->qa.scale : nullptr};
+    const dim3 qgrid((unsigned)rp.n_stir + STIR_EXTRA_BLOCKS);
     if (r == 0) {
-      hipLaunchKernelGGL(stir_query_kernel, dim3(rp.n_stir), dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log,
-                         P->stir_idx, P->stir_count, P->cw0, W, P->tree0, (uint32_t)rp.kfd, 1, (uint32_t)f, P->qa,
-                         0u, 0u, (uint32_t*)nullptr, (Ext*)nullptr);
+      hipLaunchKernelGGL(stir_query_kernel, qgrid, dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log, P->stir_idx,
+                         P->stir_count, P->cw0, W, P->tree0, (uint32_t)rp.kfd, 1, (uint32_t)f, P->qa, 0u, 0u,
+                         (uint32_t*)nullptr, (Ext*)nullptr, ss);
     } else {
-      hipLaunchKernelGGL(stir_query_kernel, dim3(rp.n_stir), dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log,
-                         P->stir_idx, P->stir_count, P->cw[(r - 1) & 1], 4 * W, P->tree[(r - 1) & 1],
-                         (uint32_t)rp.kfd, 2, (uint32_t)f, P->qa, om0, om1, P->powL, P->powR);
+      hipLaunchKernelGGL(stir_query_kernel, qgrid, dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log, P->stir_idx,
+                         P->stir_count, P->cw[(r - 1) & 1], 4 * W, P->tree[(r - 1) & 1], (uint32_t)rp.kfd, 2,
+                         (uint32_t)f, P->qa, om0, om1, P->powL, P->powR, ss);
     }
     launch_combine(s, P->ew[cur], n, om0, P->ood_pool, (uint32_t)rp.n_ood, P->ood_stride, P->powL, P->powR,
                    r > 0 ? P->stir_count : nullptr, om1);
