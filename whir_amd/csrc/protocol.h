@@ -735,13 +735,14 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_query_kernel(const DevTx*
   }
 }
 
-// Reduce the (c0, c2) partials, add the fold_with_code query terms
-// (prover.rs:509-534), write c0, c2 and draw r (expr.rs:52-70). The per-query
-// state is advanced with r (prover.rs:542-553) by the last workgroup of the
-// next fold launch (query_fold_block), off the transcript's chain.
-__global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(
-    DevTx* t, uint32_t* log, JobScalars* js, const Ext* partials, uint32_t n_part, uint32_t rs_index, int fwc,
-    QueryArrays qa, const uint32_t* count_ptr, uint32_t d, uint32_t k_data, uint32_t rho) {
+// Reduce the (c0, c2) partials, write c0, c2 and draw r (expr.rs:52-70). In
+// round 0 the fold_with_code query terms (prover.rs:509-534) are among the
+// partials (query_terms_partial, by the query workgroups of the launch before)
+// and the per-query state advances with r (prover.rs:542-553) in the next
+// fold launch (query_fold_block): both off the transcript's chain.
+__global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(DevTx* t, uint32_t* log, JobScalars* js,
+                                                                        const Ext* partials, uint32_t n_part,
+                                                                        uint32_t rs_index) {
   __shared__ Ext sh[SERIAL_THREADS];
   const uint32_t tid = threadIdx.x;
   WaveTx tx;
@@ -761,22 +762,6 @@ __global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(
       c2 = eadd(c2, b[u]);
     }
   }
-  const uint32_t count = fwc ? *count_ptr : 0;
-  const uint32_t GS = 1u << MAX_F;
-  if (fwc) {
-    const uint32_t half = 1u << (d - rho - 1);  // t_hi = t_lo + half
-    for (uint32_t q = tid; q < count; q += SERIAL_THREADS) {
-      const uint32_t index = qa.index[q];
-      const uint32_t t_lo = (index >> (k_data - d)) & (half - 1);
-      const Ext g_lo = qa.G[q * GS + t_lo], g_hi = qa.G[q * GS + t_lo + half];
-      const Ext even = emul_base(eadd(g_lo, g_hi), HALF);
-      const Ext odd = emul_base(emul_base(esub(g_lo, g_hi), HALF), qa.ts[q]);
-      const Ext s = qa.scale[q];
-      c0 = eadd(c0, emul(s, even));
-      const uint32_t wt = sub(dbl(qa.ws[q]), ONE);
-      c2 = eadd(c2, emul_base(emul(s, esub(edbl(odd), even)), wt));
-    }
-  }
   const Ext C0 = block_reduce_ext<SERIAL_THREADS>(c0, sh);
   __syncthreads();
   const Ext C2 = block_reduce_ext<SERIAL_THREADS>(c2, sh);
@@ -792,10 +777,12 @@ __global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(
 // The per-query state of fold_with_code after a draw r at fold step rho
 // (prover.rs:542-553): scale *= 1 + r (w - 1), w and t squared (t negated by
 // the index bit), and G_{rho+1}[tt] = fold(G[tt], G[tt + h], omega^-i, r) for
-// every (query, tt), four items' loads in flight. QUERY_BLOCKS workgroups of
-// SERIAL_THREADS threads (qb of them here; the two loops touch disjoint state),
-// run as the last workgroups of the next fold launch: the next
-// round_finalize_kernel reads this state after that launch ends.
+// every (query, tt), four items' loads in flight. Then, with `partials`, the
+// query terms of step rho + 1 (query_terms_partial). QUERY_BLOCKS workgroups
+// of SERIAL_THREADS threads, workgroup qb taking the queries q = qb mod
+// QUERY_BLOCKS (so that a query's terms read only its own workgroup's
+// writes), run as the last workgroups of the next fold launch: the next
+// round_finalize_kernel reads their partials after that launch ends.
 struct QueryFold {
   QueryArrays qa;
   const uint32_t* count_ptr;
@@ -806,13 +793,45 @@ struct QueryFold {
 
 constexpr uint32_t QUERY_BLOCKS = 4;
 static_assert(SC_THREADS == SERIAL_THREADS, "the query workgroups run in a fold launch");
-__device__ void query_fold_block(const QueryFold& qf, const Ext r, uint32_t qb) {
+
+// The fold_with_code terms of step rho (prover.rs:509-534) summed over the
+// queries q = qb mod QUERY_BLOCKS: c0 += s even, c2 += s (2 odd - even)(2 w - 1),
+// written as partial pair pidx. sh: SERIAL_THREADS / 64 entries.
+__device__ void query_terms_partial(const QueryArrays& qa, uint32_t count, uint32_t qb, uint32_t d, uint32_t k_data,
+                                    uint32_t rho, Ext* partials, uint32_t pidx, Ext* sh) {
+  const uint32_t GS = 1u << MAX_F;
+  const uint32_t half = 1u << (d - rho - 1);  // t_hi = t_lo + half
+  Ext c0 = ezero(), c2 = ezero();
+  for (uint32_t q = qb + QUERY_BLOCKS * threadIdx.x; q < count; q += QUERY_BLOCKS * SERIAL_THREADS) {
+    const uint32_t index = qa.index[q];
+    const uint32_t t_lo = (index >> (k_data - d)) & (half - 1);
+    const Ext g_lo = qa.G[q * GS + t_lo], g_hi = qa.G[q * GS + t_lo + half];
+    const Ext even = emul_base(eadd(g_lo, g_hi), HALF);
+    const Ext odd = emul_base(emul_base(esub(g_lo, g_hi), HALF), qa.ts[q]);
+    const Ext sc = qa.scale[q];
+    c0 = eadd(c0, emul(sc, even));
+    const uint32_t wt = sub(dbl(qa.ws[q]), ONE);
+    c2 = eadd(c2, emul_base(emul(sc, esub(edbl(odd), even)), wt));
+  }
+  const Ext s0 = block_reduce_ext<SERIAL_THREADS>(c0, sh);
+  __syncthreads();
+  const Ext s2 = block_reduce_ext<SERIAL_THREADS>(c2, sh);
+  if (threadIdx.x == 0) {
+    st_ext(partials + 2 * pidx, s0);
+    st_ext(partials + 2 * pidx + 1, s2);
+  }
+}
+
+__device__ void query_fold_block(const QueryFold& qf, const Ext r, uint32_t qb, Ext* partials, uint32_t pidx,
+                                 Ext* sh) {
   const QueryArrays& qa = qf.qa;
-  const uint32_t tid = qb * SERIAL_THREADS + threadIdx.x, count = *qf.count_ptr;
-  constexpr uint32_t S = QUERY_BLOCKS * SERIAL_THREADS;
+  const uint32_t count = *qf.count_ptr;
   const uint32_t d = qf.d, k_data = qf.k_data, rho = qf.rho;
   const uint32_t GS = 1u << MAX_F;
-  for (uint32_t q = tid; q < count; q += S) {
+  // this workgroup's queries: q = qb + QUERY_BLOCKS ql, ql < nq
+  const uint32_t nq = count > qb ? (count - qb + QUERY_BLOCKS - 1) / QUERY_BLOCKS : 0u;
+  for (uint32_t ql = threadIdx.x; ql < nq; ql += SERIAL_THREADS) {
+    const uint32_t q = qb + QUERY_BLOCKS * ql;
     const uint32_t index = qa.index[q];
     const uint32_t ws = qa.ws[q];
     qa.scale[q] = emul(qa.scale[q], eadd(eone(), emul(r, esub(efrom(ws), eone()))));
@@ -821,43 +840,57 @@ __device__ void query_fold_block(const QueryFold& qf, const Ext r, uint32_t qb) 
     if (rho + 2 <= k_data && ((index >> (k_data - rho - 2)) & 1)) ts = neg(ts);
     qa.ts[q] = ts;
   }
-  if (rho + 1 >= d) return;
-  // G_{rho+1}[tt] = fold(G[tt], G[tt + h], omega_{2^(k_data-rho)}^{-i}, r), i = b + tt*M
-  const uint32_t h = 1u << (d - rho - 1);
-  const uint32_t M = 1u << (k_data - d);
-  const uint32_t lg = k_data - rho;
-  const uint32_t lh = d - rho - 1;  // log2 h
-  const uint32_t n_items = count << lh;
-  for (uint32_t it0 = tid; it0 < n_items; it0 += 4 * S) {
-    Ext g_lo[4], g_hi[4];
-    uint32_t tw[4], gi[4];
+  if (rho + 1 < d) {
+    // G_{rho+1}[tt] = fold(G[tt], G[tt + h], omega_{2^(k_data-rho)}^{-i}, r), i = b + tt*M
+    const uint32_t h = 1u << (d - rho - 1);
+    const uint32_t M = 1u << (k_data - d);
+    const uint32_t lg = k_data - rho;
+    const uint32_t lh = d - rho - 1;  // log2 h
+    const uint32_t n_items = nq << lh;
+    for (uint32_t it0 = threadIdx.x; it0 < n_items; it0 += 4 * SERIAL_THREADS) {
+      Ext g_lo[4], g_hi[4];
+      uint32_t tw[4], gi[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint32_t it = it0 + (uint32_t)u * S;
-      gi[u] = 0xFFFFFFFFu;
-      if (it < n_items) {
-        const uint32_t q = it >> lh, tt = it & (h - 1);
-        const uint32_t i = (qa.index[q] & (M - 1)) + tt * M;
-        const uint32_t e = ((1u << lg) - i) & ((1u << lg) - 1);
-        const uint32_t e24 = e << (24 - lg);
-        gi[u] = q * GS + tt;
-        tw[u] = mul(qf.tw_lo[e24 & 4095u], qf.tw_hi[e24 >> 12]);
-        g_lo[u] = qa.G[gi[u]];
-        g_hi[u] = qa.G[gi[u] + h];
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t it = it0 + (uint32_t)u * SERIAL_THREADS;
+        gi[u] = 0xFFFFFFFFu;
+        if (it < n_items) {
+          const uint32_t q = qb + QUERY_BLOCKS * (it >> lh), tt = it & (h - 1);
+          const uint32_t i = (qa.index[q] & (M - 1)) + tt * M;
+          const uint32_t e = ((1u << lg) - i) & ((1u << lg) - 1);
+          const uint32_t e24 = e << (24 - lg);
+          gi[u] = q * GS + tt;
+          tw[u] = mul(qf.tw_lo[e24 & 4095u], qf.tw_hi[e24 >> 12]);
+          g_lo[u] = qa.G[gi[u]];
+          g_hi[u] = qa.G[gi[u] + h];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (gi[u] == 0xFFFFFFFFu) continue;
+        const Ext even = emul_base(eadd(g_lo[u], g_hi[u]), HALF);
+        const Ext odd = emul_base(emul_base(esub(g_lo[u], g_hi[u]), HALF), tw[u]);
+        qa.G[gi[u]] = eadd(even, emul(r, esub(odd, even)));
       }
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (gi[u] == 0xFFFFFFFFu) continue;
-      const Ext even = emul_base(eadd(g_lo[u], g_hi[u]), HALF);
-      const Ext odd = emul_base(emul_base(esub(g_lo[u], g_hi[u]), HALF), tw[u]);
-      qa.G[gi[u]] = eadd(even, emul(r, esub(odd, even)));
-    }
   }
+  if (!partials) return;
+  __syncthreads();  // this workgroup's queries are at step rho + 1
+  query_terms_partial(qa, count, qb, d, k_data, rho + 1, partials, pidx, sh);
 }
 
-// fold_coeffs_kernel / fold_kernel with QUERY_BLOCKS more workgroups, the
-// last ones, for query_fold_block with the same r.
+// coeffs_kernel / fold_coeffs_kernel / fold_kernel with QUERY_BLOCKS more
+// workgroups, the last ones: the query terms of step 0 as partials g..g+3
+// (coeffs_q_kernel), or query_fold_block with the same r (its terms as
+// partials g..g+3 in fold_coeffs_q_kernel; none after the last fold).
+__global__ void __launch_bounds__(SC_THREADS) coeffs_q_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
+                                                          uint64_t n, Ext* __restrict__ partials, QueryFold qf) {
+  __shared__ Ext sh[SC_THREADS];
+  const uint32_t nf = gridDim.x - QUERY_BLOCKS;
+  if (blockIdx.x >= nf)
+    query_terms_partial(qf.qa, *qf.count_ptr, blockIdx.x - nf, qf.d, qf.k_data, 0, partials, blockIdx.x, sh);
+  else coeffs_body(f, w, n, partials, blockIdx.x, nf, sh);
+}
 __global__ void __launch_bounds__(SC_THREADS) fold_coeffs_q_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
                                                                uint64_t n, const Ext* __restrict__ r_ptr,
                                                                Ext* __restrict__ fo, Ext* __restrict__ wo,
@@ -865,15 +898,16 @@ __global__ void __launch_bounds__(SC_THREADS) fold_coeffs_q_kernel(const Ext* __
   __shared__ Ext sh[SC_THREADS];
   const Ext r = ld_ext(r_ptr);
   const uint32_t nf = gridDim.x - QUERY_BLOCKS;
-  if (blockIdx.x >= nf) query_fold_block(qf, r, blockIdx.x - nf);
+  if (blockIdx.x >= nf) query_fold_block(qf, r, blockIdx.x - nf, partials, blockIdx.x, sh);
   else fold_coeffs_body(f, w, n, r, fo, wo, partials, blockIdx.x, nf, sh);
 }
 __global__ void __launch_bounds__(SC_THREADS) fold_q_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
                                                         uint64_t n, const Ext* __restrict__ r_ptr,
                                                         Ext* __restrict__ fo, Ext* __restrict__ wo, QueryFold qf) {
+  __shared__ Ext sh[SC_THREADS];
   const Ext r = ld_ext(r_ptr);
   const uint32_t nf = gridDim.x - QUERY_BLOCKS;
-  if (blockIdx.x >= nf) query_fold_block(qf, r, blockIdx.x - nf);
+  if (blockIdx.x >= nf) query_fold_block(qf, r, blockIdx.x - nf, nullptr, 0, sh);
   else fold_body(f, w, n, r, fo, wo, blockIdx.x);
 }
 

@@ -416,20 +416,26 @@ __device__ __forceinline__ void pair_coeffs(const Ext& f0, const Ext& f1, const 
   c2 = eadd(c2, emul(esub(edbl(w1), w0), esub(edbl(f1), f0)));
 }
 
-__global__ void __launch_bounds__(SC_THREADS) coeffs_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
-                                                        uint64_t n, Ext* __restrict__ partials) {
-  __shared__ Ext sh[SC_THREADS];
+// Block blk of nblk (blockDim.x = SC_THREADS); sh: SC_THREADS entries.
+__device__ __forceinline__ void coeffs_body(const Ext* __restrict__ f, const Ext* __restrict__ w, uint64_t n,
+                                            Ext* __restrict__ partials, uint32_t blk, uint32_t nblk, Ext* sh) {
   Ext c0 = ezero(), c2 = ezero();
   const uint64_t npairs = n / 2;
-  for (uint64_t i = (uint64_t)blockIdx.x * SC_THREADS + threadIdx.x; i < npairs; i += (uint64_t)gridDim.x * SC_THREADS)
+  for (uint64_t i = (uint64_t)blk * SC_THREADS + threadIdx.x; i < npairs; i += (uint64_t)nblk * SC_THREADS)
     pair_coeffs(ld_ext(f + 2 * i), ld_ext(f + 2 * i + 1), ld_ext(w + 2 * i), ld_ext(w + 2 * i + 1), c0, c2);
   const Ext s0 = block_reduce_ext<SC_THREADS>(c0, sh);
   __syncthreads();
   const Ext s2 = block_reduce_ext<SC_THREADS>(c2, sh);
   if (threadIdx.x == 0) {
-    st_ext(partials + 2 * blockIdx.x, s0);
-    st_ext(partials + 2 * blockIdx.x + 1, s2);
+    st_ext(partials + 2 * blk, s0);
+    st_ext(partials + 2 * blk + 1, s2);
   }
+}
+
+__global__ void __launch_bounds__(SC_THREADS) coeffs_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
+                                                        uint64_t n, Ext* __restrict__ partials) {
+  __shared__ Ext sh[SC_THREADS];
+  coeffs_body(f, w, n, partials, blockIdx.x, gridDim.x, sh);
 }
 
 // Fold both arrays by r (a[i] = a[2i] + r (a[2i+1] - a[2i])) and emit the next

@@ -954,15 +954,22 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
     auto qfold = [&](int rho) {
       return QueryFold{P->qa, P->stir_count, (uint32_t)f, (uint32_t)rp.kfd, (uint32_t)rho, c->tw_lo(), c->tw_hi()};
     };
+    // (and its query terms of the next step as QUERY_BLOCKS more partials)
+    const unsigned qx = r == 0 ? QUERY_BLOCKS : 0u;
     for (int t = 0; t < f; ++t) {
       unsigned g;
       if (t == 0) {
         g = std::min<unsigned>(1024, blocks_for(n / 2));
-        hipLaunchKernelGGL(coeffs_kernel, dim3(g), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n, P->partials);
+        if (r == 0)
+          hipLaunchKernelGGL(coeffs_q_kernel, dim3(g + qx), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
+                             P->partials, qfold(0));
+        else
+          hipLaunchKernelGGL(coeffs_kernel, dim3(g), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
+                             P->partials);
       } else {
         g = std::min<unsigned>(1024, blocks_for(n / 4));
         if (r == 0)
-          hipLaunchKernelGGL(fold_coeffs_q_kernel, dim3(g + QUERY_BLOCKS), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
+          hipLaunchKernelGGL(fold_coeffs_q_kernel, dim3(g + qx), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
                              P->js->rs + (rs_idx - 1), P->ep[cur ^ 1], P->ew[cur ^ 1], P->partials, qfold(t - 1));
         else
           hipLaunchKernelGGL(fold_coeffs_kernel, dim3(g), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
@@ -971,8 +978,7 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
         cur ^= 1;
       }
       hipLaunchKernelGGL(round_finalize_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js,
-                         P->partials, g, rs_idx, r == 0 ? 1 : 0, P->qa, P->stir_count, (uint32_t)f,
-                         (uint32_t)rp.kfd, (uint32_t)t);
+                         P->partials, g + qx, rs_idx);
       ++rs_idx;
     }
     if (r == 0)
