@@ -109,12 +109,19 @@ extern "C" int whir_ctx_create(int device, uint64_t seed, whir_ctx** out) {
   return WHIR_OK;
 }
 
+// Release calls in a destroy path: every one is made; the first failure is
+// reported (WHIR_EDEVICE) after the rest.
+static void release_step(int& rc, hipError_t e) {
+  if (e != hipSuccess && rc == WHIR_OK) rc = WHIR_EDEVICE;
+}
+
 extern "C" int whir_ctx_destroy(whir_ctx* c) {
   if (!c) return WHIR_OK;
-  hipFree(c->tw);
-  if (c->stream) hipStreamDestroy(c->stream);
+  int rc = WHIR_OK;
+  release_step(rc, hipFree(c->tw));
+  if (c->stream) release_step(rc, hipStreamDestroy(c->stream));
   delete c;
-  return WHIR_OK;
+  return rc;
 }
 
 extern "C" int whir_ctx_set_ntt_shape(whir_ctx* c, int first_pass_log, int log_vec) {
@@ -726,16 +733,17 @@ extern "C" int64_t whir_prover_stack_offset(const whir_prover* P, int32_t id) {
 
 extern "C" int whir_prover_destroy(whir_prover* P) {
   if (!P) return WHIR_OK;
-  if (P->arena) hipFree(P->arena);
-  if (P->host_log) hipHostFree(P->host_log);
+  int rc = WHIR_OK;
+  if (P->arena) release_step(rc, hipFree(P->arena));
+  if (P->host_log) release_step(rc, hipHostFree(P->host_log));
   for (int i = 0; i < WHIR_TIMER_COUNT; ++i)
     for (int j = 0; j < 2; ++j)
-      if (P->ev[i][j]) (void)hipEventDestroy(P->ev[i][j]);
-  if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
-  if (P->graph) (void)hipGraphDestroy(P->graph);
-  if (P->done) (void)hipEventDestroy(P->done);
+      if (P->ev[i][j]) release_step(rc, hipEventDestroy(P->ev[i][j]));
+  if (P->gexec) release_step(rc, hipGraphExecDestroy(P->gexec));
+  if (P->graph) release_step(rc, hipGraphDestroy(P->graph));
+  if (P->done) release_step(rc, hipEventDestroy(P->done));
   delete P;
-  return WHIR_OK;
+  return rc;
 }
 
 extern "C" int whir_prover_set_timers(whir_prover* P, int enable) {
