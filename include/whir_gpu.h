@@ -58,6 +58,7 @@ typedef struct whir_prover whir_prover; /* device workspace for one proof shape 
  * Poseidon2KoalaBear<16|24>::new_from_rng_128(StdRng::seed_from_u64(seed))
  * (crates/whir/src/test.rs:195-196; the reference uses seed 1000). */
 int whir_ctx_create(int device, uint64_t poseidon_seed, whir_ctx** out);
+/* Destroy calls release everything; a failed HIP release returns WHIR_EDEVICE. */
 int whir_ctx_destroy(whir_ctx* ctx);
 
 /* NTT tile shape of this context (no counterpart in the reference, whose DFT
