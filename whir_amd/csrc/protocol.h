@@ -267,7 +267,8 @@ __global__ void __launch_bounds__(SERIAL_THREADS) claim_end_kernel(DevTx* t, uin
 // ProverLayout::new_prover (prover.rs:279-353) with Svo::calculate_accumulators
 // (svo.rs:224-278) and the expression scales (prover.rs:184-208):
 //   draw alpha; f SVO rounds (write c0, c2; draw r); eq(rs) into js->eq_rs;
-//   eq1 tables of every claim scaled by alpha^rank * eq(svo_c, rs).
+//   claim c's scale alpha^rank * eq(svo_c, rs) into claim_scale[c] (the
+//   expression build applies it to the group's eq1 table).
 // The accumulators are linear in the claims' partial evaluations, so a group's
 // claims (one point) are combined with their alpha powers first and the
 // accumulators are taken once per group, as the reference does per point
@@ -278,16 +279,14 @@ __global__ void __launch_bounds__(SERIAL_THREADS) claim_end_kernel(DevTx* t, uin
 __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t* log, JobScalars* js,
                                                              const ClaimSlot* slots, const ClaimDesc* desc,
                                                              uint32_t n_claims, const uint32_t* group_start,
-                                                             uint32_t n_groups, uint32_t f, Ext* pool, Ext* pe_buf,
-                                                             Ext* gpe_buf, const uint32_t* eq1_pref) {
+                                                             uint32_t n_groups, uint32_t f, Ext* pe_buf,
+                                                             Ext* gpe_buf, Ext* claim_scale) {
   // partial evals pe[c][2^l - 2 + a] (l = 1..f), accumulators tot[ri][side][u]
   constexpr int PE_PER = (2 << MAX_F) - 2;
   constexpr int ACC_PER = 243 - 1;  // sum_{ri<5} 2 * 3^ri
   __shared__ Ext s_tot[ACC_PER];
   __shared__ Ext s_part[SERIAL_THREADS];
   __shared__ Ext s_apow[MAX_CLAIMS];  // alpha^rank, then the expression scales
-  __shared__ uint32_t s_pref[MAX_CLAIMS + 1];
-  __shared__ uint32_t s_e1[MAX_CLAIMS], s_le[MAX_CLAIMS];  // eq1_off, lead_eq1
   __shared__ Ext s_rs[MAX_F];
   __shared__ Ext s_lag[MAX_F][3];
   __shared__ Ext s_alpha;
@@ -330,11 +329,6 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t*
       if (m >= (1u << l)) v = eadd(v, o);
     }
     if (valid && b == 0) pe_buf[(uint64_t)c * PE_PER + (1u << l) - 2 + a] = v;
-  }
-  for (uint32_t c = tid; c <= n_claims; c += SERIAL_THREADS) s_pref[c] = eq1_pref[c];
-  for (uint32_t c = tid; c < n_claims; c += SERIAL_THREADS) {
-    s_e1[c] = (uint32_t)desc[c].eq1_off;
-    s_le[c] = (uint32_t)desc[c].lead_eq1;
   }
 
   // (2) draw alpha; alpha^rank per claim
@@ -473,47 +467,7 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t*
       // (y x) * 2 - x - y + 1
       e = emul(e, eadd(esub(esub(edbl(emul(y, x)), x), y), eone()));
     }
-    s_apow[c] = e;
-  }
-  __syncthreads();
-  // (7) scaled eq1 tables: the group's later claims copy the first claim's
-  // (still unscaled) table, then every first claim scales its own in place.
-  // Four loads in flight per thread; claim offsets come from LDS.
-  const uint32_t total = s_pref[n_claims];
-  int copies = 0;
-  for (uint32_t c = tid; c < n_claims; c += SERIAL_THREADS) copies |= s_e1[c] != s_le[c];
-  copies = __syncthreads_or(copies);
-  for (int phase = copies ? 0 : 1; phase < 2; ++phase) {
-    for (uint32_t it0 = tid; it0 < total; it0 += 4 * SERIAL_THREADS) {
-      Ext v[4];
-      uint32_t dst[4], cl[4];
-      bool use[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const uint32_t it = it0 + (uint32_t)u * SERIAL_THREADS;
-        use[u] = false;
-        dst[u] = 0;
-        cl[u] = 0;
-        if (it < total) {
-          uint32_t lo = 0, hi = n_claims;  // last c with s_pref[c] <= it
-          while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) / 2;
-            if (s_pref[mid] <= it) lo = mid;
-            else hi = mid;
-          }
-          const uint32_t x = it - s_pref[lo];
-          const bool first = s_e1[lo] == s_le[lo];
-          use[u] = first == (phase == 1);
-          dst[u] = s_e1[lo] + x;
-          cl[u] = lo;
-          if (use[u]) v[u] = ld_ext(pool + s_le[lo] + x);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (use[u]) st_ext(pool + dst[u], emul(v[u], s_apow[cl[u]]));
-    }
-    __syncthreads();
+    claim_scale[c] = e;  // expr_build_kernel applies it to the group's eq1 table
   }
 }
 

@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(SC_THREADS) expr_build_kernel(const uint32_t* 
                                                             const ClaimDesc* __restrict__ desc, uint32_t full0,
                                                             uint32_t n_full, uint32_t m0, uint64_t n,
                                                             Ext* __restrict__ epoly, Ext* __restrict__ ew,
-                                                            StackBlocks sb) {
+                                                            StackBlocks sb, const Ext* __restrict__ cscale) {
   constexpr int W = 1 << F_LOG;
   __shared__ Ext s_eq[W];
   if (threadIdx.x < W) s_eq[threadIdx.x] = eq_rs[threadIdx.x];
@@ -280,10 +280,10 @@ __global__ void __launch_bounds__(SC_THREADS) expr_build_kernel(const uint32_t* 
     // a wavefront's 64 consecutive j share hi: eq1 entries as scalar loads
     const uint64_t hu = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)hi);
     for (uint32_t c = full0; c < full0 + n_full; ++c)
-      w = eadd(w, emul(ld_ext(pool + desc[c].eq0_off + lo), ld_ext(pool + desc[c].eq1_off + hu)));
+      w = eadd(w, emul(ld_ext(pool + desc[c].eq0_off + lo), emul(ld_ext(pool + desc[c].lead_eq1 + hu), cscale[c])));
   } else {
     for (uint32_t c = full0; c < full0 + n_full; ++c)
-      w = eadd(w, emul(ld_ext(pool + desc[c].eq0_off + lo), ld_ext(pool + desc[c].eq1_off + hi)));
+      w = eadd(w, emul(ld_ext(pool + desc[c].eq0_off + lo), emul(ld_ext(pool + desc[c].lead_eq1 + hi), cscale[c])));
   }
   if (sb.n_class) {
     uint32_t cl = 0;
@@ -293,8 +293,10 @@ __global__ void __launch_bounds__(SC_THREADS) expr_build_kernel(const uint32_t* 
       const uint32_t seg = sb.seg0[cl] + (uint32_t)(rel >> sb.log[cl]);
       const uint64_t x = rel & ((1ull << sb.log[cl]) - 1);
       for (uint32_t q = sb.seg_start[seg]; q < sb.seg_start[seg + 1]; ++q) {
-        const ClaimDesc& d = desc[sb.seg_claims[q]];
-        w = eadd(w, emul(ld_ext(pool + d.eq0_off + (x & ((1ull << d.m0) - 1))), ld_ext(pool + d.eq1_off + (x >> d.m0))));
+        const uint32_t c = sb.seg_claims[q];
+        const ClaimDesc& d = desc[c];
+        w = eadd(w, emul(ld_ext(pool + d.eq0_off + (x & ((1ull << d.m0) - 1))),
+                         emul(ld_ext(pool + d.lead_eq1 + (x >> d.m0)), cscale[c])));
       }
     }
   }

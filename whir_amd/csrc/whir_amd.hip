@@ -432,7 +432,7 @@ struct whir_prover {
   Ext* ood_pool;
   ClaimDesc* desc;         // per SVO-phase claim (device copy of hdesc)
   uint32_t* group_start;   // first claim of each point group, n_groups + 1
-  uint32_t* eq1_pref;      // prefix sums of the claims' eq1 table sizes
+  Ext* claim_scale;        // per claim: alpha^rank eq(svo_c, rs), applied to its group's eq1 table
   uint32_t* seg_start;     // stacked polys in offset order: their claims (CSR)
   uint32_t* seg_claims;
   Ext* pe_buf;             // svo_kernel scratch
@@ -453,7 +453,7 @@ struct whir_prover {
   };
   std::vector<Group> groups;
   std::vector<ClaimDesc> hdesc;
-  std::vector<uint32_t> h_group_start, h_eq1_pref, h_seg_start, h_seg_claims;
+  std::vector<uint32_t> h_group_start, h_seg_start, h_seg_claims;
   uint32_t full0 = 0, n_full = 0;  // claims spanning the whole stacked vector
   StackBlocks sb{};
   uint32_t* host_log = nullptr;  // pinned: the log, then LOG_TAIL status words
@@ -498,7 +498,6 @@ static int claim_table(whir_prover* P, int n_ood0, uint64_t& pool_ext, uint64_t&
   for (int v = 0; v < n_ood0; ++v) P->groups.push_back({(uint32_t)(n_user + v), 1u, (uint32_t)k});
   P->hdesc.assign((size_t)nc, ClaimDesc{});
   P->h_group_start.clear();
-  P->h_eq1_pref.assign((size_t)nc + 1, 0);
   pool_ext = 0;
   part_ext = 0;
   for (const auto& G : P->groups) {
@@ -522,7 +521,6 @@ static int claim_table(whir_prover* P, int n_ood0, uint64_t& pool_ext, uint64_t&
       d.lead_eq1 = P->hdesc[G.c0].eq1_off;
       d.poly_off = (int)c < n_user ? S.offset[S.claim_poly[c]] : 0;
       d.rank = (int)c < n_user ? S.rank[c] : c;
-      P->h_eq1_pref[c + 1] = P->h_eq1_pref[c] + (1u << m1);
     }
   }
   P->h_group_start.push_back((uint32_t)nc);
@@ -631,7 +629,7 @@ static int prover_layout(whir_prover* P, const whir_params* prm, const whir_stac
       {(void**)&P->svo_pool, pool_ext * 16},
       {(void**)&P->desc, sizeof(ClaimDesc) * n_svo},
       {(void**)&P->group_start, P->h_group_start.size() * 4},
-      {(void**)&P->eq1_pref, P->h_eq1_pref.size() * 4},
+      {(void**)&P->claim_scale, sizeof(Ext) * n_svo},
       {(void**)&P->seg_start, P->h_seg_start.size() * 4},
       {(void**)&P->seg_claims, P->h_seg_claims.size() * 4},
       {(void**)&P->pe_buf, PE_PER * n_svo * 16},
@@ -701,7 +699,6 @@ static int prover_create(whir_ctx* c, const whir_params* prm, const whir_stack* 
   } up[] = {
       {P->desc, P->hdesc.data(), P->hdesc.size() * sizeof(ClaimDesc)},
       {P->group_start, P->h_group_start.data(), P->h_group_start.size() * 4},
-      {P->eq1_pref, P->h_eq1_pref.data(), P->h_eq1_pref.size() * 4},
       {P->seg_start, P->h_seg_start.data(), P->h_seg_start.size() * 4},
       {P->seg_claims, P->h_seg_claims.data(), P->h_seg_claims.size() * 4},
   };
@@ -788,7 +785,8 @@ template <int FL>
 static void launch_expr(unsigned grid, hipStream_t s, const uint32_t* poly, const Ext* eq_rs, const whir_prover* P,
                         uint32_t m0, uint64_t n, Ext* ep, Ext* ew) {
   hipLaunchKernelGGL(expr_build_kernel<FL>, dim3(grid), dim3(SC_THREADS), 0, s, poly, eq_rs, (const Ext*)P->svo_pool,
-                     (const ClaimDesc*)P->desc, P->full0, P->n_full, m0, n, ep, ew, P->sb);
+                     (const ClaimDesc*)P->desc, P->full0, P->n_full, m0, n, ep, ew, P->sb,
+                     (const Ext*)P->claim_scale);
 }
 
 #define FSWITCH(f, CALL)          \
@@ -856,8 +854,8 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
   }
   // ---- SVO rounds + expression (prover.rs:279-353, 184-229) ----
   hipLaunchKernelGGL(svo_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots, P->desc,
-                     (uint32_t)P->n_svo_claims, P->group_start, (uint32_t)n_groups, (uint32_t)f, P->svo_pool,
-                     P->pe_buf, P->gpe_buf, P->eq1_pref);
+                     (uint32_t)P->n_svo_claims, P->group_start, (uint32_t)n_groups, (uint32_t)f, P->pe_buf,
+                     P->gpe_buf, P->claim_scale);
   uint64_t n = 1ull << (k - f);
   FSWITCH(f, launch_expr<FL>(blocks_for(n), s, poly, P->js->eq_rs, P, m0, n, P->ep[0], P->ew[0]));
   int cur = 0;
