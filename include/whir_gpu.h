@@ -164,6 +164,49 @@ int whir_prover_enqueue(whir_prover* prover, const uint32_t* d_poly, void* strea
 int whir_prover_result(whir_prover* prover, uint8_t* proof, size_t cap, size_t* len, uint32_t* checkpoint,
                        uint32_t* root8);
 
+/* ---- Whir::commit / ProverStack::eval / Whir::open with the caller's transcript ----
+ * The reference threads one transcript through three calls
+ * (crates/whir/src/whir.rs:127-138 commit, crates/whir/src/sumcheck/prover.rs:64-90
+ * ProverStack::eval, whir.rs:168-180 open). These entry points do the same on
+ * the device: the transcript is a DuplexChallenger<KoalaBear, Poseidon2<16>, 16, 8>
+ * state handed in before the commitment and handed back after the opening.
+ *
+ * whir_challenger mirrors Plonky3's DuplexChallenger fields (sponge_state,
+ * input_buffer, output_buffer; Montgomery words). A state is accepted when a
+ * challenger can reach it: input_len <= 7, output_len <= 8, not both non-zero
+ * (observe clears the output), every word canonical (< p), and
+ * output_buffer[0..output_len) == sponge_state[0..output_len) (the output
+ * buffer is the state's rate prefix after a duplex; sample pops from its end).
+ * A zeroed struct is Challenger::new. */
+typedef struct {
+  uint32_t sponge_state[16];
+  uint32_t input_buffer[8];
+  uint32_t output_buffer[8];
+  uint32_t input_len;
+  uint32_t output_len;
+} whir_challenger;
+
+/* Whir::commit (whir.rs:127-138 -> commit_base, whir.rs:19-36): RS encode and
+ * Merkle-commit d_poly (2^k words; for a stacked prover the stacked vector, see
+ * whir_prover_stack_offset), then transcript.write(root). tx_in: the caller's
+ * transcript (NULL: a fresh one). d_poly stays in use until whir_prover_open
+ * has finished. Enqueued on `stream`, no host synchronisation. */
+int whir_prover_commit(whir_prover* prover, const uint32_t* d_poly, const whir_challenger* tx_in, void* stream);
+/* ProverStack::eval (prover.rs:64-90) for the next evaluation claim of the
+ * prover's claim list: computes the value at the claim's point and writes it.
+ * point (host, k_poly extension elements = 4 k_poly words): any point; NULL
+ * draws point = expand(k_poly, transcript.draw()) as the reference's tests do
+ * (crates/whir/src/test.rs:225-226). A claim that shares the previous claim's
+ * point (whir_stack claim_fresh = 0) takes NULL. */
+int whir_prover_eval(whir_prover* prover, const uint32_t* point, void* stream);
+/* Whir::open (whir.rs:168-359) after every claim has been evaluated. Then
+ * whir_prover_result returns the bytes written since the commitment's transcript
+ * state (root, claim values, opening), root8 the commitment root, and
+ * `checkpoint` the value the caller's next transcript.draw() returns. */
+int whir_prover_open(whir_prover* prover, void* stream);
+/* The transcript state after the opening (waits for it). */
+int whir_prover_challenger(whir_prover* prover, whir_challenger* out);
+
 /* ---- multi-polynomial stacking ----
  * Several multilinear polys committed and opened as one (ProverStack::register /
  * eval / layout, crates/whir/src/sumcheck/prover.rs:31-124; VerifierStack /

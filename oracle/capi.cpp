@@ -231,7 +231,7 @@ static std::vector<std::vector<F>> polys_from(int n, const int* k, const uint32_
 }
 static std::vector<StackClaim> claims_from(int n, const int* poly, const uint8_t* fresh) {
   std::vector<StackClaim> out((size_t)n);
-  for (int i = 0; i < n; ++i) out[i] = StackClaim{poly[i], fresh[i] ? 1 : 0};
+  for (int i = 0; i < n; ++i) out[i] = StackClaim{poly[i], fresh[i] > 2 ? 1 : (int)fresh[i]};
   return out;
 }
 
@@ -299,6 +299,72 @@ int oracle_whir_verify_stacked(int k, int folding, int rate, int ir, int soundne
                                 claims_from(n_claims, claim_poly, claim_new), cp);
     if (checkpoint) *checkpoint = cp.v;
     return s;
+  } catch (const std::exception&) {
+    return -2;
+  }
+}
+
+// ---- a proof inside a caller's transcript session (whir.h Session) ----
+// claim_new: 0 shared, 1 drawn, 2 given (the next k_poly extension elements of
+// `given`); prefix: words written before the commitment. challenger_out (40
+// words): sponge state 16, input buffer 8, output buffer 8, input length,
+// output length, 6 zero words — the state open hands back.
+static Session session_from(int n_prefix, const uint32_t* prefix, int n_claims, const int* claim_poly,
+                            const uint8_t* claim_new, const int* k_polys, const uint32_t* given) {
+  Session S;
+  for (int i = 0; i < n_prefix; ++i) S.prefix.push_back(F{prefix[i]});
+  size_t off = 0;
+  for (int c = 0; c < n_claims; ++c)
+    if (claim_new[c] == 2) {
+      const int k = k_polys[claim_poly[c]];
+      S.given.push_back(to_point(given + off, k));
+      off += 4 * (size_t)k;
+    }
+  return S;
+}
+
+int oracle_whir_prove_session(int k, int folding, int rate, int ir, int soundness, int security, int n_polys,
+                              const int* k_polys, const uint32_t* polys, int n_claims, const int* claim_poly,
+                              const uint8_t* claim_new, int n_prefix, const uint32_t* prefix, const uint32_t* given,
+                              uint8_t* proof, size_t cap, size_t* len, uint32_t* checkpoint, uint32_t* root,
+                              uint32_t* challenger_out) {
+  try {
+    Params p{k, folding, rate, ir, soundness, security};
+    const Session S = session_from(n_prefix, prefix, n_claims, claim_poly, claim_new, k_polys, given);
+    Challenger after(nullptr);
+    ProveResult r = whir_prove_stacked(p, polys_from(n_polys, k_polys, polys),
+                                       claims_from(n_claims, claim_poly, claim_new), &S, &after);
+    *len = r.proof.size();
+    if (checkpoint) *checkpoint = r.checkpoint.v;
+    if (root) memcpy(root, r.root.d, 32);
+    if (challenger_out) {
+      memset(challenger_out, 0, 40 * 4);
+      for (int i = 0; i < 16; ++i) challenger_out[i] = after.state[i].v;
+      for (size_t i = 0; i < after.in.size() && i < 8; ++i) challenger_out[16 + i] = after.in[i].v;
+      for (size_t i = 0; i < after.out.size() && i < 8; ++i) challenger_out[24 + i] = after.out[i].v;
+      challenger_out[32] = (uint32_t)after.in.size();
+      challenger_out[33] = (uint32_t)after.out.size();
+    }
+    if (r.proof.size() > cap) return -1;
+    memcpy(proof, r.proof.data(), r.proof.size());
+    return 0;
+  } catch (const std::exception&) {
+    return -2;
+  }
+}
+
+int oracle_whir_verify_session(int k, int folding, int rate, int ir, int soundness, int security, int n_polys,
+                               const int* k_polys, int n_claims, const int* claim_poly, const uint8_t* claim_new,
+                               int n_prefix, const uint32_t* prefix, const uint32_t* given, const uint8_t* proof,
+                               size_t len, uint32_t* checkpoint) {
+  try {
+    Params p{k, folding, rate, ir, soundness, security};
+    const Session S = session_from(n_prefix, prefix, n_claims, claim_poly, claim_new, k_polys, given);
+    F cp{0};
+    int st = whir_verify_stacked(p, proof, len, k_vec(n_polys, k_polys), claims_from(n_claims, claim_poly, claim_new),
+                                 cp, &S);
+    if (checkpoint) *checkpoint = cp.v;
+    return st;
   } catch (const std::exception&) {
     return -2;
   }

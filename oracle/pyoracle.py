@@ -87,6 +87,11 @@ def lib():
                                                 ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), u32p, u32p])
         L.oracle_whir_verify_stacked.argtypes = ([ctypes.c_int] * 7 + [i32p, ctypes.c_int, i32p, u8p, u8p,
                                                  ctypes.c_size_t, u32p])
+        L.oracle_whir_prove_session.argtypes = ([ctypes.c_int] * 7 + [i32p, u32p, ctypes.c_int, i32p, u8p,
+                                                ctypes.c_int, u32p, u32p, u8p, ctypes.c_size_t,
+                                                ctypes.POINTER(ctypes.c_size_t), u32p, u32p, u32p])
+        L.oracle_whir_verify_session.argtypes = ([ctypes.c_int] * 7 + [i32p, ctypes.c_int, i32p, u8p, ctypes.c_int,
+                                                 u32p, u32p, u8p, ctypes.c_size_t, u32p])
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_get_threads.restype = ctypes.c_int
         _lib = L
@@ -435,4 +440,62 @@ def whir_verify_stacked(k, folding, rate, ir, soundness, security, k_polys, clai
     i32p = ctypes.POINTER(ctypes.c_int)
     rc = lib().oracle_whir_verify_stacked(k, folding, rate, ir, soundness, security, len(kp), _p(kp, i32p),
                                           len(claims), _p(cp, i32p), _p(cn, u8p), _p(b, u8p), b.size, _p(chk))
+    return rc, int(chk[0])
+
+
+# ---------------- a proof inside a caller's transcript session ----------------
+GIVEN = 2  # claim kind: a point given by the caller (0 shared, 1 drawn)
+
+
+def _session_claims(claims):
+    """claims: (poly id, kind) with kind 0 shared / 1 drawn / 2 given, or a bool
+    for drawn/shared."""
+    cp = _i32([c[0] for c in claims])
+    cn = np.ascontiguousarray(np.asarray([int(c[1]) for c in claims], dtype=np.uint8))
+    return cp, cn
+
+
+def whir_prove_session(k, folding, rate, ir, soundness, security, polys, claims, prefix=(), given=()):
+    """The prover inside a caller's transcript: `prefix` words written before the
+    commitment, claims of kind 2 take the next point of `given` (k_poly x 4
+    words each). Returns (proof bytes including the prefix's, checkpoint, root,
+    challenger state after open as a dict of the whir_challenger fields)."""
+    kp = _i32([int(np.log2(len(p))) for p in polys])
+    data = _u32(np.concatenate([np.asarray(p, np.uint32) for p in polys]))
+    cp, cn = _session_claims(claims)
+    pre = _u32(np.asarray(list(prefix), np.uint32)) if len(prefix) else np.zeros(1, np.uint32)
+    gv = _u32(np.concatenate([np.asarray(g, np.uint32).reshape(-1) for g in given])) if len(given) else \
+        np.zeros(1, np.uint32)
+    i32p = ctypes.POINTER(ctypes.c_int)
+    cap = 1 << 16
+    while True:
+        buf = np.zeros(cap, np.uint8)
+        ln = ctypes.c_size_t()
+        chk = np.zeros(1, np.uint32)
+        root = np.zeros(8, np.uint32)
+        st = np.zeros(40, np.uint32)
+        rc = lib().oracle_whir_prove_session(k, folding, rate, ir, soundness, security, len(kp), _p(kp, i32p),
+                                             _p(data), len(claims), _p(cp, i32p), _p(cn, u8p), len(prefix), _p(pre),
+                                             _p(gv), _p(buf, u8p), cap, ctypes.byref(ln), _p(chk), _p(root), _p(st))
+        if rc == -2:
+            raise ValueError("bad session")
+        if rc == 0:
+            state = {"sponge_state": st[:16].copy(), "input_buffer": st[16:24].copy(),
+                     "output_buffer": st[24:32].copy(), "input_len": int(st[32]), "output_len": int(st[33])}
+            return bytes(buf[: ln.value]), int(chk[0]), root, state
+        cap = ln.value
+
+
+def whir_verify_session(k, folding, rate, ir, soundness, security, k_polys, claims, proof, prefix=(), given=()):
+    kp = _i32(k_polys)
+    cp, cn = _session_claims(claims)
+    pre = _u32(np.asarray(list(prefix), np.uint32)) if len(prefix) else np.zeros(1, np.uint32)
+    gv = _u32(np.concatenate([np.asarray(g, np.uint32).reshape(-1) for g in given])) if len(given) else \
+        np.zeros(1, np.uint32)
+    b = np.frombuffer(bytes(proof), dtype=np.uint8).copy()
+    chk = np.zeros(1, np.uint32)
+    i32p = ctypes.POINTER(ctypes.c_int)
+    rc = lib().oracle_whir_verify_session(k, folding, rate, ir, soundness, security, len(kp), _p(kp, i32p),
+                                          len(claims), _p(cp, i32p), _p(cn, u8p), len(prefix), _p(pre), _p(gv),
+                                          _p(b, u8p), b.size, _p(chk))
     return rc, int(chk[0])

@@ -814,9 +814,11 @@ ProveResult whir_prove(const Params& p, const std::vector<F>& poly, int n_points
 }
 
 ProveResult whir_prove_stacked(const Params& p, const std::vector<std::vector<F>>& polys,
-                               const std::vector<StackClaim>& claims) {
+                               const std::vector<StackClaim>& claims, const Session* session, Challenger* after) {
   const Hashers& H = Hashers::get();
   Writer w(&H.p16);
+  if (session)
+    for (F x : session->prefix) w.write_f(x);
   const int f = p.folding;
   std::vector<int> kp;
   for (auto& q : polys) kp.push_back(log2_exact(q.size()));
@@ -834,8 +836,14 @@ ProveResult whir_prove_stacked(const Params& p, const std::vector<std::vector<F>
   // test.rs:223-227 and sumcheck/test.rs:113-121) ----
   std::vector<SvoClaim> cl;
   Point pt;
+  size_t gi = 0;
   for (const StackClaim& c : claims) {
-    if (c.new_point) pt = expand(kp[c.poly], w.draw_e());
+    if (c.new_point == 1) pt = expand(kp[c.poly], w.draw_e());
+    if (c.new_point == 2) {
+      if (!session || gi >= session->given.size() || (int)session->given[gi].size() != kp[c.poly])
+        throw std::runtime_error("missing or mis-sized given point");
+      pt = session->given[gi++];
+    }
     cl.push_back(svo_eval(f, pt, polys[c.poly]));
     w.write_e(cl.back().eval);
   }
@@ -984,6 +992,7 @@ ProveResult whir_prove_stacked(const Params& p, const std::vector<std::vector<F>
     for (size_t index : idx) merkle_query(w, have_round_tree ? round_tree : base, index);
   }
   ProveResult res;
+  if (after) *after = w.ch;
   res.checkpoint = w.draw_f();  // test.rs:233
   res.root = base.root();
   res.proof = std::move(w.bytes);
@@ -1034,10 +1043,16 @@ int whir_verify(const Params& p, const uint8_t* proof, size_t len, int n_points,
 }
 
 int whir_verify_stacked(const Params& p, const uint8_t* proof, size_t len, const std::vector<int>& k_polys,
-                        const std::vector<StackClaim>& claims, F& checkpoint) {
+                        const std::vector<StackClaim>& claims, F& checkpoint, const Session* session) {
   // whir.rs:361-522 with VerifierStack::read_eval / VerifierLayout (verifier.rs:100-212).
   const Hashers& H = Hashers::get();
   Reader r(&H.p16, proof, len);
+  if (session)
+    for (F x : session->prefix) {
+      F got;
+      TRY(r.read_f(got));
+      if (got.v != x.v) return ERR_VERIFY;
+    }
   const int k = p.k, f = p.folding;
   const StackLayout L = stack_layout(k_polys, f);
   if (L.k != k) return ERR_VERIFY;
@@ -1048,9 +1063,15 @@ int whir_verify_stacked(const Params& p, const uint8_t* proof, size_t len, const
   std::vector<Point> pts(claims.size());
   std::vector<E> evals(claims.size());
   Point pt;
+  size_t gi = 0;
   for (size_t c = 0; c < claims.size(); ++c) {
     const int id = claims[c].poly;
-    if (claims[c].new_point) pt = expand(k_polys[id], r.draw_e());
+    if (claims[c].new_point == 1) pt = expand(k_polys[id], r.draw_e());
+    if (claims[c].new_point == 2) {
+      if (!session || gi >= session->given.size() || (int)session->given[gi].size() != k_polys[id])
+        return ERR_VERIFY;
+      pt = session->given[gi++];
+    }
     E ev;
     TRY(r.read_e(ev));
     // lift: point ++ selector point (mod.rs:55-57), stored in layout order (verifier.rs:204-212)

@@ -87,7 +87,7 @@ int whir_verify(const Params& p, const uint8_t* proof, size_t len, int n_points,
 // polys have the same k). The list is in transcript order.
 struct StackClaim {
   int poly;
-  int new_point;
+  int new_point;  // 0: the previous claim's point, 1: expand(k_poly, draw()), 2: the next given point
 };
 struct StackLayout {
   int k = 0;                      // log2_ceil(sum of poly sizes)
@@ -101,10 +101,21 @@ StackLayout stack_layout(const std::vector<int>& k_polys, int k_svo);
 std::vector<F> stack_polys(const StackLayout& L, const std::vector<std::vector<F>>& polys);
 // alpha exponent of each claim: its position in claims_in_layout_order (prover.rs:155-174)
 std::vector<size_t> stack_claim_ranks(const StackLayout& L, const std::vector<StackClaim>& claims);
+// A session around the proof, as a caller of Whir::commit / ProverStack::eval /
+// Whir::open drives it (whir.rs:127-180, prover.rs:64-90): `prefix` words are
+// written to the transcript before the commitment (their bytes start the
+// proof), and a claim with new_point == 2 takes the next of `given` (any
+// point of k_poly coordinates) instead of expand(k_poly, draw()). `after`
+// receives the challenger state open hands back (before the checkpoint draw).
+struct Session {
+  std::vector<F> prefix;
+  std::vector<Point> given;
+};
 ProveResult whir_prove_stacked(const Params& p, const std::vector<std::vector<F>>& polys,
-                               const std::vector<StackClaim>& claims);
+                               const std::vector<StackClaim>& claims, const Session* session = nullptr,
+                               Challenger* after = nullptr);
 int whir_verify_stacked(const Params& p, const uint8_t* proof, size_t len, const std::vector<int>& k_polys,
-                        const std::vector<StackClaim>& claims, F& checkpoint);
+                        const std::vector<StackClaim>& claims, F& checkpoint, const Session* session = nullptr);
 // write_virtual_eval's value and SVO accumulators computed per stacked poly
 // (prover.rs:241-256) against one claim on the stacked vector (its debug check,
 // prover.rs:258-266): returns 1 when both agree.

@@ -68,3 +68,49 @@ def eq_naive(z, x):
 
 def ext_list_from_m(arr):
     return [e_from_m(row) for row in arr.reshape(-1, 4)]
+
+
+class DuplexPy:
+    """DuplexChallenger<F, P16, 16, 8> restated over the oracle's permutation
+    (observe clears the output and absorbs at 8 inputs; sample duplexes when
+    input is pending or the output is empty, then pops from the end)."""
+
+    def __init__(self, oracle, state=None):
+        import numpy as np
+
+        self.o = oracle
+        self.state = np.zeros(16, np.uint32)
+        self.inp, self.out = [], []
+        if state is not None:  # a whir_challenger dict
+            self.state = np.array(state["sponge_state"], np.uint32)
+            self.inp = [int(x) for x in state["input_buffer"][: state["input_len"]]]
+            self.out = [int(x) for x in state["output_buffer"][: state["output_len"]]]
+
+    def duplex(self):
+        for i, v in enumerate(self.inp):
+            self.state[i] = v
+        self.inp = []
+        self.state = self.o.permute(16, self.state)
+        self.out = list(self.state[:8])
+
+    def observe(self, v):
+        self.out = []
+        self.inp.append(v)
+        if len(self.inp) == 8:
+            self.duplex()
+
+    def sample(self):
+        if self.inp or not self.out:
+            self.duplex()
+        return self.out.pop()
+
+    def as_state(self):
+        """The whir_challenger fields of this state."""
+        import numpy as np
+
+        ib = np.zeros(8, np.uint32)
+        ob = np.zeros(8, np.uint32)
+        ib[: len(self.inp)] = self.inp
+        ob[: len(self.out)] = self.out
+        return {"sponge_state": np.array(self.state, np.uint32), "input_buffer": ib, "output_buffer": ob,
+                "input_len": len(self.inp), "output_len": len(self.out)}

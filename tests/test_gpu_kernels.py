@@ -52,6 +52,7 @@ DFT_CASES = [
     (1 << 15, 17, 8),
     (1 << 16, 17, 16),
     (1 << 13, 16, 32),
+    (1 << 20, 21, 16),  # the base encode of 2^24 coefficients (default plan: register pass + two LDS passes)
 ]
 
 

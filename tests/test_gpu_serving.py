@@ -114,3 +114,43 @@ def test_server_errors(gpu_ctx, oracle):
         srv.take(jid, 1 << 20)
     assert e.value.code == 5  # WHIR_EOOM
     srv.close()
+
+
+def test_server_poll_reports_every_job_with_max_one(gpu_ctx, oracle):
+    """poll(max=1): every finished job is reported exactly once, by later polls
+    if need be, and its proof equals the job's proof alone."""
+    import ctypes
+
+    import whir_amd
+    from whir_amd._lib import lib
+
+    cfgs = [(10, 3, 1, 2, 1, 32, 1), (11, 3, 1, 2, 1, 32, 2), (12, 4, 1, 3, 1, 123, 1), (10, 3, 1, 2, 1, 32, 1),
+            (9, 2, 2, 2, 2, 32, 2)]
+    expected, bufs, prms = [], [], []
+    for i, cfg in enumerate(cfgs):
+        poly, proof = _alone(oracle, cfg, 400 + i)
+        expected.append(proof)
+        d = whir_amd.api.DeviceBuffer(poly.nbytes)
+        d.upload(poly)
+        bufs.append(d)
+        prms.append(whir_amd.params(*cfg))
+    srv = whir_amd.Server(4, 8 << 30, ctx=gpu_ctx)
+    try:
+        ids = [srv.submit(p, d.ptr) for p, d in zip(prms, bufs)]
+        one = (ctypes.c_int64 * 1)()
+        n = ctypes.c_int()
+        reported = []
+        for _ in range(400000):
+            assert lib().whir_server_poll(srv.ptr, one, 1, ctypes.byref(n)) == 0
+            assert n.value in (0, 1)
+            if n.value:
+                reported.append(one[0])
+            if len(reported) == len(ids):
+                break
+        assert sorted(reported) == sorted(ids)
+        # nothing is reported twice
+        assert lib().whir_server_poll(srv.ptr, one, 1, ctypes.byref(n)) == 0 and n.value == 0
+        for jid, want in zip(ids, expected):
+            assert srv.take(jid, 1 << 22)[0] == want
+    finally:
+        srv.close()

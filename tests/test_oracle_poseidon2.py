@@ -15,7 +15,7 @@ import random
 import numpy as np
 import pytest
 
-from kbpy import P, R, RINV
+from kbpy import P, R, RINV, DuplexPy
 
 M4 = [[2, 3, 1, 1], [1, 2, 3, 1], [1, 1, 2, 3], [3, 1, 1, 2]]
 
@@ -178,33 +178,6 @@ def test_sponge_and_compress(oracle):
     l = rnd.integers(0, P, 8, dtype=np.uint64).astype(np.uint32)
     r = rnd.integers(0, P, 8, dtype=np.uint64).astype(np.uint32)
     assert np.array_equal(oracle.compress16(l, r), oracle.permute(16, np.concatenate([l, r]))[:8])
-
-
-class DuplexPy:
-    """DuplexChallenger<F, P16, 16, 8> restated over the oracle's permutation."""
-
-    def __init__(self, oracle):
-        self.o = oracle
-        self.state = np.zeros(16, np.uint32)
-        self.inp, self.out = [], []
-
-    def duplex(self):
-        for i, v in enumerate(self.inp):
-            self.state[i] = v
-        self.inp = []
-        self.state = self.o.permute(16, self.state)
-        self.out = list(self.state[:8])
-
-    def observe(self, v):
-        self.out = []
-        self.inp.append(v)
-        if len(self.inp) == 8:
-            self.duplex()
-
-    def sample(self):
-        if self.inp or not self.out:
-            self.duplex()
-        return self.out.pop()
 
 
 @pytest.mark.parametrize("n_obs,n_samples", [(0, 1), (0, 20), (1, 3), (7, 9), (8, 1), (8, 9), (9, 4), (16, 17), (23, 40)])

@@ -46,6 +46,16 @@ class whir_stack(ctypes.Structure):
     ]
 
 
+class whir_challenger(ctypes.Structure):
+    _fields_ = [
+        ("sponge_state", ctypes.c_uint32 * 16),
+        ("input_buffer", ctypes.c_uint32 * 8),
+        ("output_buffer", ctypes.c_uint32 * 8),
+        ("input_len", ctypes.c_uint32),
+        ("output_len", ctypes.c_uint32),
+    ]
+
+
 # (name, restype, argtypes) for every entry point declared in include/whir_gpu.h
 _vp = ctypes.c_void_p
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -76,6 +86,10 @@ SIGNATURES = [
     ("whir_prover_enqueue", ctypes.c_int, [_vp, _vp, _vp]),
     ("whir_prover_result", ctypes.c_int,
      [_vp, _u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
+    ("whir_prover_commit", ctypes.c_int, [_vp, _vp, ctypes.POINTER(whir_challenger), _vp]),
+    ("whir_prover_eval", ctypes.c_int, [_vp, _u32p, _vp]),
+    ("whir_prover_open", ctypes.c_int, [_vp, _vp]),
+    ("whir_prover_challenger", ctypes.c_int, [_vp, ctypes.POINTER(whir_challenger)]),
     ("whir_prover_create_stacked", ctypes.c_int,
      [_vp, ctypes.POINTER(whir_params), ctypes.POINTER(whir_stack), ctypes.POINTER(_vp)]),
     ("whir_prover_stack_offset", ctypes.c_int64, [_vp, ctypes.c_int32]),

@@ -3,7 +3,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import check, lib, whir_params, whir_stack
+from ._lib import check, lib, whir_challenger, whir_params, whir_stack
 
 UNIQUE_DECODING, JOHNSON_BOUND, CAPACITY_BOUND = 0, 1, 2
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -267,6 +267,46 @@ class Prover:
         check(lib().whir_prover_result(self.ptr, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
                                        cp.ctypes.data_as(_u32p), root.ctypes.data_as(_u32p)), "whir_prover_result")
         return bytes(buf[: ln.value]), int(cp[0]), root
+
+    # Whir::commit / ProverStack::eval / Whir::open with the caller's transcript
+    # (whir.rs:127-180, prover.rs:64-90)
+    def commit(self, d_poly=None, challenger=None, stream=None):
+        """Commits d_poly (default: the uploaded input); `challenger` is a dict of
+        whir_challenger fields (None: a fresh transcript)."""
+        st = None
+        if challenger is not None:
+            st = whir_challenger()
+            for i, v in enumerate(challenger["sponge_state"]):
+                st.sponge_state[i] = int(v)
+            for i in range(int(challenger["input_len"])):
+                st.input_buffer[i] = int(challenger["input_buffer"][i])
+            for i in range(int(challenger["output_len"])):
+                st.output_buffer[i] = int(challenger["output_buffer"][i])
+            st.input_len = int(challenger["input_len"])
+            st.output_len = int(challenger["output_len"])
+        check(lib().whir_prover_commit(self.ptr, d_poly or self.input_ptr, ctypes.byref(st) if st else None, stream),
+              "whir_prover_commit")
+
+    def eval(self, point=None, stream=None):
+        """The next evaluation claim: at `point` (k_poly x 4 Montgomery words) or,
+        with None, at expand(k_poly, draw())."""
+        if point is None:
+            check(lib().whir_prover_eval(self.ptr, None, stream), "whir_prover_eval")
+        else:
+            pt = _u32(point).reshape(-1)
+            check(lib().whir_prover_eval(self.ptr, pt.ctypes.data_as(_u32p), stream), "whir_prover_eval")
+
+    def open(self, stream=None):
+        check(lib().whir_prover_open(self.ptr, stream), "whir_prover_open")
+
+    def challenger(self):
+        """The transcript state open handed back, as whir_challenger fields."""
+        st = whir_challenger()
+        check(lib().whir_prover_challenger(self.ptr, ctypes.byref(st)), "whir_prover_challenger")
+        return {"sponge_state": np.array(st.sponge_state[:], np.uint32),
+                "input_buffer": np.array(st.input_buffer[:], np.uint32),
+                "output_buffer": np.array(st.output_buffer[:], np.uint32),
+                "input_len": int(st.input_len), "output_len": int(st.output_len)}
 
     TIMER_LEAVES_BASE, TIMER_DFT_BASE = 0, 1
 

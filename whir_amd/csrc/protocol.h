@@ -117,10 +117,20 @@ __device__ __forceinline__ void expand_point_wave(const Ext& z, uint32_t k, Ext*
   }
 }
 
+// A claim point given by the caller (ProverStack::eval takes any point,
+// prover.rs:64-90): k extension coordinates, passed by value.
+struct GivenPoint {
+  Ext z[MAX_K];
+};
+
 // Draw the claim point: point = expand(k, draw()) (test.rs:225-226, prover.rs:240,
-// whir.rs:227-229), then build the eq tables of point[fsv..k) split at m0.
-__device__ void claim_begin_body(WaveTx& tx, const ClaimBegin& cb, ClaimScratch& sc) {
-  if (is_wave0()) {
+// whir.rs:227-229), or take the caller's `given` point (no draw), then build
+// the eq tables of point[fsv..k) split at m0.
+__device__ void claim_begin_body(WaveTx& tx, const ClaimBegin& cb, ClaimScratch& sc,
+                                 const GivenPoint* given = nullptr) {
+  if (given) {
+    for (uint32_t i = threadIdx.x; i < cb.k * 4; i += blockDim.x) sc.pt[i / 4].c[i % 4] = given->z[i / 4].c[i % 4];
+  } else if (is_wave0()) {
     const Ext z = tx.sample_ext();
     expand_point_wave(z, cb.k, sc.pt);
   }
@@ -163,6 +173,27 @@ __global__ void __launch_bounds__(SERIAL_THREADS) tree_top_claim_kernel(uint32_t
   }
   if (cb.slot) claim_begin_body(tx, cb, sc);
   if (is_wave0()) tx_end(tx, t, js);
+}
+
+// A claim group's begin on its own (the commit / eval / open entry points,
+// where the caller decides between a drawn and a given point after the
+// commitment): the point, written to the group's slots, and its eq tables.
+__global__ void __launch_bounds__(SERIAL_THREADS) claim_begin_kernel(DevTx* t, uint32_t* log, JobScalars* js,
+                                                                     ClaimBegin cb, GivenPoint gp, uint32_t given) {
+  __shared__ ClaimScratch sc;
+  WaveTx tx;
+  if (is_wave0()) tx_begin(tx, t, log, js);
+  claim_begin_body(tx, cb, sc, given ? &gp : nullptr);
+  if (is_wave0()) tx_end(tx, t, js);
+}
+
+// Continue a caller's transcript: the challenger state `init` (sponge state,
+// input and output buffers) with an empty proof log of init.log_cap words.
+__global__ void tx_import_kernel(DevTx* t, JobScalars* js, DevTx init) {
+  if (threadIdx.x == 0) {
+    *t = init;
+    js->pending_words = 0;
+  }
 }
 
 // Reduce the evaluation partials of a group of `count` claims drawn at one

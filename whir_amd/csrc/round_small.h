@@ -326,7 +326,10 @@ __global__ void __launch_bounds__(SERIAL_THREADS) final_phase_kernel(DevTx* t, u
     if (tid == 0) s_u = tx.log_words;
     tx.log_words += count * per;
     if (tx.log_words > tx.log_cap) tx.overflow = 1;
-    const uint32_t v = tx.sample();  // the test's checkpoint draw
+    // the test's checkpoint draw, on a copy: the stored transcript is the one
+    // Whir::open hands back (whir.rs:359), before the caller's next draw
+    WaveTx after = tx;
+    const uint32_t v = after.sample();
     if (tid == 0) {
       js->checkpoint = v;
       // the proof's status after the log's capacity, so one copy brings back

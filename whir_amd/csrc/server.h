@@ -5,9 +5,12 @@
 //   after a job finishes (codewords, trees, eq tables and the captured hipGraph
 //   of the whole proof are reused by the next job of that shape);
 // * admission (FIFO): the head job starts when a slot is free and its shape's
-//   workspace is idle, or a new one fits in the HBM budget after evicting idle
-//   workspaces of other shapes; otherwise it waits for running jobs to finish;
-// * completion: each job's last operation records an event; poll() never blocks.
+//   workspace is idle, or a new one fits in the HBM budget, or an idle
+//   workspace of another shape is large enough to be re-laid out for it;
+//   otherwise idle workspaces are released when nothing runs (a release waits
+//   for the whole device), and until then the job waits;
+// * completion: each job's last operation records an event; poll() never
+//   blocks, and reports every finished job exactly once.
 // Jobs that run together overlap on the device (different streams), and each
 // job's proof is computed by exactly the same kernel sequence as when alone.
 #pragma once
@@ -42,9 +45,21 @@ struct whir_server {
   std::map<ShapeKey, std::vector<whir_prover*>> idle;
   std::map<int64_t, ServerJob> jobs;
   std::deque<int64_t> queued;
+  std::deque<int64_t> done_pending;     // finished jobs not yet reported by poll
+  std::vector<whir_prover*> retired;    // workspaces to release at the next idle point
+  std::vector<void*> retired_host;      // pinned host logs to release at the next idle point
   int64_t next_id = 1;
   int running = 0;
 };
+
+// Device and pinned-host releases wait for the whole device, so the server
+// makes them only when none of its jobs is running.
+static void server_release_retired(whir_server* S) {
+  for (whir_prover* p : S->retired) whir_prover_destroy(p);
+  S->retired.clear();
+  for (void* h : S->retired_host) (void)hipHostFree(h);
+  S->retired_host.clear();
+}
 
 extern "C" int whir_server_create(whir_ctx* ctx, int max_slots, uint64_t hbm_budget_bytes, whir_server** out) {
   if (!ctx || !out || max_slots < 1 || max_slots > 64) return WHIR_EINVAL;
@@ -55,6 +70,7 @@ extern "C" int whir_server_create(whir_ctx* ctx, int max_slots, uint64_t hbm_bud
   for (int i = 0; i < max_slots; ++i) {
     hipStream_t st;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      for (hipStream_t made : S->streams) (void)hipStreamDestroy(made);
       delete S;
       return WHIR_EDEVICE;
     }
@@ -72,6 +88,7 @@ extern "C" int whir_server_destroy(whir_server* S) {
     if (kv.second.prover) whir_prover_destroy(kv.second.prover);
   for (auto& kv : S->idle)
     for (whir_prover* p : kv.second) whir_prover_destroy(p);
+  server_release_retired(S);
   for (hipStream_t st : S->streams) (void)hipStreamDestroy(st);
   delete S;
   return WHIR_OK;
@@ -91,20 +108,17 @@ extern "C" int whir_server_submit(whir_server* S, const whir_params* params, con
   return WHIR_OK;
 }
 
-static bool server_evict_one(whir_server* S) {
-  for (auto& kv : S->idle) {
-    if (!kv.second.empty()) {
-      whir_prover* p = kv.second.back();
-      kv.second.pop_back();
-      S->used -= p->arena_bytes;
-      whir_prover_destroy(p);
-      return true;
-    }
-  }
-  return false;
+static void server_retire(whir_server* S, whir_prover* p) {
+  S->used -= p->arena_bytes;
+  S->retired.push_back(p);
 }
 
-// A workspace for the job's shape, or nullptr if it has to wait.
+// A workspace for the job's shape, or nullptr if it has to wait (err ==
+// WHIR_OK) or can never run (err set). In order: an idle workspace of the
+// shape; a new one within the budget; an idle workspace of another shape whose
+// arena holds this shape, re-laid out in place (no device allocation or
+// release); and, only when no job is running, idle workspaces released until a
+// new one fits.
 static whir_prover* server_acquire(whir_server* S, ServerJob& j, int& err) {
   err = WHIR_OK;
   auto it = S->idle.find(shape_key(j.params));
@@ -114,11 +128,40 @@ static whir_prover* server_acquire(whir_server* S, ServerJob& j, int& err) {
     return p;
   }
   const uint64_t need = whir_prover_bytes_for(&j.params);
-  while (S->used + need > S->budget && server_evict_one(S)) {
-  }
   if (S->used + need > S->budget) {
-    if (S->running == 0) err = WHIR_EOOM;  // can never fit
-    return nullptr;
+    // the smallest idle arena that holds the new shape
+    std::vector<whir_prover*>* best_list = nullptr;
+    size_t best_i = 0;
+    for (auto& kv : S->idle)
+      for (size_t i = 0; i < kv.second.size(); ++i)
+        if (kv.second[i]->arena_bytes >= need &&
+            (!best_list || kv.second[i]->arena_bytes < (*best_list)[best_i]->arena_bytes)) {
+          best_list = &kv.second;
+          best_i = i;
+        }
+    if (best_list) {
+      whir_prover* old = (*best_list)[best_i];
+      whir_prover* p = prover_relayout(old, &j.params, S->retired_host);
+      if (p) {
+        best_list->erase(best_list->begin() + best_i);
+        whir_prover_set_graph(p, 1);
+        return p;
+      }
+    }
+    if (S->running > 0) return nullptr;  // wait: releasing memory now would stall the running jobs
+    server_release_retired(S);
+    for (auto& kv : S->idle) {
+      while (!kv.second.empty() && S->used + need > S->budget) {
+        whir_prover* p = kv.second.back();
+        kv.second.pop_back();
+        S->used -= p->arena_bytes;
+        whir_prover_destroy(p);
+      }
+    }
+    if (S->used + need > S->budget) {
+      err = WHIR_EOOM;  // can never fit
+      return nullptr;
+    }
   }
   whir_prover* p = nullptr;
   int rc = whir_prover_create(S->ctx, &j.params, &p);
@@ -132,30 +175,37 @@ static whir_prover* server_acquire(whir_server* S, ServerJob& j, int& err) {
   return p;
 }
 
+// Collects finished jobs and admits queued ones; never blocks. Every finished
+// job (also one that failed to start) is reported by exactly one poll; ids
+// beyond `max` stay queued for the next one.
 extern "C" int whir_server_poll(whir_server* S, int64_t* done_ids, int max, int* n_done) {
+  if (n_done) *n_done = 0;
   if (!S) return WHIR_EINVAL;
-  int nd = 0;
+  if (!done_ids) max = 0;
   // collect finished jobs
   for (int sl = 0; sl < S->max_slots; ++sl) {
     const int64_t id = S->slot_job[sl];
     if (id < 0) continue;
     ServerJob& j = S->jobs[id];
     const int q = whir_prover_query(j.prover);
-    if (q == 7) continue;
+    if (q == WHIR_EBUSY) continue;
     size_t len = 0;
-    j.proof.resize(whir_prover_proof_capacity(j.prover));
-    j.status = q == WHIR_OK ? whir_prover_result(j.prover, j.proof.data(), j.proof.size(), &len, &j.checkpoint,
-                                                 nullptr)
-                            : q;
+    if (q == WHIR_OK) {
+      j.proof.resize(whir_prover_proof_capacity(j.prover));
+      j.status = whir_prover_result(j.prover, j.proof.data(), j.proof.size(), &len, &j.checkpoint, nullptr);
+    } else {
+      j.status = q;
+    }
     j.proof.resize(j.status == WHIR_OK ? len : 0);
-    S->idle[shape_key(j.params)].push_back(j.prover);
+    if (j.status == WHIR_EDEVICE || q != WHIR_OK) server_retire(S, j.prover);  // not reused after a device error
+    else S->idle[shape_key(j.params)].push_back(j.prover);
     j.prover = nullptr;
     j.state = 2;
     S->slot_job[sl] = -1;
     S->running -= 1;
-    if (done_ids && nd < max) done_ids[nd] = id;
-    ++nd;
+    S->done_pending.push_back(id);
   }
+  if (S->running == 0) server_release_retired(S);
   // admit queued jobs in arrival order
   while (!S->queued.empty()) {
     int sl = -1;
@@ -173,20 +223,30 @@ extern "C" int whir_server_poll(whir_server* S, int64_t* done_ids, int max, int*
       j.status = err;
       j.state = 2;
       S->queued.pop_front();
-      if (done_ids && nd < max) done_ids[nd] = j.id;
-      ++nd;
+      S->done_pending.push_back(j.id);
       continue;
     }
     S->queued.pop_front();
+    const int rc = whir_prover_enqueue(p, j.d_poly, S->streams[sl]);
+    if (rc) {  // reported as the job's status; the workspace is not reused
+      server_retire(S, p);
+      j.status = rc;
+      j.state = 2;
+      S->done_pending.push_back(j.id);
+      continue;
+    }
     j.prover = p;
     j.slot = sl;
     j.state = 1;
     S->slot_job[sl] = j.id;
     S->running += 1;
-    int rc = whir_prover_enqueue(p, j.d_poly, S->streams[sl]);
-    if (rc) return rc;
   }
-  if (n_done) *n_done = nd < max ? nd : max;
+  int nd = 0;
+  while (nd < max && !S->done_pending.empty()) {
+    done_ids[nd++] = S->done_pending.front();
+    S->done_pending.pop_front();
+  }
+  if (n_done) *n_done = nd;
   return WHIR_OK;
 }
 
@@ -206,6 +266,12 @@ extern "C" int whir_server_take(whir_server* S, int64_t job_id, uint8_t* proof, 
     }
   }
   S->jobs.erase(it);
+  // a job taken before poll reported it is not reported afterwards
+  for (auto d = S->done_pending.begin(); d != S->done_pending.end(); ++d)
+    if (*d == job_id) {
+      S->done_pending.erase(d);
+      break;
+    }
   return st;
 }
 

@@ -451,12 +451,21 @@ struct whir_prover {
   struct Group {
     uint32_t c0, count, k;
   };
-  std::vector<Group> groups;
+  std::vector<Group> groups;  // evaluation-claim groups, then one per virtual claim
+  int n_user_groups = 0;
   std::vector<ClaimDesc> hdesc;
   std::vector<uint32_t> h_group_start, h_seg_start, h_seg_claims;
   uint32_t full0 = 0, n_full = 0;  // claims spanning the whole stacked vector
   StackBlocks sb{};
-  uint32_t* host_log = nullptr;  // pinned: the log, then LOG_TAIL status words
+  uint32_t* host_log = nullptr;  // pinned: the log, LOG_TAIL status words, the transcript (tx_off)
+  size_t tx_off = 0;             // byte offset of the DevTx in the log allocation (and in host_log)
+  size_t host_log_bytes = 0;     // needed by this shape
+  size_t host_log_cap = 0;       // allocated
+  bool result_ok = false;        // the last enqueue went through; whir_prover_result may read it
+  // commit / eval / open session: 0 idle, 1 committed (next_claim claims done), 2 opened
+  int session = 0;
+  int next_claim = 0;
+  const uint32_t* session_poly = nullptr;
   // optional HIP-event timers around kernel classes (whir_prover_set_timers)
   bool timers = false;
   hipEvent_t ev[WHIR_TIMER_COUNT][2] = {};
@@ -495,6 +504,7 @@ static int claim_table(whir_prover* P, int n_ood0, uint64_t& pool_ext, uint64_t&
     if (S.claim_fresh[c]) P->groups.push_back({(uint32_t)c, 1u, kc});
     else P->groups.back().count++;
   }
+  P->n_user_groups = (int)P->groups.size();
   for (int v = 0; v < n_ood0; ++v) P->groups.push_back({(uint32_t)(n_user + v), 1u, (uint32_t)k});
   P->hdesc.assign((size_t)nc, ClaimDesc{});
   P->h_group_start.clear();
@@ -609,10 +619,13 @@ static int prover_layout(whir_prover* P, const whir_params* prm, const whir_stac
   const uint64_t pow_l = 1ull << (m_pow / 2), pow_r = 1ull << (m_pow - m_pow / 2);
   const uint64_t part_ext = std::max<uint64_t>({group_part_ext, 1ull << P->m1_svo, 4096});
   constexpr uint64_t PE_PER = (2 << MAX_F) - 2;
+  // the transcript state sits right after the log and its status words, so
+  // one copy brings back the proof and the transcript that open hands back
+  P->tx_off = ((pl.log_words + LOG_TAIL) * 4 + 255) & ~(size_t)255;
+  P->host_log_bytes = P->tx_off + sizeof(DevTx);
   items = {
-      {(void**)&P->tx, sizeof(DevTx)},
       {(void**)&P->js, sizeof(JobScalars)},
-      {(void**)&P->log, (pl.log_words + LOG_TAIL) * 4},
+      {(void**)&P->log, P->host_log_bytes},
       {(void**)&P->input, (1ull << k) * 4},
       {(void**)&P->cw0, cw0_words * 4},
       {(void**)&P->tree0, (2 * H0 - 1) * 32},
@@ -664,6 +677,35 @@ extern "C" size_t whir_prover_bytes_for(const whir_params* prm) {
   return arena_size(items);
 }
 
+// Carves P's buffers (laid out in `items`) out of P->arena and uploads the
+// shape's static claim tables.
+static int prover_bind(whir_prover* P, const std::vector<ArenaItem>& items) {
+  char* p = (char*)P->arena;
+  for (auto& it : items) {
+    *it.ptr = p;
+    p += align_up(it.bytes);
+  }
+  P->tx = (DevTx*)((char*)P->log + P->tx_off);
+  const struct {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  } up[] = {
+      {P->desc, P->hdesc.data(), P->hdesc.size() * sizeof(ClaimDesc)},
+      {P->group_start, P->h_group_start.data(), P->h_group_start.size() * 4},
+      {P->seg_start, P->h_seg_start.data(), P->h_seg_start.size() * 4},
+      {P->seg_claims, P->h_seg_claims.data(), P->h_seg_claims.size() * 4},
+  };
+  for (const auto& u : up)
+    if (hipMemcpy(u.dst, u.src, u.bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipGetLastError();
+      return WHIR_EDEVICE;
+    }
+  P->sb.seg_start = P->seg_start;
+  P->sb.seg_claims = P->seg_claims;
+  return WHIR_OK;
+}
+
 static int prover_create(whir_ctx* c, const whir_params* prm, const whir_stack* st, whir_prover** out) {
   if (!c || !prm || !out) return WHIR_EINVAL;
   whir_prover* P = new whir_prover();
@@ -681,36 +723,62 @@ static int prover_create(whir_ctx* c, const whir_params* prm, const whir_stack* 
     return WHIR_EOOM;
   }
   P->arena_bytes = total;
-  char* p = (char*)P->arena;
-  for (auto& it : items) {
-    *it.ptr = p;
-    p += align_up(it.bytes);
-  }
-  const Plan& pl = P->plan;
-  if (hipHostMalloc((void**)&P->host_log, (pl.log_words + LOG_TAIL) * 4, hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void**)&P->host_log, P->host_log_bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
     whir_prover_destroy(P);
     return WHIR_EOOM;
   }
-  // the static claim table of this shape
-  const struct {
-    void* dst;
-    const void* src;
-    size_t bytes;
-  } up[] = {
-      {P->desc, P->hdesc.data(), P->hdesc.size() * sizeof(ClaimDesc)},
-      {P->group_start, P->h_group_start.data(), P->h_group_start.size() * 4},
-      {P->seg_start, P->h_seg_start.data(), P->h_seg_start.size() * 4},
-      {P->seg_claims, P->h_seg_claims.data(), P->h_seg_claims.size() * 4},
-  };
-  for (const auto& u : up)
-    if (hipMemcpy(u.dst, u.src, u.bytes, hipMemcpyHostToDevice) != hipSuccess) {
-      whir_prover_destroy(P);
-      return WHIR_EDEVICE;
-    }
-  P->sb.seg_start = P->seg_start;
-  P->sb.seg_claims = P->seg_claims;
+  P->host_log_cap = P->host_log_bytes;
+  if ((rc = prover_bind(P, items))) {
+    whir_prover_destroy(P);
+    return rc;
+  }
   *out = P;
   return WHIR_OK;
+}
+
+// Re-lays out the idle one-poly workspace P for the proof shape `prm` inside
+// P's own arena: no device memory is allocated or released (a release waits
+// for the whole device). P's pinned host log is kept when it is large enough;
+// otherwise a new one is allocated and the old one goes to `host_to_free`.
+// Returns the new workspace, P having been consumed, or nullptr with P
+// unchanged when the shape does not fit.
+static whir_prover* prover_relayout(whir_prover* P, const whir_params* prm, std::vector<void*>& host_to_free) {
+  whir_prover* Q = new whir_prover();
+  Q->ctx = P->ctx;
+  std::vector<ArenaItem> items;
+  if (prover_layout(Q, prm, nullptr, items) || arena_size(items) > P->arena_bytes) {
+    delete Q;
+    return nullptr;
+  }
+  Q->arena = P->arena;
+  Q->arena_bytes = P->arena_bytes;
+  if (P->host_log_cap >= Q->host_log_bytes) {
+    Q->host_log = P->host_log;
+    Q->host_log_cap = P->host_log_cap;
+  } else {
+    if (hipHostMalloc((void**)&Q->host_log, Q->host_log_bytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      delete Q;
+      return nullptr;
+    }
+    Q->host_log_cap = Q->host_log_bytes;
+  }
+  if (prover_bind(Q, items)) {
+    if (Q->host_log != P->host_log) host_to_free.push_back(Q->host_log);
+    delete Q;
+    return nullptr;
+  }
+  if (Q->host_log != P->host_log) host_to_free.push_back(P->host_log);
+  // P's remaining resources: its graph (captured for the old shape), events
+  if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
+  if (P->graph) (void)hipGraphDestroy(P->graph);
+  for (int i = 0; i < WHIR_TIMER_COUNT; ++i)
+    for (int j = 0; j < 2; ++j)
+      if (P->ev[i][j]) (void)hipEventDestroy(P->ev[i][j]);
+  Q->done = P->done;
+  delete P;
+  return Q;
 }
 
 extern "C" int whir_prover_create(whir_ctx* c, const whir_params* prm, whir_prover** out) {
@@ -799,15 +867,37 @@ static void launch_expr(unsigned grid, hipStream_t s, const uint32_t* poly, cons
     default: return WHIR_EINVAL;  \
   }
 
-// The whole proof as one kernel sequence on stream s (no host synchronisation).
-static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
+// Claim group g's begin (its point and eq tables); an empty begin past the
+// last group.
+static ClaimBegin group_begin(const whir_prover* P, int g) {
+  ClaimBegin cb{};
+  if (g >= 0 && g < (int)P->groups.size()) {
+    const whir_prover::Group& G = P->groups[g];
+    const ClaimDesc& d = P->hdesc[G.c0];
+    cb.slot = P->slots + G.c0;
+    cb.count = G.count;
+    cb.eq0 = P->svo_pool + d.eq0_off;
+    cb.eq1 = P->svo_pool + d.eq1_off;
+    cb.k = G.k;
+    cb.fsv = (uint32_t)P->plan.f;
+    cb.m0 = d.m0;
+    cb.m1 = d.m1;
+  }
+  return cb;
+}
+
+// ---- commit: RS encode + Merkle + transcript.write(root) (whir.rs:19-36,
+// 127-138). `first` is the claim begin that runs in the root's kernel (the
+// fused proof draws the first claim point there); `import` continues a
+// caller's transcript instead of starting a fresh one.
+static int enqueue_commit(whir_prover* P, const uint32_t* poly, hipStream_t s, const ClaimBegin& first,
+                          const DevTx* import) {
   whir_ctx* c = P->ctx;
   const Plan& pl = P->plan;
   const int k = pl.k, f = pl.f;
   const uint32_t W = 1u << f;
-  int rc;  // the transcript starts fresh in the base commitment's tree_top_claim_kernel
-
-  // ---- commit: RS encode + Merkle (whir.rs:19-36) ----
+  int rc;
+  if (import) hipLaunchKernelGGL(tx_import_kernel, dim3(1), dim3(64), 0, s, P->tx, P->js, *import);
   const int log_h0 = k + pl.rate - f;
   if (P->timers) HIP_TRY(hipEventRecord(P->ev[WHIR_TIMER_DFT_BASE][0], s));
   if ((rc = ntt_enqueue(c, poly, 1ull << (k - f), log_h0, f, P->cw0, P->ntt_tmp, s))) return rc;
@@ -818,40 +908,52 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
                            &top_n)))
     return rc;
   P->timer_pending = P->timers;
-
-  // ---- evaluation claims + OOD virtual claims (prover.rs:64-90, 231-277) ----
-  // Claims drawn at one point form a group (one compress launch over its
-  // polys). transcript.write(root) runs in the first group's kernel; each
-  // group's end runs in the same kernel as the next group's draw.
-  const uint32_t m0 = P->m0_svo;
-  const int n_groups = (int)P->groups.size();
-  auto group_begin = [&](int g) {
-    ClaimBegin cb{};
-    if (g < n_groups) {
-      const whir_prover::Group& G = P->groups[g];
-      const ClaimDesc& d = P->hdesc[G.c0];
-      cb.slot = P->slots + G.c0;
-      cb.count = G.count;
-      cb.eq0 = P->svo_pool + d.eq0_off;
-      cb.eq1 = P->svo_pool + d.eq1_off;
-      cb.k = G.k;
-      cb.fsv = (uint32_t)f;
-      cb.m0 = d.m0;
-      cb.m1 = d.m1;
-    }
-    return cb;
-  };
   // the tree's top levels, transcript.write(root) and the first group's draw
   hipLaunchKernelGGL(tree_top_claim_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, top, top_n, P->tx, P->log, P->js,
-                     group_begin(0), (uint32_t)pl.log_words);
-  for (int g = 0; g < n_groups; ++g) {
-    const ClaimBegin cb = group_begin(g);
+                     first, import ? 0u : (uint32_t)pl.log_words);
+  return WHIR_OK;
+}
+
+// ---- evaluation claims + OOD virtual claims (prover.rs:64-90, 231-277) ----
+// Claims at one point form a group (one compress launch over its polys).
+// Groups g0..g1-1, each begun already; each group's end runs in the same
+// kernel as the next group's begin (groups up to g1 - 1).
+static int enqueue_groups(whir_prover* P, const uint32_t* poly, hipStream_t s, int g0, int g1) {
+  const int f = P->plan.f;
+  for (int g = g0; g < g1; ++g) {
+    const ClaimBegin cb = group_begin(P, g);
     const whir_prover::Group& G = P->groups[g];
     FSWITCH(f, launch_compress_hi<FL>(dim3(1u << cb.m1, G.count), s, poly, cb.eq0, cb.eq1, cb.m0, P->partials,
                                       P->desc, G.c0));
+    const ClaimBegin next = g + 1 < g1 ? group_begin(P, g + 1) : ClaimBegin{};
     hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, cb.slot, G.count,
-                       P->partials, 1u << cb.m1, (uint32_t)f, group_begin(g + 1));
+                       P->partials, 1u << cb.m1, (uint32_t)f, next);
   }
+  return WHIR_OK;
+}
+
+static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s);
+
+// The whole proof as one kernel sequence on stream s (no host synchronisation):
+// the test driver's commit, n claims at drawn points and open
+// (crates/whir/src/test.rs:218-235).
+static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
+  int rc;
+  if ((rc = enqueue_commit(P, poly, s, group_begin(P, 0), nullptr))) return rc;
+  if ((rc = enqueue_groups(P, poly, s, 0, (int)P->groups.size()))) return rc;
+  return enqueue_open(P, poly, s);
+}
+
+// Whir::open once every claim group has been evaluated (whir.rs:168-359): SVO
+// rounds, expression, WHIR rounds, final phase.
+static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
+  whir_ctx* c = P->ctx;
+  const Plan& pl = P->plan;
+  const int k = pl.k, f = pl.f;
+  const uint32_t W = 1u << f;
+  int rc;
+  const int n_groups = (int)P->groups.size();
+  const uint32_t m0 = P->m0_svo;
   // ---- SVO rounds + expression (prover.rs:279-353, 184-229) ----
   hipLaunchKernelGGL(svo_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots, P->desc,
                      (uint32_t)P->n_svo_claims, P->group_start, (uint32_t)n_groups, (uint32_t)f, P->pe_buf,
@@ -860,6 +962,8 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
   FSWITCH(f, launch_expr<FL>(blocks_for(n), s, poly, P->js->eq_rs, P, m0, n, P->ep[0], P->ew[0]));
   int cur = 0;
   uint32_t rs_idx = (uint32_t)f;
+  uint32_t* top = nullptr;
+  uint32_t top_n = 0;
 
   // ---- WHIR rounds (whir.rs:205-312) ----
   for (int r = 0; r < pl.n_rounds; ++r) {
@@ -1016,12 +1120,17 @@ static int prover_enqueue(whir_prover* P, const uint32_t* poly, hipStream_t s) {
   return WHIR_OK;
 }
 
+// The log, the status words final_phase_kernel leaves after it and the
+// transcript state, in one device-to-host copy.
+static int enqueue_result_copy(whir_prover* P, hipStream_t s) {
+  HIP_TRY(hipMemcpyAsync(P->host_log, P->log, P->host_log_bytes, hipMemcpyDeviceToHost, s));
+  return WHIR_OK;
+}
+
 static int enqueue_with_copies(whir_prover* P, const uint32_t* d_poly, hipStream_t s) {
   int rc = prover_enqueue(P, d_poly, s);
   if (rc) return rc;
-  // the log and the status words final_phase_kernel leaves after it
-  HIP_TRY(hipMemcpyAsync(P->host_log, P->log, (P->plan.log_words + LOG_TAIL) * 4, hipMemcpyDeviceToHost, s));
-  return WHIR_OK;
+  return enqueue_result_copy(P, s);
 }
 
 extern "C" int whir_prover_set_graph(whir_prover* P, int enable) {
@@ -1030,33 +1139,47 @@ extern "C" int whir_prover_set_graph(whir_prover* P, int enable) {
   return WHIR_OK;
 }
 
+static int prover_enqueue_impl(whir_prover* P, const uint32_t* d_poly, hipStream_t s) {
+  if (!P->graph_mode) return enqueue_with_copies(P, d_poly, s);
+  // The graph always proves the prover's own input buffer.
+  if (P->timers) return WHIR_EINVAL;
+  if (d_poly != P->input)
+    HIP_TRY(hipMemcpyAsync(P->input, d_poly, (1ull << P->plan.k) * 4, hipMemcpyDeviceToDevice, s));
+  if (!P->gexec) {
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_with_copies(P, P->input, s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      return rc ? rc : WHIR_EDEVICE;
+    }
+    hipGraphExec_t ge = nullptr;
+    if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      return WHIR_EDEVICE;
+    }
+    P->graph = g;
+    P->gexec = ge;
+    P->graph_stream = s;
+  }
+  HIP_TRY(hipGraphLaunch(P->gexec, s));
+  return WHIR_OK;
+}
+
 extern "C" int whir_prover_enqueue(whir_prover* P, const uint32_t* d_poly, void* stream) {
   if (!P || !d_poly) return WHIR_EINVAL;
   hipStream_t s = pick_stream(P->ctx, stream);
   P->last_stream = s;
+  P->result_ok = false;  // a failed enqueue leaves no result to read
+  P->session = 0;
   if (!P->done) HIP_TRY(hipEventCreateWithFlags(&P->done, hipEventDisableTiming));
-  if (!P->graph_mode) {
-    int rc = enqueue_with_copies(P, d_poly, s);
-    if (rc) return rc;
-  } else {
-    // The graph always proves the prover's own input buffer.
-    if (P->timers) return WHIR_EINVAL;
-    if (d_poly != P->input)
-      HIP_TRY(hipMemcpyAsync(P->input, d_poly, (1ull << P->plan.k) * 4, hipMemcpyDeviceToDevice, s));
-    if (!P->gexec) {
-      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue_with_copies(P, P->input, s);
-      hipGraph_t g = nullptr;
-      hipError_t e = hipStreamEndCapture(s, &g);
-      if (rc) return rc;
-      if (e != hipSuccess) return WHIR_EDEVICE;
-      P->graph = g;
-      HIP_TRY(hipGraphInstantiate(&P->gexec, g, nullptr, nullptr, 0));
-      P->graph_stream = s;
-    }
-    HIP_TRY(hipGraphLaunch(P->gexec, s));
-  }
+  int rc = prover_enqueue_impl(P, d_poly, s);
+  if (rc) return rc;
   HIP_TRY(hipEventRecord(P->done, s));
+  P->result_ok = true;
   return WHIR_OK;
 }
 
@@ -1097,9 +1220,8 @@ extern "C" int whir_prover_query(whir_prover* P) {
 
 extern "C" int whir_prover_result(whir_prover* P, uint8_t* proof, size_t cap, size_t* len, uint32_t* checkpoint,
                                   uint32_t* root8) {
-  if (!P) return WHIR_EINVAL;
-  if (P->done) HIP_TRY(hipEventSynchronize(P->done));
-  else HIP_TRY(hipDeviceSynchronize());
+  if (!P || !P->result_ok || !P->done) return WHIR_EINVAL;
+  HIP_TRY(hipEventSynchronize(P->done));
   if (P->timer_pending) {
     for (int i = 0; i < WHIR_TIMER_COUNT; ++i) {
       float ms = 0.f;
@@ -1126,14 +1248,134 @@ extern "C" size_t whir_prover_device_bytes(const whir_prover* P) { return P ? P-
 
 extern "C" int whir_prove_host(whir_ctx* c, const whir_params* prm, const uint32_t* poly, uint8_t* proof, size_t cap,
                                size_t* len, uint32_t* checkpoint, uint32_t* root8) {
+  if (!poly) return WHIR_EINVAL;
   whir_prover* P = nullptr;
   int rc = whir_prover_create(c, prm, &P);
   if (rc) return rc;
-  HIP_TRY(hipMemcpy(P->input, poly, (1ull << P->plan.k) * 4, hipMemcpyHostToDevice));
-  rc = whir_prover_enqueue(P, P->input, nullptr);
+  if (hipMemcpy(P->input, poly, (1ull << P->plan.k) * 4, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    rc = WHIR_EDEVICE;
+  }
+  if (!rc) rc = whir_prover_enqueue(P, P->input, nullptr);
   if (!rc) rc = whir_prover_result(P, proof, cap, len, checkpoint, root8);
-  whir_prover_destroy(P);
-  return rc;
+  const int rd = whir_prover_destroy(P);
+  return rc ? rc : rd;
+}
+
+// ---- commit / eval / open with a caller's transcript ----
+
+static bool canonical_word(uint32_t w) { return w < kb::P; }
+
+// whir_challenger -> DevTx with an empty log of `cap` words; false if the
+// state is not one a DuplexChallenger can be in (see whir_gpu.h).
+static bool challenger_import(const whir_challenger* c, uint32_t cap, DevTx& t) {
+  if (c->input_len > 7 || c->output_len > 8 || (c->input_len && c->output_len)) return false;
+  memset(&t, 0, sizeof(t));
+  for (int i = 0; i < 16; ++i) {
+    if (!canonical_word(c->sponge_state[i])) return false;
+    t.state[i] = c->sponge_state[i];
+  }
+  for (uint32_t i = 0; i < c->input_len; ++i) {
+    if (!canonical_word(c->input_buffer[i])) return false;
+    t.in_buf[i] = c->input_buffer[i];
+  }
+  // the output buffer is the sponge state's prefix left by the last duplex
+  for (uint32_t i = 0; i < c->output_len; ++i)
+    if (c->output_buffer[i] != c->sponge_state[i]) return false;
+  for (int i = 0; i < 8; ++i) t.out_buf[i] = c->sponge_state[i];
+  t.in_len = c->input_len;
+  t.out_len = c->output_len;
+  t.log_words = 0;
+  t.log_cap = cap;
+  t.overflow = 0;
+  return true;
+}
+
+extern "C" int whir_prover_commit(whir_prover* P, const uint32_t* d_poly, const whir_challenger* tx_in, void* stream) {
+  if (!P || !d_poly) return WHIR_EINVAL;
+  DevTx init;
+  if (tx_in && !challenger_import(tx_in, (uint32_t)P->plan.log_words, init)) return WHIR_EINVAL;
+  hipStream_t s = pick_stream(P->ctx, stream);
+  P->last_stream = s;
+  P->result_ok = false;
+  P->session = 0;
+  int rc = enqueue_commit(P, d_poly, s, ClaimBegin{}, tx_in ? &init : nullptr);
+  if (rc) return rc;
+  HIP_TRY(hipGetLastError());
+  P->session = 1;
+  P->next_claim = 0;
+  P->session_poly = d_poly;
+  return WHIR_OK;
+}
+
+extern "C" int whir_prover_eval(whir_prover* P, const uint32_t* point, void* stream) {
+  if (!P || P->session != 1) return WHIR_EINVAL;
+  const int c = P->next_claim;
+  const int n_user = (int)P->stack.claim_poly.size();
+  if (c >= n_user) return WHIR_EINVAL;
+  int g = 0;
+  while (g + 1 < P->n_user_groups && (int)P->groups[g + 1].c0 <= c) ++g;
+  const whir_prover::Group& G = P->groups[g];
+  const bool lead = (int)G.c0 == c;
+  if (!lead && point) return WHIR_EINVAL;  // a claim at the previous claim's point
+  hipStream_t s = pick_stream(P->ctx, stream);
+  const int f = P->plan.f;
+  const ClaimBegin cb = group_begin(P, g);
+  if (lead) {
+    GivenPoint gp;
+    memset(&gp, 0, sizeof(gp));
+    if (point) {
+      for (uint32_t j = 0; j < G.k; ++j)
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t w = point[4 * j + q];
+          if (!canonical_word(w)) return WHIR_EINVAL;
+          gp.z[j].c[q] = w;
+        }
+    }
+    hipLaunchKernelGGL(claim_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, cb, gp,
+                       point ? 1u : 0u);
+  }
+  // this claim alone: its compression with the group's eq tables, then its value
+  FSWITCH(f, launch_compress_hi<FL>(dim3(1u << cb.m1, 1), s, P->session_poly, cb.eq0, cb.eq1, cb.m0, P->partials,
+                                    P->desc, (uint32_t)c));
+  hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots + c, 1u,
+                     P->partials, 1u << cb.m1, (uint32_t)f, ClaimBegin{});
+  HIP_TRY(hipGetLastError());
+  P->next_claim = c + 1;
+  return WHIR_OK;
+}
+
+extern "C" int whir_prover_open(whir_prover* P, void* stream) {
+  if (!P || P->session != 1 || P->next_claim != (int)P->stack.claim_poly.size()) return WHIR_EINVAL;
+  hipStream_t s = pick_stream(P->ctx, stream);
+  if (!P->done) HIP_TRY(hipEventCreateWithFlags(&P->done, hipEventDisableTiming));
+  const int ng = (int)P->groups.size();
+  int rc = WHIR_OK;
+  if (P->n_user_groups < ng) {  // the OOD virtual claims (whir.rs:187-192), drawn
+    hipLaunchKernelGGL(claim_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js,
+                       group_begin(P, P->n_user_groups), GivenPoint{}, 0u);
+    rc = enqueue_groups(P, P->session_poly, s, P->n_user_groups, ng);
+  }
+  if (!rc) rc = enqueue_open(P, P->session_poly, s);
+  if (!rc) rc = enqueue_result_copy(P, s);
+  P->session = 0;
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(P->done, s));
+  P->result_ok = true;
+  return WHIR_OK;
+}
+
+extern "C" int whir_prover_challenger(whir_prover* P, whir_challenger* out) {
+  if (!P || !out || !P->result_ok || !P->done) return WHIR_EINVAL;
+  HIP_TRY(hipEventSynchronize(P->done));
+  const DevTx* t = (const DevTx*)((const char*)P->host_log + P->tx_off);
+  memset(out, 0, sizeof(*out));
+  for (int i = 0; i < 16; ++i) out->sponge_state[i] = t->state[i];
+  out->input_len = t->in_len;
+  out->output_len = t->out_len;
+  for (uint32_t i = 0; i < t->in_len && i < 8; ++i) out->input_buffer[i] = t->in_buf[i];
+  for (uint32_t i = 0; i < t->out_len && i < 8; ++i) out->output_buffer[i] = t->state[i];
+  return WHIR_OK;
 }
 
 // ============================ memory helpers ============================
