@@ -122,6 +122,19 @@ int whir_eval_ext(whir_ctx* ctx, const uint32_t* d_poly, uint32_t k, const uint3
 int whir_compress_hi(whir_ctx* ctx, const uint32_t* d_poly, uint32_t k, const uint32_t* d_z, uint32_t m,
                      uint32_t* d_out, void* stream);
 
+/* No counterpart in the reference: whir_compress_hi for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). It allocates and releases nothing; its
+ * scratch (the point's split eq tables and the per-block partial sums) is the
+ * library's static device array of WHIR_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends. Calls needing more (m > 28 when k - m = 5) return
+ * WHIR_EINVAL. Every call on one GPU shares that array: calls that overlap on
+ * one GPU (other streams, other threads) race and must be serialised by the
+ * caller. Same arguments and result as whir_compress_hi. */
+#define WHIR_NOALLOC_SCRATCH_BYTES (16u << 20)
+int whir_compress_hi_noalloc(whir_ctx* ctx, const uint32_t* d_poly, uint32_t k, const uint32_t* d_z, uint32_t m,
+                             uint32_t* d_out, void* stream);
+
 /* Replaces SplitEq::compress_lo (crates/poly/src/eq.rs:400-539): binds the low
  * m <= 5 variables of a base polynomial; d_out has 2^(k-m) extension values. */
 int whir_compress_lo(whir_ctx* ctx, const uint32_t* d_poly, uint32_t k, const uint32_t* d_z, uint32_t m,

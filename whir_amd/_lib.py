@@ -74,6 +74,7 @@ SIGNATURES = [
     ("whir_eval_ext", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp, _vp]),
     ("whir_compress_hi", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp]),
     ("whir_compress_lo", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp]),
+    ("whir_compress_hi_noalloc", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp]),
     ("whir_sumcheck_coeffs", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint32, _vp, _vp]),
     ("whir_fix_lo_var", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, _vp, _vp]),
     ("whir_combine_eqs", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _u32p, _vp]),

@@ -188,19 +188,24 @@ extern "C" int whir_eval_ext(whir_ctx* c, const uint32_t* d_poly, uint32_t k, co
 }
 
 // compress_hi over the top m variables of a 2^k base polynomial, k - m <= 5;
-// eval_base is the case m = k.
-extern "C" int whir_compress_hi(whir_ctx* c, const uint32_t* d_poly, uint32_t k, const uint32_t* d_z, uint32_t m,
-                                uint32_t* d_out, void* stream) {
-  if (!c || !d_poly || !d_z || !d_out || m < 1 || m > k || k - m > (uint32_t)MAX_F || m > MAX_K) return WHIR_EINVAL;
-  hipStream_t s = pick_stream(c, stream);
-  Ext* pool;
-  uint32_t m0, m1;
-  uint64_t stride;
-  int rc = eq_split_enqueue(d_z, 1, m, nullptr, nullptr, s, &pool, m0, m1, stride);
-  if (rc) return rc;
+// eval_base is the case m = k. Scratch: the point's split eq tables (`pool`,
+// compress_hi_pool_words(m) Ext) and the per-block partial sums (`partials`,
+// compress_hi_part_words(k, m) Ext); both are written in full before they are
+// read, so neither needs clearing.
+static bool compress_hi_args_ok(const uint32_t* d_poly, uint32_t k, const uint32_t* d_z, uint32_t m,
+                                const uint32_t* d_out) {
+  return d_poly && d_z && d_out && m >= 1 && m <= k && k - m <= (uint32_t)MAX_F && m <= (uint32_t)MAX_K;
+}
+static uint64_t compress_hi_pool_ext(uint32_t m) { return (1ull << (m / 2)) + (1ull << (m - m / 2)); }
+static uint64_t compress_hi_part_ext(uint32_t k, uint32_t m) { return (1ull << (m - m / 2)) << (k - m); }
+
+static int compress_hi_enqueue(const uint32_t* d_poly, uint32_t k, const uint32_t* d_z, uint32_t m, uint32_t* d_out,
+                               hipStream_t s, Ext* pool, Ext* partials) {
+  const uint32_t m0 = m / 2, m1 = m - m0;
+  const uint32_t one[4] = {ONE, 0, 0, 0};
+  hipLaunchKernelGGL(eq_split_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, (const Ext*)d_z, m, ext_arg(one),
+                     ext_arg(one), m0, m1, pool, compress_hi_pool_ext(m));
   const uint32_t W = 1u << (k - m);
-  Ext* partials;
-  HIP_TRY(hipMallocAsync((void**)&partials, ((1ull << m1) * W) * 16, s));
   const unsigned grid = 1u << m1;
   const Ext* eq1 = pool + (1ull << m0);
   switch (k - m) {
@@ -212,10 +217,44 @@ extern "C" int whir_compress_hi(whir_ctx* c, const uint32_t* d_poly, uint32_t k,
     default: hipLaunchKernelGGL(compress_hi_kernel<5>, dim3(grid), dim3(SC_THREADS), 0, s, d_poly, pool, eq1, m0, partials, nullptr, 0u); break;
   }
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, partials, grid, W, (Ext*)d_out);
-  HIP_TRY(hipFreeAsync(partials, s));
-  HIP_TRY(hipFreeAsync(pool, s));
   HIP_TRY(hipGetLastError());
   return WHIR_OK;
+}
+
+extern "C" int whir_compress_hi(whir_ctx* c, const uint32_t* d_poly, uint32_t k, const uint32_t* d_z, uint32_t m,
+                                uint32_t* d_out, void* stream) {
+  if (!c || !compress_hi_args_ok(d_poly, k, d_z, m, d_out)) return WHIR_EINVAL;
+  hipStream_t s = pick_stream(c, stream);
+  Ext *pool = nullptr, *partials = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&pool, compress_hi_pool_ext(m) * 16, s));
+  if (hipMallocAsync((void**)&partials, compress_hi_part_ext(k, m) * 16, s) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFreeAsync(pool, s);
+    return WHIR_EOOM;
+  }
+  const int rc = compress_hi_enqueue(d_poly, k, d_z, m, d_out, s, pool, partials);
+  HIP_TRY(hipFreeAsync(partials, s));
+  HIP_TRY(hipFreeAsync(pool, s));
+  return rc;
+}
+
+// whir_compress_hi with its scratch in the library's static device array
+// (whir_ctx::scratch, the address of `scratch` on the context's device):
+// no allocation or release of any kind.
+extern "C" int whir_compress_hi_noalloc(whir_ctx* c, const uint32_t* d_poly, uint32_t k, const uint32_t* d_z,
+                                        uint32_t m, uint32_t* d_out, void* stream) {
+  if (!c || !c->scratch || !compress_hi_args_ok(d_poly, k, d_z, m, d_out)) return WHIR_EINVAL;
+  const uint64_t pool_bytes = (compress_hi_pool_ext(m) * 16 + 255) & ~255ull;
+  const uint64_t need = pool_bytes + compress_hi_part_ext(k, m) * 16;
+  if (need > WHIR_NOALLOC_SCRATCH_BYTES) {
+    fprintf(stderr, "whir_compress_hi_noalloc: k=%u m=%u needs %llu bytes of scratch, above the static %u\n", k, m,
+            (unsigned long long)need, (unsigned)WHIR_NOALLOC_SCRATCH_BYTES);
+    return WHIR_EINVAL;
+  }
+  hipStream_t s = pick_stream(c, stream);
+  Ext* pool = (Ext*)c->scratch;
+  Ext* partials = (Ext*)((char*)c->scratch + pool_bytes);
+  return compress_hi_enqueue(d_poly, k, d_z, m, d_out, s, pool, partials);
 }
 
 extern "C" int whir_compress_lo(whir_ctx* c, const uint32_t* d_poly, uint32_t k, const uint32_t* d_z, uint32_t m,

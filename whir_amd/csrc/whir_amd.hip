@@ -37,9 +37,15 @@ using whir_host::RoundPlan;
 
 // ============================ context ============================
 
+// Static device scratch of the allocation-free entry points (one array per
+// GPU, alive until the process ends). Host code uses the per-device address
+// from hipGetSymbolAddress (whir_ctx::scratch), never this name.
+static __device__ __align__(256) unsigned char scratch[WHIR_NOALLOC_SCRATCH_BYTES];
+
 struct whir_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  void* scratch = nullptr;  // this device's address of `scratch`
   // NTT tile shape: largest pass 2^ntt_max_logl rows, 2^ntt_log_threads threads per workgroup
   int ntt_max_logl = 8;
   int ntt_log_threads = 8;
@@ -84,6 +90,11 @@ extern "C" int whir_ctx_create(int device, uint64_t seed, whir_ctx** out) {
   HIP_TRY(hipSetDevice(device));
   whir_ctx* c = new whir_ctx();
   c->device = device;
+  if (hipGetSymbolAddress(&c->scratch, HIP_SYMBOL(scratch)) != hipSuccess || !c->scratch) {
+    (void)hipGetLastError();
+    delete c;
+    return WHIR_EDEVICE;
+  }
   int rc = upload_constants(seed);
   if (rc) {
     delete c;
