@@ -1,7 +1,9 @@
 """Benchmark of the MI355X WHIR prover hot path.
 
 Workload (one GPU): WHIR commit + prove of one multilinear polynomial with
-2^24 KoalaBear coefficients (seeded, uniform in [0, p)), rate 1/2
+2^24 KoalaBear coefficients — the reference tests' seed-1 polynomial
+Poly::rand(&mut rng(1), 24), generated by the library (whir_poly_rand) before
+timing — rate 1/2
 (log_inv_rate 1), folding factor 4, initial reduction 3, JohnsonBound
 soundness at 123 bits, one evaluation claim at a point drawn from the
 transcript (the reference's run_whir_poseidon flow,
@@ -13,19 +15,25 @@ polynomial is resident in HBM before the timed region starts.
 The JSON line also carries
   * roofline: the dominant kernel, the base-tree Poseidon2-24 leaf hash
     (one permutation per lane, VALU-bound). Its launches are timed live with HIP
-    events on the prover's stream inside the timed region; achieved = VALU lane
-    instructions issued per second (instructions per lane from the kernel's own
-    gfx950 code object, whir_amd/kernel_stats.json) against the MI355X VALU
-    issue peak (256 CUs x 4 SIMD x 16 lanes x 2.4 GHz). `traffic` comes from
-    the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present.
+    events on the prover's stream inside the timed region; achieved =
+    algorithmic 32-bit multiply operations per second: per W24 permutation 982
+    modular multiplications (8 x 24 full-round cubes and 23 partial-round cubes
+    at 2 each, 23 x 24 internal-diagonal products) of 4 multiplies each,
+    against the MI355X VALU lane rate (256 CUs x 4 SIMD x 16 lanes x 2.4 GHz).
+    Beside it, the issued-instruction utilisation: the kernel's VALU
+    instructions per lane (from its own gfx950 code object,
+    whir_amd/kernel_stats.json) per second against the same peak. `traffic`
+    comes from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary.
   * roofline_dft: the base RS encode (three NTT passes), timed the same way.
     It issues 4.3 G VALU lane instructions (rocprofv3 SQ_INSTS_VALU, read from
     the same summary) against 0.74 GB of HBM traffic, so its bound is the VALU
     issue rate (110 us at peak) rather than HBM (92 us at 8 TB/s); both
     views are in the object.
   * cpu_baseline: the CPU oracle (a multithreaded C++ restatement of the
-    reference prover, "port") proving the same polynomial on the host cores;
-    its proof bytes are compared with the GPU's (bit_exact).
+    reference prover, "port") proving the same polynomial on the host cores
+    this process may use (its CPU affinity, capped by OMP_NUM_THREADS where set:
+    the box's CPU share), median / min / max over 5 runs; its proof bytes are
+    compared with the GPU's (bit_exact).
   * batched: 64 independent jobs, variable counts uniform in 16..22 (sizes
     log-uniform), rates 1/2 and 1/4 mixed, folding 4, seeded; proven by the
     proof server (hipGraph-captured proofs on several streams, device-resident
@@ -56,10 +64,33 @@ HBM_PEAK_GBS = 8000.0
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "merkle_leaves16_k24_pmc.txt")
 
 
+MODMULS_PER_P24 = 8 * 24 * 2 + 23 * 2 + 23 * 24  # 982: cubes of full and partial rounds, internal diagonal
+MULOPS_PER_MODMUL = 4
+
+
 def synthetic_poly(seed, k):
-    """Seeded coefficients, uniform in [0, p), as Montgomery words."""
+    """Seeded coefficients, uniform in [0, p), as Montgomery words (the batched
+    and served legs' many jobs)."""
     rng = np.random.default_rng(seed)
     return rng.integers(0, 0x7F000001, size=1 << k, dtype=np.uint64).astype(np.uint32)
+
+
+def host_cpu():
+    """(CPU model, logical CPUs of the host, CPUs this process may use)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit():
+        usable = min(usable, int(omp))
+    return model, os.cpu_count(), usable
 
 
 def read_traffic(key="traffic_bytes_per_launch"):
@@ -74,21 +105,31 @@ def read_traffic(key="traffic_bytes_per_launch"):
     return None
 
 
-def cpu_baseline(poly, k, gpu_proof, threads):
+def cpu_baseline(poly, k, gpu_proof, threads, runs=5):
     from oracle import pyoracle as O
 
     O.set_threads(threads)
-    t0 = time.perf_counter()
-    proof, cp, root = O.whir_prove(k, FOLDING, RATE, IR, SOUNDNESS, SECURITY, N_POINTS, poly)
-    dt = time.perf_counter() - t0
+    times, same = [], True
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        proof, cp, root = O.whir_prove(k, FOLDING, RATE, IR, SOUNDNESS, SECURITY, N_POINTS, poly)
+        times.append(time.perf_counter() - t0)
+        same &= proof == gpu_proof
     rc, _ = O.whir_verify(k, FOLDING, RATE, IR, SOUNDNESS, SECURITY, N_POINTS, gpu_proof)
+    model, n_host, _ = host_cpu()
     return {
-        "value": round(dt * 1e3, 1),
+        "value": round(float(np.median(times)) * 1e3, 1),
         "unit": "ms",
         "cores": threads,
         "kind": "port",
-        "sample": f"one full commit+prove at k={k} by the CPU oracle (C++/OpenMP restatement, not Plonky3)",
-        "bit_exact": proof == gpu_proof,
+        "sample": (f"{runs} full commit+prove runs at k={k} of the same polynomial by the CPU oracle "
+                   "(C++/OpenMP restatement of the reference prover, not Plonky3); value = median"),
+        "min_ms": round(min(times) * 1e3, 1),
+        "max_ms": round(max(times) * 1e3, 1),
+        "runs": runs,
+        "host_logical_cpus": n_host,
+        "cpu_model": model,
+        "bit_exact": bool(same),
         "oracle_verifier_accepts_gpu_proof": rc == 0,
     }
 
@@ -260,6 +301,21 @@ def serving_leg(ctx, slots, n_jobs=240, seed=5, loads=(0.5, 1.0, 1.5)):
         "loads": [],
     }
     same = True
+    # cold start: a fresh server at load 1.0, so every shape's first job pays
+    # its workspace allocation and graph capture inside the served latency
+    cold = whir_amd.Server(slots, 64 << 30, ctx=ctx)
+    arr = np.cumsum(np.random.default_rng(seed + 2).exponential(1.0 / plain_capacity, n_jobs))
+    arr -= arr[0]
+    done, proofs = _run_server_arrivals(cold, specs, bufs, arr)
+    cold.close()
+    lat = (done - arr) * 1e3
+    out["server_cold_start_load_1.0"] = {
+        "proofs_per_s": round(n_jobs / done.max(), 1),
+        "p50_ms": round(_percentile(lat, 50), 3),
+        "p99_ms": round(_percentile(lat, 99), 3),
+        "shapes": len({tuple(getattr(p, f) for f, _ in p._fields_) for p in specs}),
+    }
+    same &= proofs == ref
     for load in loads:
         lam = load * plain_capacity
         arrivals = np.cumsum(rng.exponential(1.0 / lam, n_jobs))
@@ -321,7 +377,7 @@ def main():
     ctx = whir_amd.Context(0, 1000)
     params = whir_params(args.k, FOLDING, RATE, IR, SOUNDNESS, SECURITY, N_POINTS)
     prover = whir_amd.Prover(params, ctx)
-    poly = synthetic_poly(1, args.k)
+    poly = whir_amd.poly_rand(1, args.k)  # Poly::rand(&mut rng(1), k), test.rs:213-219
     prover.upload(poly)  # resident in HBM before timing
 
     for _ in range(args.warmup):
@@ -362,20 +418,23 @@ def main():
         kstats = {}
     instr = kstats.get("leaves16_valu_per_lane")
     leaf_s = leaf_ms / leaf_n / 1e3
-    achieved = instr * perms / leaf_s / 1e12 if instr else None
+    achieved = perms * MODMULS_PER_P24 * MULOPS_PER_MODMUL / leaf_s / 1e12
+    issued = instr * perms / leaf_s / 1e12 if instr else None
     roofline = {
         "kernel": "merkle_leaves_kernel<16> (Poseidon2-24 leaf hash of the base codeword)",
         "bound": "valu",
-        "achieved": round(achieved, 2) if achieved else None,
+        "achieved": round(achieved, 2),
         "peak": round(VALU_PEAK_T, 2),
-        "unit": "T VALU lane-instr/s",
-        "frac": round(achieved / VALU_PEAK_T, 3) if achieved else None,
+        "unit": "T int32-mul-op/s (982 modmul x 4 per W24 permutation, algorithmic)",
+        "frac": round(achieved / VALU_PEAK_T, 3),
         "traffic": read_traffic(),
         "algorithmic_bytes": perms * (64 + 32),
         "avg_launch_us": round(leaf_s * 1e6, 1),
         "perms_per_launch": perms,
         "perms_per_s": round(perms / leaf_s, 0),
-        "valu_instr_per_perm": instr,
+        "issued_valu_instr_per_perm": instr,
+        "issued_valu_T_per_s": round(issued, 2) if issued else None,
+        "issued_valu_frac": round(issued / VALU_PEAK_T, 3) if issued else None,
     }
     dft_s = dft_ms / dft_n / 1e3
     dft_traffic = read_traffic("ntt_base_traffic_bytes")
@@ -427,7 +486,7 @@ def main():
     if not args.no_serving:
         out["serving"] = serving_leg(ctx, args.slots, n_jobs=args.serve_jobs)
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or host_cpu()[2]
         out["cpu_baseline"] = cpu_baseline(poly, args.k, proof, threads)
     print(json.dumps(out), flush=True)
 

@@ -294,6 +294,12 @@ int whir_server_take(whir_server* server, int64_t job_id, uint8_t* proof, size_t
                      uint32_t* checkpoint);
 int whir_server_stats(const whir_server* server, int* running, int* queued, uint64_t* hbm_bytes);
 
+/* ---- seeded inputs ----
+ * Replaces Poly::<F>::rand(&mut rng(seed), k) (crates/poly/src/poly.rs:47-52,
+ * rng = StdRng::seed_from_u64, crates/common/src/test.rs:1-4): out (host, 2^k
+ * words) receives the polynomial the reference's tests prove for that seed. */
+int whir_poly_rand(uint64_t seed, uint32_t k, uint32_t* out);
+
 /* ---- device memory helpers (plain HIP allocations) ---- */
 int whir_device_alloc(size_t bytes, void** out);
 int whir_device_free(void* ptr);

@@ -48,3 +48,14 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
         assert "no CPU fallback" in str(e)
     else:
         raise AssertionError("expected a RuntimeError")
+
+
+def test_poly_rand_is_the_oracles_seeded_polynomial(oracle):
+    """whir_poly_rand is host code (no device call): Poly::rand(&mut rng(seed), k)
+    of the library against the oracle's restatement, for several seeds."""
+    import numpy as np
+
+    import whir_amd
+
+    for seed, k in ((1, 10), (2, 0), (1000, 13), (7, 16)):
+        assert np.array_equal(whir_amd.poly_rand(seed, k), oracle.rand_poly(seed, k)), (seed, k)

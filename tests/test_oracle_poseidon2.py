@@ -10,6 +10,7 @@
 * sponge24 / compress16 follow PaddingFreeSponge / TruncatedPermutation.
 The constant values (ChaCha12 stream, seed 1000) and V are recalled from
 Plonky3 and remain parity unpinned against the crate itself."""
+import math
 import random
 
 import numpy as np
@@ -189,3 +190,61 @@ def test_challenger_semantics(oracle, n_obs, n_samples):
         d.observe(v)
     want = [d.sample() for _ in range(n_samples)]
     assert [int(x) for x in oracle.challenger_probe(obs, n_samples)] == [int(x) for x in want]
+
+
+# ---- round numbers, pinned by the Poseidon2 paper's 128-bit security search ----
+# Restatement of the round-number search of the Poseidon2 paper's parameter
+# script (Grassi, Khovratovich, Schofnegger 2023, with the Groebner-basis bound
+# of eprint 2023/537): the cheapest (R_F, R_P) by S-box count that satisfies the
+# statistical, interpolation and three Groebner bounds plus the binomial bound,
+# each candidate then raised by the security margin (R_F + 2, R_P * 1.075
+# rounded up). Plonky3's poseidon2_round_numbers_128 tabulates the outputs of
+# this search; the oracle's R_F = 8 and R_P = 20 / 23 must be exactly them.
+def _logb(x, b):
+    return math.log(x) / math.log(b)
+
+
+def _round_bounds_hold(p, t, r_f, r_p, alpha, m):
+    n = math.ceil(_logb(p, 2))
+    rf1 = 6 if m <= (math.floor(_logb(p, 2) - (alpha - 1) / 2.0)) * (t + 1) else 10  # statistical
+    rf2 = 1 + math.ceil(_logb(2, alpha) * min(m, n)) + math.ceil(_logb(t, alpha)) - r_p  # interpolation
+    rf3 = _logb(2, alpha) * min(m, _logb(p, 2)) - r_p  # Groebner 1
+    rf4 = t - 1 + _logb(2, alpha) * min(m / (t + 1), _logb(p, 2) / 2.0) - r_p  # Groebner 2
+    rf5 = (t - 2 + m / (2 * _logb(alpha, 2)) - r_p) / (t - 1)  # Groebner 3
+    rf_max = max(math.ceil(rf1), math.ceil(rf2), math.ceil(rf3), math.ceil(rf4), math.ceil(rf5))
+    if r_f < rf_max:
+        return False
+    r3 = t // 3
+    over = int((r_f - 1) * t + r_p + r3 + r3 * (r_f / 2.0) + r_p + alpha)
+    under = int(r3 * (r_f / 2.0) + r_p + alpha)
+    # log2 C(over, under) through lgamma (the exact binomial has thousands of digits)
+    binom_log = (math.lgamma(over + 1) - math.lgamma(under + 1) - math.lgamma(over - under + 1)) / math.log(2)
+    return math.ceil(2 * binom_log) >= m
+
+
+def poseidon2_round_numbers(p, t, alpha, m=128):
+    best, min_cost, best_rf = None, float("inf"), 0
+    for r_p in range(1, 500):
+        for r_f in range(4, 100):
+            if r_f % 2 == 0 and _round_bounds_hold(p, t, r_f, r_p, alpha, m):
+                r_f += 2
+                r_p = int(math.ceil(r_p * 1.075))
+                cost = t * r_f + r_p
+                if cost < min_cost or (cost == min_cost and r_f < best_rf):
+                    best, min_cost, best_rf = (r_f, r_p), cost, r_f
+    return best
+
+
+def test_round_numbers_from_security_bounds(oracle):
+    # the S-box degree is the smallest d > 1 with gcd(d, p - 1) = 1 (x^d a permutation)
+    assert min(d for d in range(2, 20) if math.gcd(d, P - 1) == 1) == 3
+    for width in (16, 24):
+        r_f, r_p = poseidon2_round_numbers(P, width, 3)
+        assert r_f == 8
+        ei, et, ci, _ = oracle.poseidon2_constants(width, 1000)
+        assert ei.shape[0] + et.shape[0] == r_f and ci.size == r_p
+    assert poseidon2_round_numbers(P, 16, 3) == (8, 20) and poseidon2_round_numbers(P, 24, 3) == (8, 23)
+    # the same search for BabyBear (p = 15 * 2^27 + 1, whose smallest S-box degree is 7)
+    bb = 15 * (1 << 27) + 1
+    assert min(d for d in range(2, 20) if math.gcd(d, bb - 1) == 1) == 7
+    assert poseidon2_round_numbers(bb, 16, 7) == (8, 13) and poseidon2_round_numbers(bb, 24, 7) == (8, 21)

@@ -23,6 +23,7 @@ from .api import (  # noqa: F401
     StackedProver,
     Whir,
     params,
+    poly_rand,
     prover_bytes_for,
     dft_batch,
     merkle_commit,

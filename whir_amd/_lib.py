@@ -113,6 +113,7 @@ SIGNATURES = [
                                          ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     ("whir_prove_host", ctypes.c_int,
      [_vp, ctypes.POINTER(whir_params), _u32p, _u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
+    ("whir_poly_rand", ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, _u32p]),
     ("whir_device_alloc", ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
     ("whir_device_free", ctypes.c_int, [_vp]),
     ("whir_memcpy_h2d", ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),

@@ -418,6 +418,14 @@ class Whir:
         return pr.result()
 
 
+def poly_rand(seed, k):
+    """Poly::<F>::rand(&mut rng(seed), k) (crates/poly/src/poly.rs:47-52): the
+    polynomial the reference's tests prove for `seed`, as Montgomery words."""
+    out = np.empty(1 << k, np.uint32)
+    check(lib().whir_poly_rand(seed, k, out.ctypes.data_as(_u32p)), "whir_poly_rand")
+    return out
+
+
 def params(k, folding=4, rate=1, initial_reduction=3, soundness=JOHNSON_BOUND, security_level=123, n_points=1):
     return whir_params(k, folding, rate, initial_reduction, soundness, security_level, n_points)
 

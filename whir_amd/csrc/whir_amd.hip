@@ -1389,6 +1389,19 @@ extern "C" int whir_prover_challenger(whir_prover* P, whir_challenger* out) {
   return WHIR_OK;
 }
 
+// ============================ seeded inputs ============================
+
+// Poly::<F>::rand(&mut rng(seed), k) (crates/poly/src/poly.rs:47-52 with
+// crates/common/src/test.rs:1-4): 2^k StandardUniform KoalaBear draws of
+// StdRng::seed_from_u64(seed), as Montgomery words, on the host.
+extern "C" int whir_poly_rand(uint64_t seed, uint32_t k, uint32_t* out) {
+  if (!out || k > 32) return WHIR_EINVAL;
+  whir_host::ChaChaRng r(seed);
+  const uint64_t n = 1ull << k;
+  for (uint64_t i = 0; i < n; ++i) out[i] = r.next_koalabear();
+  return WHIR_OK;
+}
+
 // ============================ memory helpers ============================
 
 extern "C" int whir_device_alloc(size_t bytes, void** out) {
