@@ -34,6 +34,9 @@ The JSON line also carries
     this process may use (its CPU affinity, capped by OMP_NUM_THREADS where set:
     the box's CPU share), median / min / max over 5 runs; its proof bytes are
     compared with the GPU's (bit_exact).
+  * autotune: the tuner's choices for the headline shape's transforms and
+    trees (whir_ctx_set_autotune: candidates timed at workspace creation,
+    before the timed region; the batched and served legs reuse them).
   * batched: 64 independent jobs, variable counts uniform in 16..22 (sizes
     log-uniform), rates 1/2 and 1/4 mixed, folding 4, seeded; proven by the
     proof server (hipGraph-captured proofs on several streams, device-resident
@@ -346,6 +349,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--k", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default NTT / Merkle shapes instead of the tuner's per-size choices")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -374,7 +379,7 @@ def main():
     import whir_amd
     from whir_amd._lib import whir_params
 
-    ctx = whir_amd.Context(0, 1000)
+    ctx = whir_amd.Context(0, 1000, autotune=not args.no_autotune)
     params = whir_params(args.k, FOLDING, RATE, IR, SOUNDNESS, SECURITY, N_POINTS)
     prover = whir_amd.Prover(params, ctx)
     poly = whir_amd.poly_rand(1, args.k)  # Poly::rand(&mut rng(1), k), test.rs:213-219
@@ -480,6 +485,9 @@ def main():
         "roofline": roofline,
         "roofline_dft": roofline_dft,
     }
+    if not args.no_autotune:
+        out["autotune"] = [t for t in ctx.tuning() if t["kind"] == "ntt" and t["n_in"] >= (1 << 12)
+                           or t["kind"] == "merkle" and t["log_rows"] >= 15][:12]
     prover.close()
     if not args.no_batched:
         out["batched"] = batched_leg(ctx, args.slots)

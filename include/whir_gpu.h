@@ -73,6 +73,33 @@ int whir_ctx_set_ntt_tuning(whir_ctx* ctx, int max_log_pass, int log_threads);
  * depend on it. Defaults 4, 2. */
 int whir_ctx_set_ntt_shape(whir_ctx* ctx, int first_pass_log, int log_vec);
 
+/* NTT pass count of this context (no counterpart in the reference): with
+ * two_pass = 1 a transform of 2^17..2^20 rows runs in two passes of up to 1024
+ * rows (three register stages each), with 0 in three passes of <= 256 rows.
+ * Results do not depend on it. Default 1. */
+int whir_ctx_set_ntt_passes(whir_ctx* ctx, int two_pass);
+
+/* Merkle tree shape of this context (no counterpart in the reference, whose
+ * tree runs on CPU threads): two lanes per permutation for leaf layers and
+ * compression layers of at most 2^pair_max_log rows / compressions (0..24), and
+ * LDS-resident cooperative subtrees (5 levels per launch) from
+ * 2^coop_from_log nodes down (5..24). Results do not depend on it. Defaults 15, 15. */
+int whir_ctx_set_merkle_tuning(whir_ctx* ctx, int pair_max_log, int coop_from_log);
+
+/* Autotuner (no counterpart in the reference, whose DFT plan and tree are
+ * fixed, crates/whir/src/test.rs:215-216, crates/merkle/src/poseidon_packed.rs:14):
+ * with enable = 1, each whir_prover_create / _create_stacked on this context
+ * times every candidate NTT configuration (pass count, first-pass radix,
+ * workgroup size, vector width) and Merkle configuration (two-lane threshold,
+ * cooperative-subtree depth) for each transform and tree of its proof shape not
+ * tuned yet, on the new workspace's buffers, and later launches of those sizes
+ * on this context use the fastest. Proof bytes do not depend on it. Default 0. */
+int whir_ctx_set_autotune(whir_ctx* ctx, int enable);
+/* The tuner's choices so far as a JSON array (one object per transform /
+ * tree: candidates timed, the choice, its time and the default's). *len gets
+ * the length without the terminating NUL; buf may be NULL to query it. */
+int whir_ctx_tuning(whir_ctx* ctx, char* buf, size_t cap, size_t* len);
+
 /* ---- component entry points ---- */
 
 /* Replaces TwoAdicSubgroupDft::dft_batch (whir.rs:34): each of the `width`

@@ -76,6 +76,35 @@ def test_dft_batch(gpu_ctx, oracle, n_in, log_h, width):
 NTT_LAYOUTS = [(5, 8, 4, 2), (5, 8, 3, 2), (5, 7, 2, 1), (4, 6, 4, 0), (8, 8, 4, 2), (6, 8, 0, 1)]
 LAYOUT_CASES = [(1 << 12, 13, 16), (1 << 11, 13, 64), (1 << 13, 14, 8), (1 << 10, 12, 4), (1 << 17, 18, 16)]
 
+# two passes of up to 1024 rows (three register stages per pass) against three
+# passes, for the transform sizes where the choice exists (2^17..2^20 rows)
+TWO_PASS_CASES = [(1 << 17, 18, 16), (1 << 18, 19, 4), ((1 << 18) - 5, 19, 16), (1 << 19, 21, 2), (1 << 20, 21, 16),
+                  (1 << 20, 20, 1), (1 << 17, 19, 64)]
+
+
+@pytest.mark.parametrize("two_pass", [0, 1])
+def test_dft_two_pass_plan(oracle, two_pass):
+    import whir_amd
+    from whir_amd._lib import lib
+
+    ctx = whir_amd.Context(0, 1000)
+    try:
+        assert lib().whir_ctx_set_ntt_passes(ctx.ptr, two_pass) == 0
+        for layout in ((8, 8), (8, 7), (8, 6)):
+            assert lib().whir_ctx_set_ntt_tuning(ctx.ptr, *layout) == 0
+            for n_in, log_h, width in TWO_PASS_CASES:
+                if layout != (8, 8) and n_in < (1 << 19):
+                    continue
+                rng = np.random.default_rng(n_in + log_h + width)
+                mat = _rand_field(rng, (n_in, width))
+                print(f"dft two_pass={two_pass} tuning={layout} n_in={n_in} log_h={log_h} width={width}", flush=True)
+                got = whir_amd.dft_batch(mat, log_h, ctx=ctx)
+                padded = np.zeros((1 << log_h, width), np.uint32)
+                padded[:n_in] = mat
+                assert np.array_equal(got, oracle.dft_batch(padded)), (two_pass, layout, n_in, log_h, width)
+    finally:
+        ctx.close()
+
 
 @pytest.mark.parametrize("layout", NTT_LAYOUTS, ids=lambda t: "-".join(map(str, t)))
 def test_dft_pass_layouts(oracle, layout):

@@ -65,6 +65,10 @@ SIGNATURES = [
     ("whir_ctx_destroy", ctypes.c_int, [_vp]),
     ("whir_ctx_set_ntt_tuning", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     ("whir_ctx_set_ntt_shape", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    ("whir_ctx_set_ntt_passes", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("whir_ctx_set_merkle_tuning", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    ("whir_ctx_set_autotune", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("whir_ctx_tuning", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     ("whir_dft_batch", ctypes.c_int, [_vp, _vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp]),
     ("whir_merkle_commit", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp, _u32p, _vp]),
     ("whir_poseidon2_permute", ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint64, _vp]),

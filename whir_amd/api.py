@@ -46,12 +46,25 @@ class DeviceBuffer:
 
 
 class Context:
-    """whir_ctx: device, Poseidon2 constants (seed 1000 by default), twiddles."""
+    """whir_ctx: device, Poseidon2 constants (seed 1000 by default), twiddles.
+    With autotune, workspaces created on it tune their NTT and Merkle shapes."""
 
-    def __init__(self, device=0, poseidon_seed=1000):
+    def __init__(self, device=0, poseidon_seed=1000, autotune=False):
         p = ctypes.c_void_p()
         check(lib().whir_ctx_create(device, poseidon_seed, ctypes.byref(p)), "whir_ctx_create")
         self.ptr = p.value
+        if autotune:
+            check(lib().whir_ctx_set_autotune(self.ptr, 1), "whir_ctx_set_autotune")
+
+    def tuning(self):
+        """The autotuner's choices so far (list of dicts)."""
+        import json
+
+        n = ctypes.c_size_t()
+        check(lib().whir_ctx_tuning(self.ptr, None, 0, ctypes.byref(n)), "whir_ctx_tuning")
+        buf = ctypes.create_string_buffer(n.value + 1)
+        check(lib().whir_ctx_tuning(self.ptr, buf, n.value + 1, ctypes.byref(n)), "whir_ctx_tuning")
+        return json.loads(buf.value.decode())
 
     def close(self):
         if self.ptr:

@@ -72,7 +72,7 @@ struct Vec {
 };
 
 template <int V>
-__device__ __forceinline__ Vec<V> vload(const uint32_t* p) {
+KB_HD Vec<V> vload(const uint32_t* p) {
   Vec<V> r;
   if constexpr (V == 4) {
     const uint4 t = *reinterpret_cast<const uint4*>(p);
@@ -90,13 +90,13 @@ __device__ __forceinline__ Vec<V> vload(const uint32_t* p) {
   return r;
 }
 template <int V>
-__device__ __forceinline__ void vstore(uint32_t* p, const Vec<V>& x) {
+KB_HD void vstore(uint32_t* p, const Vec<V>& x) {
   if constexpr (V == 4) *reinterpret_cast<uint4*>(p) = make_uint4(x.v[0], x.v[1], x.v[2], x.v[3]);
   else if constexpr (V == 2) *reinterpret_cast<uint2*>(p) = make_uint2(x.v[0], x.v[1]);
   else *p = x.v[0];
 }
 template <int V>
-__device__ __forceinline__ void vscale(Vec<V>& x, uint32_t w) {
+KB_HD void vscale(Vec<V>& x, uint32_t w) {
 #pragma unroll
   for (int c = 0; c < V; ++c) x.v[c] = mul(x.v[c], w);
 }
@@ -111,7 +111,7 @@ constexpr int bitrev(int i, int bits) {
 // bit-reversed registers; twiddles w_{2^(s+1)}^j are compile-time constants and
 // j = 0 multiplies are skipped (17 multiplies for R = 16).
 template <int LR, int V>
-__device__ __forceinline__ void dft_regs(Vec<V> (&x)[1 << LR]) {
+KB_HD void dft_regs(Vec<V> (&x)[1 << LR]) {
   constexpr int R = 1 << LR;
   Vec<V> y[R];
 #pragma unroll
@@ -158,7 +158,7 @@ struct NttArgs {
   uint32_t n_pass;
 };
 
-__device__ __forceinline__ uint32_t tw24(const uint32_t* lo, const uint32_t* hi, uint32_t e) {
+KB_HD uint32_t tw24(const uint32_t* lo, const uint32_t* hi, uint32_t e) {
   return mul(lo[e & 4095u], hi[(e >> 12) & 4095u]);
 }
 
@@ -271,6 +271,171 @@ __global__ void __launch_bounds__(256) ntt_kernel(NttArgs a) {
       vstore<V>(p, z[tb]);
     }
   }
+}
+
+// ---- passes of L = R1 R2 R3 rows: three register stages, two LDS exchanges ----
+// A transform of 2^17..2^20 rows in two such passes (L up to 1024) reads and
+// writes the work array once less than in three passes of <= 256 rows. Row
+// index of a line's element: j = j1 T + j2 R3 + j3 (T = R2 R3 threads per
+// line, thread jb = j2 R3 + j3); output t = t1 + R1 t2 + R1 R2 t3:
+//   stage A (thread jb):   DFT_R1 over j1 -> t1, times w_L^(jb t1);
+//   stage B (task t1, j3): DFT_R2 over j2 -> t2, times w_T^(j3 t2);
+//   stage C (task t1, t2): DFT_R3 over j3 -> t3.
+// The stages are host/device functions of (workgroup, thread) with the
+// workgroup barriers between them, so the same code runs in a host emulation.
+template <int LR1, int LR2, int LR3, int V>
+struct Ntt3 {
+  static constexpr int R1 = 1 << LR1, R2 = 1 << LR2, R3 = 1 << LR3;
+  static constexpr int LT = LR2 + LR3, T = 1 << LT;  // threads per line
+  static constexpr int LL = LR1 + LT;                // log2 L
+  static constexpr int KB = (R1 * R3) >> LT;         // stage-B tasks per thread
+  static constexpr int KC = (R1 * R2) >> LT;         // stage-C tasks per thread
+  static constexpr int LV = V == 4 ? 2 : V == 2 ? 1 : 0;
+  static_assert(LR1 >= LR2 && LR1 >= LR3, "stage A is the widest");
+
+  struct Lane {  // the thread's place: line (b, m, col) and its offsets
+    uint32_t cq, jb, b, w, m, col, base, jstride;
+  };
+  static KB_HD Lane lane(const NttArgs& a, uint32_t bid, uint32_t tid) {
+    Lane l;
+    l.cq = tid & ((1u << a.log_qv) - 1);
+    l.jb = tid >> a.log_qv;
+    const uint32_t log_lpb = a.log_m + a.log_b - LV;  // lines per block
+    const uint32_t line = (bid << a.log_qv) + l.cq;
+    l.b = line >> log_lpb;
+    l.w = (line & ((1u << log_lpb) - 1)) << LV;  // m B + col
+    l.m = l.w >> a.log_b;
+    l.col = l.w & ((1u << a.log_b) - 1);
+    l.jstride = 1u << (a.log_m + a.log_b);
+    l.base = (l.b << (a.log_m + LL + a.log_b)) + l.w;
+    return l;
+  }
+  // LDS slot of (row position i, line cq) in Vec<V> units
+  static KB_HD uint32_t slot(const NttArgs& a, uint32_t i, uint32_t cq) { return (i << a.log_qv) + cq; }
+
+  static KB_HD void stage_a(const NttArgs& a, const Lane& l, Vec<V>* lds) {
+    Vec<V> x[R1];
+    if (!a.first) {
+      #pragma unroll
+      for (int j1 = 0; j1 < R1; ++j1) x[j1] = vload<V>(a.src + l.base + (((uint32_t)j1 << LT) + l.jb) * l.jstride);
+    } else {
+      // input row r = j M + m, coset e, input column c; twist w_N^(e r)
+      const uint32_t e = l.col >> a.log_c, c = l.col & ((1u << a.log_c) - 1);
+      uint32_t tw = ONE, st = ONE;
+      if (e) {
+        const uint32_t sh = 24 - a.log_nn;
+        tw = tw24(a.tw_lo, a.tw_hi, (e * ((l.jb << a.log_m) + l.m)) << sh);
+        st = tw24(a.tw_lo, a.tw_hi, (e << (LT + a.log_m)) << sh);
+      }
+      #pragma unroll
+      for (int j1 = 0; j1 < R1; ++j1) {
+        const uint32_t r = ((((uint32_t)j1 << LT) + l.jb) << a.log_m) + l.m;
+        if (r < a.n_in) {
+          x[j1] = vload<V>(a.src + (r << a.log_c) + c);
+        } else {
+          #pragma unroll
+          for (int q = 0; q < V; ++q) x[j1].v[q] = 0u;
+        }
+        if (e) {
+          vscale<V>(x[j1], tw);
+          tw = mul(tw, st);
+        }
+      }
+    }
+    dft_regs<LR1, V>(x);
+    if (l.jb) {  // w_L^(jb t1)
+      const uint32_t w1 = tw24(a.tw_lo, a.tw_hi, l.jb << (24 - LL));
+      uint32_t cur = w1;
+      #pragma unroll
+      for (int t1 = 1; t1 < R1; ++t1) {
+        vscale<V>(x[t1], cur);
+        cur = mul(cur, w1);
+      }
+    }
+    #pragma unroll
+    for (int t1 = 0; t1 < R1; ++t1) lds[slot(a, ((uint32_t)t1 << LT) + l.jb, l.cq)] = x[t1];
+  }
+
+  // stage B, reading half: task q = jb + k T -> (t1, j3); y[k] its R2 outputs
+  static KB_HD void stage_b_read(const NttArgs& a, const Lane& l, const Vec<V>* lds, Vec<V> (&y)[KB][R2]) {
+    #pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const uint32_t q = l.jb + ((uint32_t)k << LT);
+      const uint32_t t1 = q >> LR3, j3 = q & (R3 - 1);
+      #pragma unroll
+      for (int j2 = 0; j2 < R2; ++j2) y[k][j2] = lds[slot(a, (t1 << LT) + ((uint32_t)j2 << LR3) + j3, l.cq)];
+      dft_regs<LR2, V>(y[k]);
+      if (j3) {  // w_T^(j3 t2)
+        const uint32_t w1 = tw24(a.tw_lo, a.tw_hi, j3 << (24 - LT));
+        uint32_t cur = w1;
+        #pragma unroll
+        for (int t2 = 1; t2 < R2; ++t2) {
+          vscale<V>(y[k][t2], cur);
+          cur = mul(cur, w1);
+        }
+      }
+    }
+  }
+  static KB_HD void stage_b_write(const NttArgs& a, const Lane& l, Vec<V>* lds, const Vec<V> (&y)[KB][R2]) {
+    #pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const uint32_t q = l.jb + ((uint32_t)k << LT);
+      const uint32_t t1 = q >> LR3, j3 = q & (R3 - 1);
+      #pragma unroll
+      for (int t2 = 0; t2 < R2; ++t2)
+        lds[slot(a, ((((uint32_t)t2 << LR1) + t1) << LR3) + j3, l.cq)] = y[k][t2];
+    }
+  }
+
+  // stage C: task u = jb + k T = t1 + R1 t2; outputs t = u + R1 R2 t3
+  static KB_HD void stage_c(const NttArgs& a, const Lane& l, const Vec<V>* lds) {
+    uint32_t nat0 = 0;
+    if (a.natural) nat0 = a.n_pass == 2 ? l.b : (l.b >> a.log_l2) + ((l.b & ((1u << a.log_l2) - 1)) << a.log_l1);
+    const uint32_t log_nat_stride = a.n_pass == 2 ? a.log_l1 : a.log_l1 + a.log_l2;
+    const uint32_t log_mprev = a.log_m + LL;
+    #pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const uint32_t u = l.jb + ((uint32_t)k << LT);
+      Vec<V> z[R3];
+      #pragma unroll
+      for (int j3 = 0; j3 < R3; ++j3) z[j3] = lds[slot(a, (u << LR3) + (uint32_t)j3, l.cq)];
+      dft_regs<LR3, V>(z);
+      if (a.twiddle_out && l.m) {  // w_{M_{p-1}}^(m t)
+        const uint32_t sh = 24 - log_mprev;
+        uint32_t cur = tw24(a.tw_lo, a.tw_hi, (l.m * u) << sh);
+        const uint32_t st = tw24(a.tw_lo, a.tw_hi, (l.m << (LR1 + LR2)) << sh);
+        #pragma unroll
+        for (int t3 = 0; t3 < R3; ++t3) {
+          vscale<V>(z[t3], cur);
+          cur = mul(cur, st);
+        }
+      }
+      #pragma unroll
+      for (int t3 = 0; t3 < R3; ++t3) {
+        const uint32_t t = u + ((uint32_t)t3 << (LR1 + LR2));
+        uint32_t* p;
+        if (a.natural) p = a.dst + (((nat0 + (t << log_nat_stride))) << a.log_b) + l.col;
+        else p = a.dst + l.base + t * l.jstride;
+        vstore<V>(p, z[t3]);
+      }
+    }
+  }
+};
+
+template <int LR1, int LR2, int LR3, int V>
+__global__ void __launch_bounds__(512) ntt3_kernel(NttArgs a) {
+  using K = Ntt3<LR1, LR2, LR3, V>;
+  extern __shared__ uint32_t lds_words[];
+  Vec<V>* lds = reinterpret_cast<Vec<V>*>(lds_words);
+  const typename K::Lane l = K::lane(a, blockIdx.x, threadIdx.x);
+  K::stage_a(a, l, lds);
+  __syncthreads();
+  Vec<V> y[K::KB][K::R2];
+  K::stage_b_read(a, l, lds, y);
+  __syncthreads();
+  K::stage_b_write(a, l, lds, y);
+  __syncthreads();
+  K::stage_c(a, l, lds);
 }
 
 }  // namespace kb

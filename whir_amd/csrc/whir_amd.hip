@@ -5,13 +5,18 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <functional>
+#include <map>
 #include <mutex>
+#include <string>
+#include <tuple>
 #include <vector>
 
 #include "field.h"
 #include "poseidon2.h"
 #include "tx.h"
 #include "ntt.h"
+#include "ntt_plan.h"
 #include "merkle.h"
 #include "sumcheck.h"
 #include "protocol.h"
@@ -42,21 +47,54 @@ using whir_host::RoundPlan;
 // from hipGetSymbolAddress (whir_ctx::scratch), never this name.
 static __device__ __align__(256) unsigned char scratch[WHIR_NOALLOC_SCRATCH_BYTES];
 
+// How one NTT runs (results do not depend on it): largest pass 2^max_logl rows,
+// 2^log_threads threads per workgroup; the first pass of a transform of >= 3
+// passes as an LDS-free radix-2^first_log register pass (0: all passes even);
+// 2^max_lv words per lane and line; transforms of 2^17..2^20 rows in two passes
+// of up to 1024 rows (ntt3_kernel) when two_pass is set.
+struct NttConfig {
+  int max_logl = 8, log_threads = 8, first_log = 4, max_lv = 2, two_pass = 0;
+};
+// How one Merkle tree runs: two lanes per permutation (p2pair.h) for leaf
+// layers and compression layers of at most pair_max rows / compressions
+// (measured on MI355X: below ~2^16 one lane's instruction stream is exposed,
+// 2^15 rows of 64 words 55.7 -> 38.2 us, a 2^15-compression layer 15.3 -> 12.0
+// us; from 2^17 up one lane per permutation wins); from coop_from nodes down,
+// cooperative LDS-resident subtrees of up to COOP_MAX_NODES nodes per launch
+// (the latency-bound top of the tree).
+struct MerkleConfig {
+  uint64_t pair_max = 1ull << 15;
+  uint64_t coop_from = 1ull << 15;
+};
+
 struct whir_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   void* scratch = nullptr;  // this device's address of `scratch`
-  // NTT tile shape: largest pass 2^ntt_max_logl rows, 2^ntt_log_threads threads per workgroup
-  int ntt_max_logl = 8;
-  int ntt_log_threads = 8;
-  // first pass of a transform of >= 3 passes as an LDS-free radix-2^ntt_first_log
-  // register pass (0: all passes even); vector width 2^ntt_max_lv words per line
-  int ntt_first_log = 4;
-  int ntt_max_lv = 2;
+  NttConfig ntt;            // defaults (whir_ctx_set_ntt_*)
+  MerkleConfig merkle;      // default (whir_ctx_set_merkle_tuning)
+  // autotuner: per transform (n_in, log N, log C) and per tree (log H, row
+  // words), the configuration measured fastest on this device
+  bool autotune = false;
+  std::mutex tune_mu;
+  std::map<std::tuple<uint64_t, int, int>, NttConfig> ntt_tuned;
+  std::map<std::pair<int, uint32_t>, MerkleConfig> merkle_tuned;
+  std::vector<std::string> tune_log;  // one JSON object per tuned transform / tree
   uint32_t* tw = nullptr;  // tw_lo[4096] || tw_hi[4096]
   const uint32_t* tw_lo() const { return tw; }
   const uint32_t* tw_hi() const { return tw + 4096; }
 };
+
+static NttConfig ntt_config(whir_ctx* c, uint64_t n_in, int log_nn, int log_c) {
+  std::lock_guard<std::mutex> g(c->tune_mu);
+  auto it = c->ntt_tuned.find(std::make_tuple(n_in, log_nn, log_c));
+  return it == c->ntt_tuned.end() ? c->ntt : it->second;
+}
+static MerkleConfig merkle_config(whir_ctx* c, int log_h, uint32_t row_words) {
+  std::lock_guard<std::mutex> g(c->tune_mu);
+  auto it = c->merkle_tuned.find(std::make_pair(log_h, row_words));
+  return it == c->merkle_tuned.end() ? c->merkle : it->second;
+}
 
 static int upload_constants(uint64_t seed) {
   Poseidon2Consts c;
@@ -138,64 +176,40 @@ extern "C" int whir_ctx_destroy(whir_ctx* c) {
 extern "C" int whir_ctx_set_ntt_shape(whir_ctx* c, int first_pass_log, int log_vec) {
   if (!c || !(first_pass_log == 0 || (first_pass_log >= 2 && first_pass_log <= 4)) || log_vec < 0 || log_vec > 2)
     return WHIR_EINVAL;
-  c->ntt_first_log = first_pass_log;
-  c->ntt_max_lv = log_vec;
+  c->ntt.first_log = first_pass_log;
+  c->ntt.max_lv = log_vec;
   return WHIR_OK;
 }
 
 extern "C" int whir_ctx_set_ntt_tuning(whir_ctx* c, int max_log_pass, int log_threads) {
   if (!c || max_log_pass < 1 || max_log_pass > 8 || log_threads < 6 || log_threads > 8) return WHIR_EINVAL;
-  c->ntt_max_logl = max_log_pass;
-  c->ntt_log_threads = log_threads;
+  c->ntt.max_logl = max_log_pass;
+  c->ntt.log_threads = log_threads;
+  return WHIR_OK;
+}
+
+extern "C" int whir_ctx_set_ntt_passes(whir_ctx* c, int two_pass) {
+  if (!c || two_pass < 0 || two_pass > 1) return WHIR_EINVAL;
+  c->ntt.two_pass = two_pass;
+  return WHIR_OK;
+}
+
+extern "C" int whir_ctx_set_merkle_tuning(whir_ctx* c, int pair_max_log, int coop_from_log) {
+  if (!c || pair_max_log < 0 || pair_max_log > 24 || coop_from_log < 5 || coop_from_log > 24) return WHIR_EINVAL;
+  c->merkle.pair_max = 1ull << pair_max_log;
+  c->merkle.coop_from = 1ull << coop_from_log;
+  return WHIR_OK;
+}
+
+extern "C" int whir_ctx_set_autotune(whir_ctx* c, int enable) {
+  if (!c) return WHIR_EINVAL;
+  c->autotune = enable != 0;
   return WHIR_OK;
 }
 
 static hipStream_t pick_stream(whir_ctx* c, void* s) { return s ? (hipStream_t)s : c->stream; }
 
 // ============================ DFT ============================
-
-// Plan of one coset-form transform: N = 2^log_nn output rows of C = 2^log_c
-// columns from n_in input rows (n = 2^log_n >= n_in rows of the work array,
-// B = 2^log_b = (N / n) C columns), P passes of L_p = 2^logl[p] <= 256.
-struct NttPlan {
-  int log_nn = 0, log_c = 0, log_n = 0, log_b = 0, lv = 0, P = 0;
-  int logl[3] = {0, 0, 0};
-  bool reg_first = false;  // pass 0 is ntt_kernel<logl[0], 0, V> (no LDS)
-};
-
-static bool ntt_plan(uint64_t n_in, int log_nn, int log_c, NttPlan& pl, int max_logl, int first_log = 0,
-                     int max_lv = 2) {
-  if (log_nn < 1 || log_nn > 24 || log_c < 0 || log_c > 10 || log_nn + log_c > 31) return false;
-  if (n_in > (1ull << log_nn)) return false;
-  int log_n = 1;
-  while ((1ull << log_n) < n_in) ++log_n;
-  if (log_n > log_nn) return false;
-  pl.log_nn = log_nn;
-  pl.log_c = log_c;
-  pl.log_n = log_n;
-  pl.log_b = log_c + (log_nn - log_n);
-  pl.lv = std::min(std::min(2, max_lv), log_c);
-  pl.reg_first = false;
-  if (first_log > 0 && log_n > 2 * std::min(max_logl, 8) && log_n - first_log <= 16) {
-    // a transform that needs three passes anyway: a register-only first pass,
-    // then two even passes
-    pl.reg_first = true;
-    pl.P = 3;
-    pl.logl[0] = first_log;
-    pl.logl[1] = (log_n - first_log + 1) / 2;
-    pl.logl[2] = log_n - first_log - pl.logl[1];
-    return true;
-  }
-  max_logl = std::max(max_logl, (log_n + 2) / 3);  // at most three passes
-  pl.P = (log_n + max_logl - 1) / max_logl;
-  if (pl.P > 3) return false;
-  int rem = log_n;
-  for (int p = 0; p < pl.P; ++p) {
-    pl.logl[p] = (rem + (pl.P - p) - 1) / (pl.P - p);
-    rem -= pl.logl[p];
-  }
-  return true;
-}
 
 typedef void (*NttKernelFn)(NttArgs);
 
@@ -205,6 +219,15 @@ static NttKernelFn ntt_reg_kernel_for(int logl) {
     case 2: return ntt_kernel<2, 0, V>;
     case 3: return ntt_kernel<3, 0, V>;
     case 4: return ntt_kernel<4, 0, V>;
+    default: return nullptr;
+  }
+}
+
+template <int V>
+static NttKernelFn ntt3_kernel_for(int logl) {
+  switch (logl) {
+    case 9: return ntt3_kernel<3, 3, 3, V>;
+    case 10: return ntt3_kernel<4, 3, 3, V>;
     default: return nullptr;
   }
 }
@@ -226,45 +249,30 @@ static NttKernelFn ntt_kernel_for(int logl) {
 
 // out (N x C, natural rows) = DFT of the n_in-row matrix `in` zero-padded to
 // N = 2^log_nn rows; `tmp` has room for N * C words (used when P >= 2).
+static bool ntt_plan_for(const NttConfig& cfg, uint64_t n_in, int log_nn, int log_c, NttPlan& pl) {
+  return ntt_plan(n_in, log_nn, log_c, pl, cfg.max_logl, cfg.first_log, cfg.max_lv, cfg.two_pass);
+}
+
+// `force`: run with this configuration instead of the tuned or default one.
 static int ntt_enqueue(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_nn, int log_c, uint32_t* out,
-                       uint32_t* tmp, hipStream_t s) {
+                       uint32_t* tmp, hipStream_t s, const NttConfig* force = nullptr) {
+  const NttConfig cfg = force ? *force : ntt_config(c, n_in, log_nn, log_c);
   NttPlan pl;
-  if (!ntt_plan(n_in, log_nn, log_c, pl, c->ntt_max_logl, c->ntt_first_log, c->ntt_max_lv)) return WHIR_EINVAL;
-  int used = 0;
+  if (!ntt_plan_for(cfg, n_in, log_nn, log_c, pl)) return WHIR_EINVAL;
   for (int p = 0; p < pl.P; ++p) {
+    NttPass ps;
+    ntt_pass_setup(pl, p, n_in, cfg.log_threads, in, out, tmp, c->tw_lo(), c->tw_hi(), ps);
     const int l = pl.logl[p];
-    const bool reg = pl.reg_first && p == 0;
-    const int lr2 = reg ? 0 : l / 2;
-    used += l;
-    NttArgs a;
-    memset(&a, 0, sizeof(a));
-    a.tw_lo = c->tw_lo();
-    a.tw_hi = c->tw_hi();
-    a.n_in = (uint32_t)n_in;
-    a.log_c = (uint32_t)log_c;
-    a.log_nn = (uint32_t)log_nn;
-    a.log_b = (uint32_t)pl.log_b;
-    a.log_n = (uint32_t)pl.log_n;
-    a.log_m = (uint32_t)(pl.log_n - used);
-    const int log_lines = pl.log_n + pl.log_b - pl.lv - l;
-    a.log_qv = (uint32_t)std::max(0, std::min(c->ntt_log_threads - lr2, log_lines));
-    a.first = p == 0;
-    a.twiddle_out = p + 1 < pl.P;
-    a.natural = pl.P > 1 && p + 1 == pl.P;
-    a.log_l1 = (uint32_t)pl.logl[0];
-    a.log_l2 = (uint32_t)pl.logl[1];
-    a.n_pass = (uint32_t)pl.P;
-    a.src = p == 0 ? in : tmp;
-    a.dst = p + 1 == pl.P ? out : tmp;
-    const unsigned grid = 1u << (log_lines - (int)a.log_qv);
-    const unsigned threads = 1u << (lr2 + (int)a.log_qv);
-    const size_t lds = reg ? 0 : ((size_t)1 << (l + (int)a.log_qv + pl.lv)) * 4;
-    NttKernelFn fn = reg ? (pl.lv == 2 ? ntt_reg_kernel_for<4>(l) : pl.lv == 1 ? ntt_reg_kernel_for<2>(l)
-                                                                            : ntt_reg_kernel_for<1>(l))
+    NttKernelFn fn = ps.kind == NTT_PASS_REG
+                         ? (pl.lv == 2 ? ntt_reg_kernel_for<4>(l) : pl.lv == 1 ? ntt_reg_kernel_for<2>(l)
+                                                                               : ntt_reg_kernel_for<1>(l))
+                     : ps.kind == NTT_PASS_TRI
+                         ? (pl.lv == 2 ? ntt3_kernel_for<4>(l) : pl.lv == 1 ? ntt3_kernel_for<2>(l)
+                                                                            : ntt3_kernel_for<1>(l))
                          : (pl.lv == 2 ? ntt_kernel_for<4>(l) : pl.lv == 1 ? ntt_kernel_for<2>(l)
-                                                                            : ntt_kernel_for<1>(l));
-    if (!fn) return WHIR_EINVAL;
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(threads), lds, s, a);
+                                                                           : ntt_kernel_for<1>(l));
+    if (!fn || ps.lds > 65536) return WHIR_EINVAL;
+    hipLaunchKernelGGL(fn, dim3(ps.grid), dim3(ps.threads), ps.lds, s, ps.a);
   }
   return WHIR_OK;
 }
@@ -278,7 +286,7 @@ extern "C" int whir_dft_batch(whir_ctx* c, const uint32_t* d_in, uint64_t n_in_r
   hipStream_t s = pick_stream(c, stream);
   uint32_t* tmp = nullptr;
   NttPlan pl;
-  if (!ntt_plan(n_in_rows, (int)log_h, log_c, pl, c->ntt_max_logl, c->ntt_first_log, c->ntt_max_lv))
+  if (!ntt_plan_for(ntt_config(c, n_in_rows, (int)log_h, log_c), n_in_rows, (int)log_h, log_c, pl))
     return WHIR_EINVAL;
   if (pl.P > 1) HIP_TRY(hipMallocAsync((void**)&tmp, ((size_t)width << log_h) * 4, s));
   int rc = ntt_enqueue(c, d_in, n_in_rows, (int)log_h, log_c, d_out, tmp, s);
@@ -289,16 +297,10 @@ extern "C" int whir_dft_batch(whir_ctx* c, const uint32_t* d_in, uint64_t n_in_r
 
 // ============================ Merkle ============================
 
-// Trees of at most this many rows (compressions per layer) use two lanes per
-// permutation (p2pair.h): below ~2^16 one lane's instruction stream is exposed
-// (measured on MI355X: 2^15 rows of 64 words 55.7 -> 38.2 us, a 2^15-compression
-// layer 15.3 -> 12.0 us); from 2^17 up the lane-per-permutation kernels win.
-constexpr uint64_t MERKLE_PAIR_MAX = 1ull << 15;
-
 static int leaves_enqueue(const uint32_t* rows, uint64_t n_rows, uint32_t row_words, uint32_t* leaves,
-                          hipStream_t s) {
+                          hipStream_t s, const MerkleConfig& mc) {
   const dim3 grid((unsigned)((n_rows + MERKLE_THREADS - 1) / MERKLE_THREADS)), block(MERKLE_THREADS);
-  if (n_rows <= MERKLE_PAIR_MAX && row_words % 16 == 0) {
+  if (n_rows <= mc.pair_max && row_words % 16 == 0) {
     const dim3 grid2((unsigned)((2 * n_rows + MERKLE_THREADS - 1) / MERKLE_THREADS));
     switch (row_words) {
 #define LEAF_PAIR_CASE(W) \
@@ -320,24 +322,23 @@ static int leaves_enqueue(const uint32_t* rows, uint64_t n_rows, uint32_t row_wo
 
 // Leaves + every layer into `layers` (2*2^log_h - 1 digests). Root at the end.
 // Wide layers use one compression per lane (throughput-bound); from
-// MERKLE_COOP_FROM nodes down, cooperative subtrees (latency-bound part).
-constexpr uint64_t MERKLE_COOP_FROM = 1ull << 15;
-
+// mc.coop_from nodes down, cooperative subtrees (latency-bound part).
 // With `top` set, the last cooperative launch (the top <= COOP_MAX_NODES nodes
 // down to the root) is left to the caller: *top / *top_n = that layer and its
 // node count (tree_top_claim_kernel runs it with the transcript step).
 static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, uint32_t* layers, hipStream_t s,
-                          hipEvent_t* leaf_ev = nullptr, uint32_t** top = nullptr, uint32_t* top_n = nullptr) {
+                          const MerkleConfig& mc, hipEvent_t* leaf_ev = nullptr, uint32_t** top = nullptr,
+                          uint32_t* top_n = nullptr) {
   const uint64_t H = 1ull << log_h;
   if (leaf_ev) HIP_TRY(hipEventRecord(leaf_ev[0], s));
-  int rc = leaves_enqueue(rows, H, row_words, layers, s);
+  int rc = leaves_enqueue(rows, H, row_words, layers, s, mc);
   if (rc) return rc;
   if (leaf_ev) HIP_TRY(hipEventRecord(leaf_ev[1], s));
   uint32_t* cur = layers;
   uint64_t n = H;
-  while (n > MERKLE_COOP_FROM) {
+  while (n > std::max<uint64_t>(mc.coop_from, COOP_MAX_NODES)) {
     const uint64_t half = n / 2;
-    if (half <= MERKLE_PAIR_MAX)
+    if (half <= mc.pair_max)
       hipLaunchKernelGGL(merkle_layer_pair_kernel, dim3((unsigned)((2 * half + MERKLE_THREADS - 1) / MERKLE_THREADS)),
                          dim3(MERKLE_THREADS), 0, s, cur, half, cur + n * 8);
     else
@@ -369,7 +370,7 @@ extern "C" int whir_merkle_commit(whir_ctx* c, const uint32_t* d_rows, uint32_t 
                                   uint32_t* d_layers, uint32_t* root_host, void* stream) {
   if (!c || !d_rows || !d_layers || log_h > 28) return WHIR_EINVAL;
   hipStream_t s = pick_stream(c, stream);
-  int rc = merkle_enqueue(d_rows, (int)log_h, row_words, d_layers, s);
+  int rc = merkle_enqueue(d_rows, (int)log_h, row_words, d_layers, s, merkle_config(c, (int)log_h, row_words));
   if (rc) return rc;
   HIP_TRY(hipGetLastError());
   if (root_host) {
@@ -792,14 +793,214 @@ static whir_prover* prover_relayout(whir_prover* P, const whir_params* prm, std:
   return Q;
 }
 
+// ============================ autotuner ============================
+// At workspace creation (whir_ctx_set_autotune), every transform and tree of
+// the proof shape that the context has not tuned yet is run in each candidate
+// configuration on the workspace's own buffers; the fastest (minimum of three
+// timed runs, HIP events on the context's stream) is used from then on by every
+// launch of that size. Candidates differ only in how the work is laid out, so
+// every one gives the same words (tests/test_gpu_autotune.py).
+
+static double time_min_ms(hipStream_t s, const std::function<int()>& run, int reps = 3) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    (void)hipGetLastError();
+    if (e0) (void)hipEventDestroy(e0);
+    return -1.0;
+  }
+  double best = -1.0;
+  if (run() == WHIR_OK) {  // warm-up
+    for (int r = 0; r < reps; ++r) {
+      float ms = 0.f;
+      if (hipEventRecord(e0, s) != hipSuccess || run() != WHIR_OK || hipEventRecord(e1, s) != hipSuccess ||
+          hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        (void)hipGetLastError();
+        best = -1.0;
+        break;
+      }
+      if (best < 0 || ms < best) best = ms;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+
+// One candidate per distinct launch sequence (plan, kernels, workgroup shape).
+static std::vector<NttConfig> ntt_candidates(whir_ctx* c, uint64_t n_in, int log_nn, int log_c) {
+  std::vector<NttConfig> out;
+  std::vector<std::string> seen;
+  for (int two : {0, 1})
+    for (int fl : {4, 3, 0})
+      for (int lt : {8, 7})
+        for (int lv : {2, 1}) {
+          NttConfig cfg;
+          cfg.two_pass = two;
+          cfg.first_log = fl;
+          cfg.log_threads = lt;
+          cfg.max_lv = lv;
+          NttPlan pl;
+          if (!ntt_plan_for(cfg, n_in, log_nn, log_c, pl)) continue;
+          std::string sig;
+          bool ok = true;
+          for (int p = 0; p < pl.P; ++p) {
+            NttPass ps;
+            ntt_pass_setup(pl, p, n_in, cfg.log_threads, nullptr, nullptr, nullptr, c->tw_lo(), c->tw_hi(), ps);
+            ok &= ps.lds <= 65536;
+            sig += std::to_string(ps.kind) + "," + std::to_string(pl.logl[p]) + "," + std::to_string(pl.lv) + "," +
+                   std::to_string(ps.threads) + ";";
+          }
+          if (!ok || std::find(seen.begin(), seen.end(), sig) != seen.end()) continue;
+          seen.push_back(sig);
+          out.push_back(cfg);
+        }
+  return out;
+}
+
+static std::string merkle_signature(int log_h, uint32_t row_words, const MerkleConfig& mc) {
+  const uint64_t H = 1ull << log_h;
+  std::string sig = (H <= mc.pair_max && row_words % 16 == 0) ? "P" : "L";
+  uint64_t n = H;
+  while (n > std::max<uint64_t>(mc.coop_from, COOP_MAX_NODES)) {
+    sig += (n / 2 <= mc.pair_max) ? 'p' : 'l';
+    n /= 2;
+  }
+  return sig + "|" + std::to_string(n);
+}
+
+static std::vector<MerkleConfig> merkle_candidates(int log_h, uint32_t row_words) {
+  std::vector<MerkleConfig> out;
+  std::vector<std::string> seen;
+  for (int pm : {15, 14, 16, 13})
+    for (int cf : {15, 13, 14, 16, 17}) {
+      MerkleConfig mc;
+      mc.pair_max = 1ull << pm;
+      mc.coop_from = 1ull << cf;
+      const std::string sig = merkle_signature(log_h, row_words, mc);
+      if (std::find(seen.begin(), seen.end(), sig) != seen.end()) continue;
+      seen.push_back(sig);
+      out.push_back(mc);
+    }
+  return out;
+}
+
+// Transforms and trees below these sizes are launch-bound: not tuned.
+constexpr uint64_t TUNE_MIN_NTT_WORDS = 1ull << 16;
+constexpr int TUNE_MIN_TREE_LOG = 12;
+
+static void tune_ntt(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_nn, int log_c, uint32_t* out,
+                     uint32_t* tmp) {
+  const auto key = std::make_tuple(n_in, log_nn, log_c);
+  {
+    std::lock_guard<std::mutex> g(c->tune_mu);
+    if (c->ntt_tuned.count(key)) return;
+  }
+  if (((uint64_t)1 << (log_nn + log_c)) < TUNE_MIN_NTT_WORDS) return;
+  const std::vector<NttConfig> cand = ntt_candidates(c, n_in, log_nn, log_c);
+  if (cand.size() < 2) return;
+  NttConfig best = c->ntt;
+  double best_ms = -1.0, default_ms = -1.0;
+  for (const NttConfig& cfg : cand) {
+    const double ms = time_min_ms(c->stream, [&] { return ntt_enqueue(c, in, n_in, log_nn, log_c, out, tmp, c->stream, &cfg); });
+    if (ms < 0) continue;
+    if (cfg.two_pass == c->ntt.two_pass && cfg.first_log == c->ntt.first_log && cfg.log_threads == c->ntt.log_threads &&
+        cfg.max_lv == c->ntt.max_lv)
+      default_ms = ms;
+    if (best_ms < 0 || ms < best_ms) {
+      best_ms = ms;
+      best = cfg;
+    }
+  }
+  if (best_ms < 0) return;
+  char buf[320];
+  snprintf(buf, sizeof(buf),
+           "{\"kind\": \"ntt\", \"n_in\": %llu, \"log_rows\": %d, \"log_cols\": %d, \"candidates\": %zu, "
+           "\"two_pass\": %d, \"first_pass_log\": %d, \"log_threads\": %d, \"log_vec\": %d, \"best_us\": %.1f, "
+           "\"default_us\": %.1f}",
+           (unsigned long long)n_in, log_nn, log_c, cand.size(), best.two_pass, best.first_log, best.log_threads,
+           best.max_lv, best_ms * 1e3, default_ms * 1e3);
+  std::lock_guard<std::mutex> g(c->tune_mu);
+  c->ntt_tuned[key] = best;
+  c->tune_log.push_back(buf);
+}
+
+static void tune_merkle(whir_ctx* c, const uint32_t* rows, int log_h, uint32_t row_words, uint32_t* layers) {
+  const auto key = std::make_pair(log_h, row_words);
+  {
+    std::lock_guard<std::mutex> g(c->tune_mu);
+    if (c->merkle_tuned.count(key)) return;
+  }
+  if (log_h < TUNE_MIN_TREE_LOG) return;
+  const std::vector<MerkleConfig> cand = merkle_candidates(log_h, row_words);
+  if (cand.size() < 2) return;
+  MerkleConfig best = c->merkle;
+  double best_ms = -1.0, default_ms = -1.0;
+  for (const MerkleConfig& mc : cand) {
+    const double ms = time_min_ms(c->stream, [&] { return merkle_enqueue(rows, log_h, row_words, layers, c->stream, mc); });
+    if (ms < 0) continue;
+    if (mc.pair_max == c->merkle.pair_max && mc.coop_from == c->merkle.coop_from) default_ms = ms;
+    if (best_ms < 0 || ms < best_ms) {
+      best_ms = ms;
+      best = mc;
+    }
+  }
+  if (best_ms < 0) return;
+  int pm = 0, cf = 0;
+  while ((1ull << pm) < best.pair_max) ++pm;
+  while ((1ull << cf) < best.coop_from) ++cf;
+  char buf[320];
+  snprintf(buf, sizeof(buf),
+           "{\"kind\": \"merkle\", \"log_rows\": %d, \"row_words\": %u, \"candidates\": %zu, "
+           "\"pair_max_log\": %d, \"coop_from_log\": %d, \"best_us\": %.1f, \"default_us\": %.1f}",
+           log_h, row_words, cand.size(), pm, cf, best_ms * 1e3, default_ms * 1e3);
+  std::lock_guard<std::mutex> g(c->tune_mu);
+  c->merkle_tuned[key] = best;
+  c->tune_log.push_back(buf);
+}
+
+// Every transform and tree of P's proof shape, timed on P's buffers (their
+// contents are scratch until a proof runs).
+static void prover_autotune(whir_prover* P) {
+  whir_ctx* c = P->ctx;
+  const Plan& pl = P->plan;
+  const int k = pl.k, f = pl.f;
+  const uint32_t W = 1u << f;
+  const int log_h0 = k + pl.rate - f;
+  tune_ntt(c, P->input, 1ull << (k - f), log_h0, f, P->cw0, P->ntt_tmp);
+  tune_merkle(c, P->cw0, log_h0, W, P->tree0);
+  for (const RoundPlan& rp : pl.rounds) {
+    tune_ntt(c, (const uint32_t*)P->ep[0], 1ull << (rp.kc - f), rp.log_rows, f + 2, P->cw[0], P->ntt_tmp);
+    tune_merkle(c, P->cw[0], rp.log_rows, 4 * W, P->tree[0]);
+  }
+  (void)hipGetLastError();
+}
+
+extern "C" int whir_ctx_tuning(whir_ctx* c, char* buf, size_t cap, size_t* len) {
+  if (!c) return WHIR_EINVAL;
+  std::string js = "[";
+  {
+    std::lock_guard<std::mutex> g(c->tune_mu);
+    for (size_t i = 0; i < c->tune_log.size(); ++i) js += (i ? ", " : "") + c->tune_log[i];
+  }
+  js += "]";
+  if (len) *len = js.size();
+  if (!buf || cap < js.size() + 1) return buf ? WHIR_ETRANSCRIPT : WHIR_OK;
+  memcpy(buf, js.c_str(), js.size() + 1);
+  return WHIR_OK;
+}
+
 extern "C" int whir_prover_create(whir_ctx* c, const whir_params* prm, whir_prover** out) {
-  return prover_create(c, prm, nullptr, out);
+  int rc = prover_create(c, prm, nullptr, out);
+  if (rc == WHIR_OK && c->autotune) prover_autotune(*out);
+  return rc;
 }
 
 extern "C" int whir_prover_create_stacked(whir_ctx* c, const whir_params* prm, const whir_stack* st,
                                           whir_prover** out) {
   if (!st) return WHIR_EINVAL;
-  return prover_create(c, prm, st, out);
+  int rc = prover_create(c, prm, st, out);
+  if (rc == WHIR_OK && c->autotune) prover_autotune(*out);
+  return rc;
 }
 
 extern "C" int64_t whir_prover_stack_offset(const whir_prover* P, int32_t id) {
@@ -915,8 +1116,8 @@ static int enqueue_commit(whir_prover* P, const uint32_t* poly, hipStream_t s, c
   if (P->timers) HIP_TRY(hipEventRecord(P->ev[WHIR_TIMER_DFT_BASE][1], s));
   uint32_t* top = nullptr;
   uint32_t top_n = 0;
-  if ((rc = merkle_enqueue(P->cw0, log_h0, W, P->tree0, s, P->timers ? P->ev[WHIR_TIMER_LEAVES_BASE] : nullptr, &top,
-                           &top_n)))
+  if ((rc = merkle_enqueue(P->cw0, log_h0, W, P->tree0, s, merkle_config(c, log_h0, W),
+                           P->timers ? P->ev[WHIR_TIMER_LEAVES_BASE] : nullptr, &top, &top_n)))
     return rc;
   P->timer_pending = P->timers;
   // the tree's top levels, transcript.write(root) and the first group's draw
@@ -985,7 +1186,9 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
     if ((rc = ntt_enqueue(c, (const uint32_t*)P->ep[cur], 1ull << (rp.kc - f), rp.log_rows, f + 2, cw, P->ntt_tmp, s)))
       return rc;
     const bool small = r > 0 && small_round_fits(rp.kc, rp.n_ood, rp.n_stir);
-    if ((rc = merkle_enqueue(cw, rp.log_rows, 4 * W, tree, s, nullptr, small ? nullptr : &top, &top_n))) return rc;
+    if ((rc = merkle_enqueue(cw, rp.log_rows, 4 * W, tree, s, merkle_config(c, rp.log_rows, 4 * W), nullptr,
+                             small ? nullptr : &top, &top_n)))
+      return rc;
     const uint64_t Hr = 1ull << rp.log_rows;
     const uint32_t* root = tree + (2 * Hr - 2) * 8;
     if (small) {
