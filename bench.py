@@ -38,11 +38,13 @@ The JSON line also carries
     trees (whir_ctx_set_autotune: candidates timed at workspace creation,
     before the timed region; the batched and served legs reuse them).
   * batched: 64 independent jobs, variable counts uniform in 16..22 (sizes
-    log-uniform), rates 1/2 and 1/4 mixed, folding 4, seeded; proven by the
-    proof server (hipGraph-captured proofs on several streams, device-resident
-    workspaces) against the same 64 jobs run one by one on the plain path
-    (eager launches, one stream, workspaces cached per shape). Proof bytes of
-    every job are compared between the two.
+    log-uniform), rates 1/2 and 1/4 mixed, folding 4, seeded; grouped by shape
+    into lockstep batches (every kernel launched once per batch, one grid
+    slice per job) running side by side on 12 streams, the whole set captured
+    as one hipGraph (whir_multi); against the same 64 jobs one by one on the
+    plain path (eager launches, one stream, workspaces cached per shape), with
+    the proof server (one captured graph per job over several streams) beside
+    them. Proof bytes of every job are compared between all three.
   * serving: Poisson arrivals of jobs with 16..24 variables, rate 1/2 or 1/4,
     1..4 evaluation claims, seeded; offered at three loads (fractions of the
     plain path's measured capacity on this job mix); proofs/s and p50/p99
@@ -186,9 +188,11 @@ def _drain(srv, specs, bufs):
     return [srv.take(j, 1 << 22)[0] for j in ids]
 
 
-def batched_leg(ctx, slots, reps=3, n_jobs=64, seed=4):
-    """64 independent proofs of mixed sizes: server (overlapped, graph-captured)
-    versus the plain path job by job."""
+def batched_leg(ctx, slots, reps=3, n_jobs=64, seed=4, streams=12):
+    """64 independent proofs of mixed sizes: grouped by shape into lockstep
+    batches side by side, the whole set one captured hipGraph (whir_multi);
+    beside it the proof server (one graph per job over `slots` streams) and the
+    plain path job by job."""
     import whir_amd
 
     rng = np.random.default_rng(seed)
@@ -198,9 +202,16 @@ def batched_leg(ctx, slots, reps=3, n_jobs=64, seed=4):
     bufs = _upload_jobs(specs, lambda i, s: synthetic_poly(1000 + i, s.k))
     plain = PlainPath(ctx)
     srv = whir_amd.Server(slots, 64 << 30, ctx=ctx)
+    multi = whir_amd.MultiBatch(specs, streams=streams, ctx=ctx, graph=1)
     ref = [plain.prove(p, b.ptr) for p, b in zip(specs, bufs)]  # warm-up: workspaces
     _drain(srv, specs, bufs)  # warm-up: workspaces + graph capture
-    t_plain, t_srv, same = [], [], True
+
+    def run_multi():
+        multi.enqueue([b.ptr for b in bufs])
+        return [multi.result(i)[0] for i in range(n_jobs)]
+
+    run_multi()  # warm-up: graph capture
+    t_plain, t_srv, t_multi, same = [], [], [], True
     for _ in range(reps):
         whir_amd.lib().whir_device_synchronize()
         t0 = time.perf_counter()
@@ -211,18 +222,27 @@ def batched_leg(ctx, slots, reps=3, n_jobs=64, seed=4):
         got = _drain(srv, specs, bufs)
         t_srv.append(time.perf_counter() - t0)
         same &= got == ref
+        t0 = time.perf_counter()
+        got = run_multi()
+        t_multi.append(time.perf_counter() - t0)
+        same &= got == ref
+    multi.close()
     srv.close()
     plain.close()
-    tp, ts = min(t_plain), min(t_srv)
+    tp, ts, tm = min(t_plain), min(t_srv), min(t_multi)
     return {
         "workload": f"{n_jobs} jobs, k uniform in [16,22] (seed {seed}), rates 1/2+1/4 mixed, folding 4, 1 claim",
         "jobs_by_k": {int(k): int((ks == k).sum()) for k in sorted(set(ks.tolist()))},
-        "server_slots": slots,
+        "method": (f"lockstep batches per shape ({len(set((int(k), int(r)) for k, r in zip(ks, rates)))} shapes), "
+                   f"side by side on {streams} streams, the whole set one captured hipGraph"),
         "plain_proofs_per_s": round(n_jobs / tp, 1),
-        "batched_proofs_per_s": round(n_jobs / ts, 1),
-        "speedup": round(tp / ts, 3),
+        "batched_proofs_per_s": round(n_jobs / tm, 1),
+        "speedup": round(tp / tm, 3),
         "plain_ms": round(tp * 1e3, 2),
-        "batched_ms": round(ts * 1e3, 2),
+        "batched_ms": round(tm * 1e3, 2),
+        "server_slots": slots,
+        "server_proofs_per_s": round(n_jobs / ts, 1),
+        "server_ms": round(ts * 1e3, 2),
         "reps": reps,
         "bit_identical": bool(same),
     }

@@ -327,6 +327,49 @@ int whir_server_stats(const whir_server* server, int* running, int* queued, uint
  * words) receives the polynomial the reference's tests prove for that seed. */
 int whir_poly_rand(uint64_t seed, uint32_t k, uint32_t* out);
 
+/* ---- lockstep batches of same-shape proofs ----
+ * No counterpart in the reference (its tests prove one polynomial at a time).
+ * A batch holds `jobs` workspaces of one proof shape and proves all of them as
+ * one kernel sequence: every kernel is launched once for the batch, one slice
+ * of its grid per job (the launches of a single proof for the whole batch, the
+ * transcripts' serial chains side by side). Each job's proof bytes are those of
+ * the job proven alone. The inputs are the batch's own device buffers
+ * (whir_batch_input) or are copied from d_polys. */
+typedef struct whir_batch whir_batch;
+int whir_batch_create(whir_ctx* ctx, const whir_params* params, int jobs, whir_batch** out);
+int whir_batch_destroy(whir_batch* batch);
+/* Device buffer of 2^k words holding job `job`'s polynomial. */
+uint32_t* whir_batch_input(whir_batch* batch, int job);
+/* Capture the batch's proof as a hipGraph on the first enqueue, replay it after. */
+int whir_batch_set_graph(whir_batch* batch, int enable);
+/* Enqueues every job's proof on `stream` (d_polys: NULL, or per job a device
+ * polynomial copied into the job's input first, NULL entries keep the input). */
+int whir_batch_enqueue(whir_batch* batch, const uint32_t* const* d_polys, void* stream);
+/* WHIR_OK once the last enqueued batch has finished, WHIR_EBUSY while running. */
+int whir_batch_query(whir_batch* batch);
+/* Job `job`'s proof, its checkpoint draw and commitment root (waits). */
+int whir_batch_result(whir_batch* batch, int job, uint8_t* proof, size_t cap, size_t* len, uint32_t* checkpoint,
+                      uint32_t* root8);
+size_t whir_batch_proof_capacity(const whir_batch* batch);
+size_t whir_batch_device_bytes(const whir_batch* batch);
+
+/* Many independent jobs of mixed shapes (no counterpart in the reference): the
+ * jobs (params[0..n_jobs), one claim count each) are grouped by shape into
+ * lockstep batches, the batches run side by side on up to `streams` internal
+ * streams, forked from and joined back into the caller's stream. Graph mode 1
+ * captures the whole set once as one hipGraph (each enqueue is one graph
+ * launch); mode 2 captures each batch as its own graph, launched on its
+ * stream; 0 launches eagerly. Job i's bytes are those of job i proven alone. */
+typedef struct whir_multi whir_multi;
+int whir_multi_create(whir_ctx* ctx, const whir_params* params, int n_jobs, int streams, whir_multi** out);
+int whir_multi_destroy(whir_multi* multi);
+uint32_t* whir_multi_input(whir_multi* multi, int job);
+int whir_multi_set_graph(whir_multi* multi, int mode);
+int whir_multi_enqueue(whir_multi* multi, const uint32_t* const* d_polys, void* stream);
+int whir_multi_result(whir_multi* multi, int job, uint8_t* proof, size_t cap, size_t* len, uint32_t* checkpoint,
+                      uint32_t* root8);
+size_t whir_multi_proof_capacity(const whir_multi* multi, int job);
+
 /* ---- device memory helpers (plain HIP allocations) ---- */
 int whir_device_alloc(size_t bytes, void** out);
 int whir_device_free(void* ptr);

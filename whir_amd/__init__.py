@@ -15,9 +15,11 @@ raise.
 from ._lib import WhirError, lib, library_path  # noqa: F401
 from .api import (  # noqa: F401
     CAPACITY_BOUND,
+    Batch,
     JOHNSON_BOUND,
     UNIQUE_DECODING,
     Context,
+    MultiBatch,
     Prover,
     Server,
     StackedProver,

@@ -117,6 +117,25 @@ SIGNATURES = [
                                          ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]),
     ("whir_prove_host", ctypes.c_int,
      [_vp, ctypes.POINTER(whir_params), _u32p, _u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
+    ("whir_batch_create", ctypes.c_int, [_vp, ctypes.POINTER(whir_params), ctypes.c_int, ctypes.POINTER(_vp)]),
+    ("whir_batch_destroy", ctypes.c_int, [_vp]),
+    ("whir_batch_input", _vp, [_vp, ctypes.c_int]),
+    ("whir_batch_set_graph", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("whir_batch_enqueue", ctypes.c_int, [_vp, ctypes.POINTER(_vp), _vp]),
+    ("whir_batch_query", ctypes.c_int, [_vp]),
+    ("whir_batch_result", ctypes.c_int,
+     [_vp, ctypes.c_int, _u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
+    ("whir_batch_proof_capacity", ctypes.c_size_t, [_vp]),
+    ("whir_batch_device_bytes", ctypes.c_size_t, [_vp]),
+    ("whir_multi_create", ctypes.c_int,
+     [_vp, ctypes.POINTER(whir_params), ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    ("whir_multi_destroy", ctypes.c_int, [_vp]),
+    ("whir_multi_input", _vp, [_vp, ctypes.c_int]),
+    ("whir_multi_set_graph", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("whir_multi_enqueue", ctypes.c_int, [_vp, ctypes.POINTER(_vp), _vp]),
+    ("whir_multi_result", ctypes.c_int,
+     [_vp, ctypes.c_int, _u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
+    ("whir_multi_proof_capacity", ctypes.c_size_t, [_vp, ctypes.c_int]),
     ("whir_poly_rand", ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, _u32p]),
     ("whir_device_alloc", ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
     ("whir_device_free", ctypes.c_int, [_vp]),

@@ -401,6 +401,105 @@ class StackedProver(Prover):
         check(lib().whir_prover_enqueue_stacked(self.ptr, ptrs, stream), "whir_prover_enqueue_stacked")
 
 
+class Batch:
+    """whir_batch: `jobs` proofs of one shape as one lockstep kernel sequence
+    (every launch covers all jobs; each job's bytes are its bytes alone)."""
+
+    def __init__(self, params, jobs, ctx=None, graph=False):
+        self.ctx = ctx or default_context()
+        self.params = params
+        self.jobs = int(jobs)
+        p = ctypes.c_void_p()
+        check(lib().whir_batch_create(self.ctx.ptr, ctypes.byref(params), self.jobs, ctypes.byref(p)),
+              "whir_batch_create")
+        self.ptr = p.value
+        self.capacity = lib().whir_batch_proof_capacity(self.ptr)
+        if graph:
+            check(lib().whir_batch_set_graph(self.ptr, 1), "whir_batch_set_graph")
+
+    def input_ptr(self, job):
+        return lib().whir_batch_input(self.ptr, job)
+
+    def upload(self, job, poly):
+        a = _u32(poly)
+        assert a.size == 1 << self.params.k
+        check(lib().whir_memcpy_h2d(self.input_ptr(job), a.ctypes.data, a.nbytes), "whir_memcpy_h2d")
+
+    def enqueue(self, d_polys=None, stream=None):
+        ptrs = None if d_polys is None else (ctypes.c_void_p * self.jobs)(*d_polys)
+        check(lib().whir_batch_enqueue(self.ptr, ptrs, stream), "whir_batch_enqueue")
+
+    def done(self):
+        rc = lib().whir_batch_query(self.ptr)
+        if rc not in (0, 7):
+            check(rc, "whir_batch_query")
+        return rc == 0
+
+    def result(self, job):
+        buf = np.zeros(self.capacity, np.uint8)
+        ln = ctypes.c_size_t()
+        cp = np.zeros(1, np.uint32)
+        root = np.zeros(8, np.uint32)
+        check(lib().whir_batch_result(self.ptr, job, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
+                                      cp.ctypes.data_as(_u32p), root.ctypes.data_as(_u32p)), "whir_batch_result")
+        return bytes(buf[: ln.value]), int(cp[0]), root
+
+    def close(self):
+        if self.ptr:
+            lib().whir_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MultiBatch:
+    """whir_multi: independent jobs of mixed shapes, grouped by shape into
+    lockstep batches that run side by side on `streams` streams (graph: the
+    whole set as one captured hipGraph)."""
+
+    def __init__(self, params_list, streams=8, ctx=None, graph=1):
+        """graph: 0 eager, 1 one captured graph for the whole set, 2 one per batch."""
+        self.ctx = ctx or default_context()
+        self.n = len(params_list)
+        arr = (whir_params * self.n)(*params_list)
+        p = ctypes.c_void_p()
+        check(lib().whir_multi_create(self.ctx.ptr, arr, self.n, streams, ctypes.byref(p)), "whir_multi_create")
+        self.ptr = p.value
+        check(lib().whir_multi_set_graph(self.ptr, int(graph)), "whir_multi_set_graph")
+
+    def input_ptr(self, job):
+        return lib().whir_multi_input(self.ptr, job)
+
+    def enqueue(self, d_polys=None, stream=None):
+        ptrs = None if d_polys is None else (ctypes.c_void_p * self.n)(*d_polys)
+        check(lib().whir_multi_enqueue(self.ptr, ptrs, stream), "whir_multi_enqueue")
+
+    def result(self, job):
+        cap = lib().whir_multi_proof_capacity(self.ptr, job)
+        buf = np.zeros(cap, np.uint8)
+        ln = ctypes.c_size_t()
+        cp = np.zeros(1, np.uint32)
+        root = np.zeros(8, np.uint32)
+        check(lib().whir_multi_result(self.ptr, job, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
+                                      cp.ctypes.data_as(_u32p), root.ctypes.data_as(_u32p)), "whir_multi_result")
+        return bytes(buf[: ln.value]), int(cp[0]), root
+
+    def close(self):
+        if self.ptr:
+            lib().whir_multi_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Whir:
     """Mirror of ``Whir::new`` (crates/whir/src/whir.rs:103-125) with the
     reference's Poseidon2 trees and transcript built in."""

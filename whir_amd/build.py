@@ -34,18 +34,20 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 
-# kernels whose per-lane VALU instruction count bench.py needs
+# kernels whose per-lane VALU instruction count bench.py needs (mangled-name
+# prefixes: kernel name and template arguments, any parameter list)
 COUNTED = {
-    "leaves16": "_ZN2kb20merkle_leaves_kernelILi16EEEvPKjmPj",
-    "leaves64": "_ZN2kb20merkle_leaves_kernelILi64EEEvPKjmPj",
-    "layer": "_ZN2kb19merkle_layer_kernelEPKjmPj",
+    "leaves16": "_ZN2kb20merkle_leaves_kernelILi16EEEv",
+    "leaves64": "_ZN2kb20merkle_leaves_kernelILi64EEEv",
+    "layer": "_ZN2kb19merkle_layer_kernelE",
 }
 
 
-def count_valu(asm, symbol):
-    i = asm.find(symbol + ":")
-    if i < 0:
+def count_valu(asm, prefix):
+    m = re.search(r"^(" + re.escape(prefix) + r"\w*):", asm, re.M)
+    if not m:
         return None
+    i = m.start()
     j = asm.find(".Lfunc_end", i)
     n = 0
     for line in asm[i:j].splitlines():

@@ -305,7 +305,7 @@ extern "C" int whir_combine_eqs(whir_ctx* c, uint32_t* d_w, uint32_t k, const ui
   int rc = eq_split_enqueue(d_points, n, k, nullptr, alpha, s, &pool, m0, m1, stride);
   if (rc) return rc;
   const uint64_t N = 1ull << k;
-  launch_combine(s, (Ext*)d_w, N, m0, pool, n, stride, nullptr, nullptr, nullptr, m1);
+  launch_combine(s, 1u, (Ext*)d_w, N, m0, pool, n, stride, nullptr, nullptr, nullptr, m1);
   HIP_TRY(hipFreeAsync(pool, s));
   HIP_TRY(hipGetLastError());
   return WHIR_OK;
@@ -330,7 +330,7 @@ extern "C" int whir_combine_pows(whir_ctx* c, uint32_t* d_w, uint32_t k, const u
   hipLaunchKernelGGL(pow_tables_kernel, dim3(blocks_for((uint64_t)n * ((1ull << m0) + (1ull << m1)))),
                      dim3(SC_THREADS), 0, s, d_vars, (const Ext*)scales, (const uint32_t*)count, m0, m1, powL, powR);
   const uint64_t N = 1ull << k;
-  launch_combine(s, (Ext*)d_w, N, m0, nullptr, 0u, 0, powL, powR, count, m1);
+  launch_combine(s, 1u, (Ext*)d_w, N, m0, nullptr, 0u, 0, powL, powR, count, m1);
   HIP_TRY(hipFreeAsync(powR, s));
   HIP_TRY(hipFreeAsync(powL, s));
   HIP_TRY(hipFreeAsync(count, s));

@@ -7,6 +7,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include "job.h"
 
 #define KB_HD __host__ __device__ __forceinline__
 

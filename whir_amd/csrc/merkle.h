@@ -17,8 +17,10 @@ constexpr int MERKLE_THREADS = 256;
 
 // One sponge per lane; ROW_WORDS is the flattened row length (base words).
 template <int ROW_WORDS>
-__global__ void __launch_bounds__(MERKLE_THREADS) merkle_leaves_kernel(const uint32_t* __restrict__ rows, uint64_t n_rows,
-                                                                   uint32_t* __restrict__ leaves) {
+__global__ void __launch_bounds__(MERKLE_THREADS) merkle_leaves_kernel(JP<const uint32_t> rows_, uint64_t n_rows,
+                                                                   JP<uint32_t> leaves_) {
+  const uint32_t* __restrict__ rows = rows_;
+  uint32_t* __restrict__ leaves = leaves_;
   const uint64_t i = (uint64_t)blockIdx.x * MERKLE_THREADS + threadIdx.x;
   if (i >= n_rows) return;
   const uint32_t* row = rows + i * ROW_WORDS;
@@ -53,8 +55,10 @@ __global__ void __launch_bounds__(MERKLE_THREADS) merkle_leaves_kernel(const uin
 }
 
 // parent[j] = C(prev[j], prev[j + half]) for j < half.
-__global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_kernel(const uint32_t* __restrict__ prev, uint64_t half,
-                                                                  uint32_t* __restrict__ next) {
+__global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_kernel(JP<const uint32_t> prev_, uint64_t half,
+                                                                  JP<uint32_t> next_) {
+  const uint32_t* __restrict__ prev = prev_;
+  uint32_t* __restrict__ next = next_;
   const uint64_t j = (uint64_t)blockIdx.x * MERKLE_THREADS + threadIdx.x;
   if (j >= half) return;
   uint32_t l[8], r[8], o[8];
@@ -73,9 +77,11 @@ __global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_kernel(const uint
 // lane 1 words 12..15 (its words 0..3; its words 4..11 are the capacity).
 // ROW_WORDS is a multiple of 16.
 template <int ROW_WORDS>
-__global__ void __launch_bounds__(MERKLE_THREADS) merkle_leaves_pair_kernel(const uint32_t* __restrict__ rows,
+__global__ void __launch_bounds__(MERKLE_THREADS) merkle_leaves_pair_kernel(JP<const uint32_t> rows_,
                                                                         uint64_t n_rows,
-                                                                        uint32_t* __restrict__ leaves) {
+                                                                        JP<uint32_t> leaves_) {
+  const uint32_t* __restrict__ rows = rows_;
+  uint32_t* __restrict__ leaves = leaves_;
   static_assert(ROW_WORDS % 16 == 0, "whole chunks");
   const uint64_t i = ((uint64_t)blockIdx.x * MERKLE_THREADS + threadIdx.x) >> 1;
   if (i >= n_rows) return;  // both lanes of a pair leave together
@@ -117,8 +123,10 @@ __global__ void __launch_bounds__(MERKLE_THREADS) merkle_leaves_pair_kernel(cons
 
 // parent[j] = C(prev[j], prev[j + half]) on a lane pair: lane 0 holds the left
 // digest (words 0..7), lane 1 the right one (words 8..15).
-__global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_pair_kernel(const uint32_t* __restrict__ prev,
-                                                                       uint64_t half, uint32_t* __restrict__ next) {
+__global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_pair_kernel(JP<const uint32_t> prev_,
+                                                                       uint64_t half, JP<uint32_t> next_) {
+  const uint32_t* __restrict__ prev = prev_;
+  uint32_t* __restrict__ next = next_;
   const uint64_t j = ((uint64_t)blockIdx.x * MERKLE_THREADS + threadIdx.x) >> 1;
   if (j >= half) return;
   PairLane<16> L;
@@ -179,7 +187,8 @@ __device__ __forceinline__ void coop_subtrees_body(uint32_t* layer, uint32_t n0,
   }
 }
 
-__global__ void __launch_bounds__(COOP_THREADS) merkle_coop_subtrees_kernel(uint32_t* layer, uint32_t n0, uint32_t G) {
+__global__ void __launch_bounds__(COOP_THREADS) merkle_coop_subtrees_kernel(JP<uint32_t> layer_, uint32_t n0, uint32_t G) {
+  uint32_t* layer = layer_;
   __shared__ uint32_t s_node[COOP_MAX_NODES * 8];
   CoopP16 K;
   K.load();

@@ -140,10 +140,10 @@ KB_HD void dft_regs(Vec<V> (&x)[1 << LR]) {
 }
 
 struct NttArgs {
-  const uint32_t* src;    // first pass: the n_in x C input; later: the n x B work array
-  uint32_t* dst;          // the n x B work array, or the output (final pass)
-  const uint32_t* tw_lo;  // w24^i, i < 4096
-  const uint32_t* tw_hi;  // w24^(4096 i), i < 4096
+  JP<const uint32_t> src;    // first pass: the n_in x C input; later: the n x B work array
+  JP<uint32_t> dst;          // the n x B work array, or the output (final pass)
+  JP<const uint32_t> tw_lo;  // w24^i, i < 4096
+  JP<const uint32_t> tw_hi;  // w24^(4096 i), i < 4096
   uint32_t n_in;          // first pass: input rows (rows >= n_in read as zero)
   uint32_t log_c;         // first pass: input columns C (>= V)
   uint32_t log_nn;        // first pass: log2 N (order of the coset twist)

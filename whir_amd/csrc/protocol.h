@@ -39,12 +39,12 @@ struct JobScalars {
 
 // Per-query state (structure of arrays, MAX_STIR entries each).
 struct QueryArrays {
-  uint32_t* index;
-  uint32_t* ws;     // omega^index, squared each fold round (prover.rs:463-469)
-  uint32_t* ts;     // omega^-(index mod mid)            (prover.rs:471-481)
-  uint32_t* var;    // pow-claim variable omega^index     (whir.rs:242-246)
-  Ext* scale;       // alpha^(n_eq + q), updated by fold rounds
-  Ext* G;           // fold-with-code values, 2^d per query
+  JP<uint32_t> index;
+  JP<uint32_t> ws;     // omega^index, squared each fold round (prover.rs:463-469)
+  JP<uint32_t> ts;     // omega^-(index mod mid)            (prover.rs:471-481)
+  JP<uint32_t> var;    // pow-claim variable omega^index     (whir.rs:242-246)
+  JP<Ext> scale;       // alpha^(n_eq + q), updated by fold rounds
+  JP<Ext> G;           // fold-with-code values, 2^d per query
 };
 
 
@@ -72,9 +72,9 @@ __device__ __forceinline__ void lagrange012(const Ext& r, Ext (&l)[3]) {
 // Where the next claim's draw goes (a claim kernel can finish one claim and
 // begin the next in the same launch).
 struct ClaimBegin {
-  ClaimSlot* slot;  // first slot of the group; nullptr: nothing to begin
-  Ext* eq0;
-  Ext* eq1;
+  JP<ClaimSlot> slot;  // first slot of the group; nullptr: nothing to begin
+  JP<Ext> eq0;
+  JP<Ext> eq1;
   uint32_t k, fsv, m0, m1;
   uint32_t count;   // claims sharing the point (slots slot[0 .. count))
 };
@@ -155,9 +155,13 @@ __device__ __forceinline__ void write_root_body(WaveTx& tx, const uint32_t* root
 // is set, the first claim's draw (claim_begin_body). fresh_cap != 0 starts a
 // new transcript with a log of that many words (the proof's first commitment).
 static_assert(COOP_THREADS == SERIAL_THREADS, "one workgroup shape for the tree top and the claim");
-__global__ void __launch_bounds__(SERIAL_THREADS) tree_top_claim_kernel(uint32_t* layer, uint32_t n0, DevTx* t,
-                                                                        uint32_t* log, JobScalars* js,
+__global__ void __launch_bounds__(SERIAL_THREADS) tree_top_claim_kernel(JP<uint32_t> layer_, uint32_t n0, JP<DevTx> t_,
+                                                                        JP<uint32_t> log_, JP<JobScalars> js_,
                                                                         ClaimBegin cb, uint32_t fresh_cap) {
+  uint32_t* layer = layer_;
+  DevTx* t = t_;
+  uint32_t* log = log_;
+  JobScalars* js = js_;
   __shared__ uint32_t s_node[COOP_MAX_NODES * 8];
   __shared__ ClaimScratch sc;
   {
@@ -178,8 +182,11 @@ __global__ void __launch_bounds__(SERIAL_THREADS) tree_top_claim_kernel(uint32_t
 // A claim group's begin on its own (the commit / eval / open entry points,
 // where the caller decides between a drawn and a given point after the
 // commitment): the point, written to the group's slots, and its eq tables.
-__global__ void __launch_bounds__(SERIAL_THREADS) claim_begin_kernel(DevTx* t, uint32_t* log, JobScalars* js,
+__global__ void __launch_bounds__(SERIAL_THREADS) claim_begin_kernel(JP<DevTx> t_, JP<uint32_t> log_, JP<JobScalars> js_,
                                                                      ClaimBegin cb, GivenPoint gp, uint32_t given) {
+  DevTx* t = t_;
+  uint32_t* log = log_;
+  JobScalars* js = js_;
   __shared__ ClaimScratch sc;
   WaveTx tx;
   if (is_wave0()) tx_begin(tx, t, log, js);
@@ -189,9 +196,9 @@ __global__ void __launch_bounds__(SERIAL_THREADS) claim_begin_kernel(DevTx* t, u
 
 // Continue a caller's transcript: the challenger state `init` (sponge state,
 // input and output buffers) with an empty proof log of init.log_cap words.
-__global__ void tx_import_kernel(DevTx* t, JobScalars* js, DevTx init) {
+__global__ void tx_import_kernel(JP<DevTx> t, JP<JobScalars> js, DevTx init) {
   if (threadIdx.x == 0) {
-    *t = init;
+    *t.get() = init;
     js->pending_words = 0;
   }
 }
@@ -279,10 +286,15 @@ __device__ void claim_end_values(ClaimSlot* slot, uint32_t count, const Ext* par
   }
 }
 
-__global__ void __launch_bounds__(SERIAL_THREADS) claim_end_kernel(DevTx* t, uint32_t* log, JobScalars* js,
-                                                                   ClaimSlot* slot, uint32_t count,
-                                                                   const Ext* partials, uint32_t n_part, uint32_t fsv,
+__global__ void __launch_bounds__(SERIAL_THREADS) claim_end_kernel(JP<DevTx> t_, JP<uint32_t> log_, JP<JobScalars> js_,
+                                                                   JP<ClaimSlot> slot_, uint32_t count,
+                                                                   JP<const Ext> partials_, uint32_t n_part, uint32_t fsv,
                                                                    ClaimBegin next) {
+  DevTx* t = t_;
+  uint32_t* log = log_;
+  JobScalars* js = js_;
+  ClaimSlot* slot = slot_;
+  const Ext* partials = partials_;
   __shared__ ClaimEndScratch cs;
   __shared__ Ext s_eval[MAX_CLAIMS];
   __shared__ ClaimScratch sc;
@@ -307,11 +319,20 @@ __global__ void __launch_bounds__(SERIAL_THREADS) claim_end_kernel(DevTx* t, uin
 // in layout order, then the virtual claims).
 // Scratch: pe_buf (n_claims x PE_PER), gpe_buf (n_groups x PE_PER); eq1_pref:
 // prefix sums of the claims' eq1 sizes (n_claims + 1).
-__global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(DevTx* t, uint32_t* log, JobScalars* js,
-                                                             const ClaimSlot* slots, const ClaimDesc* desc,
-                                                             uint32_t n_claims, const uint32_t* group_start,
-                                                             uint32_t n_groups, uint32_t f, Ext* pe_buf,
-                                                             Ext* gpe_buf, Ext* claim_scale) {
+__global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(JP<DevTx> t_, JP<uint32_t> log_, JP<JobScalars> js_,
+                                                             JP<const ClaimSlot> slots_, JP<const ClaimDesc> desc_,
+                                                             uint32_t n_claims, JP<const uint32_t> group_start_,
+                                                             uint32_t n_groups, uint32_t f, JP<Ext> pe_buf_,
+                                                             JP<Ext> gpe_buf_, JP<Ext> claim_scale_) {
+  DevTx* t = t_;
+  uint32_t* log = log_;
+  JobScalars* js = js_;
+  const ClaimSlot* slots = slots_;
+  const ClaimDesc* desc = desc_;
+  const uint32_t* group_start = group_start_;
+  Ext* pe_buf = pe_buf_;
+  Ext* gpe_buf = gpe_buf_;
+  Ext* claim_scale = claim_scale_;
   // partial evals pe[c][2^l - 2 + a] (l = 1..f), accumulators tot[ri][side][u]
   constexpr int PE_PER = (2 << MAX_F) - 2;
   constexpr int ACC_PER = 243 - 1;  // sum_{ri<5} 2 * 3^ri
@@ -558,21 +579,24 @@ __device__ uint32_t sort_dedup_block(uint32_t* s_v, uint32_t n, uint32_t* s_scan
 //   prover.rs:419-425), which the same round's openings launch reads; the
 //   rest of alpha's scaling is stir_scales_block's.
 struct StirBegin {
-  ClaimSlot* end_slot;  // OOD claim whose value this kernel writes first (nullptr: none)
-  const Ext* partials;  // its eval_ext partials
+  JP<ClaimSlot> end_slot;  // OOD claim whose value this kernel writes first (nullptr: none)
+  JP<const Ext> partials;  // its eval_ext partials
   uint32_t n_part;
   uint32_t n, bits;
-  uint32_t* idx_out;
-  uint32_t* count_out;
-  Ext* eq_pool;         // OOD claims' split eq tables (eq0: 2^m0 entries, then eq1)
+  JP<uint32_t> idx_out;
+  JP<uint32_t> count_out;
+  JP<Ext> eq_pool;         // OOD claims' split eq tables (eq0: 2^m0 entries, then eq1)
   uint64_t eq_stride;
   uint32_t n_eq, m0, m1;
   QueryArrays qa;
   uint32_t query_scales;  // 0: stir_query_kernel's extra workgroups write them
 };
 
-__global__ void __launch_bounds__(SERIAL_THREADS) stir_begin_kernel(DevTx* t, uint32_t* log, JobScalars* js,
+__global__ void __launch_bounds__(SERIAL_THREADS) stir_begin_kernel(JP<DevTx> t_, JP<uint32_t> log_, JP<JobScalars> js_,
                                                                     StirBegin a) {
+  DevTx* t = t_;
+  uint32_t* log = log_;
+  JobScalars* js = js_;
   __shared__ ClaimEndScratch cs;
   __shared__ Ext s_eval[1];
   __shared__ uint32_t s_v[MAX_STIR];
@@ -614,10 +638,10 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_begin_kernel(DevTx* t, ui
 // openings launch do it, off the transcript's chain.
 constexpr uint32_t STIR_EXTRA_BLOCKS = 4;
 struct StirScales {
-  Ext* eq_pool;  // OOD claims' split eq tables (eq0: 2^m0 entries, then eq1)
+  JP<Ext> eq_pool;  // OOD claims' split eq tables (eq0: 2^m0 entries, then eq1)
   uint64_t eq_stride;
   uint32_t n_eq, m0, m1;
-  Ext* scale;    // query scales to write; nullptr: stir_begin_kernel wrote them
+  JP<Ext> scale;    // query scales to write; nullptr: stir_begin_kernel wrote them
 };
 __device__ void stir_scales_block(const StirScales& ss, const Ext alpha, uint32_t count, uint32_t eb) {
   constexpr uint32_t S = STIR_EXTRA_BLOCKS * SERIAL_THREADS;
@@ -666,13 +690,22 @@ __device__ __forceinline__ uint32_t opening_word(const uint32_t* cw, uint32_t ro
 // mode 2 (later rounds): pow-claim variable var = omega^index (whir.rs:242-246)
 // and the query's pow tables for combine_kernel: powL[q][e] = var^e (e < 2^m0),
 // powR[q][e] = scale_q var^(e 2^m0) (e < 2^m1).
-__global__ void __launch_bounds__(SERIAL_THREADS) stir_query_kernel(const DevTx* t, JobScalars* js, uint32_t* log,
-                                                                    const uint32_t* idx, const uint32_t* count_ptr,
-                                                                    const uint32_t* cw, uint32_t row_words,
-                                                                    const uint32_t* tree, uint32_t depth, int mode,
+__global__ void __launch_bounds__(SERIAL_THREADS) stir_query_kernel(JP<const DevTx> t_, JP<JobScalars> js_, JP<uint32_t> log_,
+                                                                    JP<const uint32_t> idx_, JP<const uint32_t> count_ptr_,
+                                                                    JP<const uint32_t> cw_, uint32_t row_words,
+                                                                    JP<const uint32_t> tree_, uint32_t depth, int mode,
                                                                     uint32_t d, QueryArrays qa, uint32_t m0,
-                                                                    uint32_t m1, uint32_t* powL, Ext* powR,
+                                                                    uint32_t m1, JP<uint32_t> powL_, JP<Ext> powR_,
                                                                     StirScales ss) {
+  const DevTx* t = t_;
+  JobScalars* js = js_;
+  uint32_t* log = log_;
+  const uint32_t* idx = idx_;
+  const uint32_t* count_ptr = count_ptr_;
+  const uint32_t* cw = cw_;
+  const uint32_t* tree = tree_;
+  uint32_t* powL = powL_;
+  Ext* powR = powR_;
   const uint32_t count = *count_ptr;
   const uint32_t n_open = gridDim.x - STIR_EXTRA_BLOCKS;  // workgroups for openings
   if (blockIdx.x >= n_open) {
@@ -725,9 +758,13 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_query_kernel(const DevTx*
 // partials (query_terms_partial, by the query workgroups of the launch before)
 // and the per-query state advances with r (prover.rs:542-553) in the next
 // fold launch (query_fold_block): both off the transcript's chain.
-__global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(DevTx* t, uint32_t* log, JobScalars* js,
-                                                                        const Ext* partials, uint32_t n_part,
+__global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(JP<DevTx> t_, JP<uint32_t> log_, JP<JobScalars> js_,
+                                                                        JP<const Ext> partials_, uint32_t n_part,
                                                                         uint32_t rs_index) {
+  DevTx* t = t_;
+  uint32_t* log = log_;
+  JobScalars* js = js_;
+  const Ext* partials = partials_;
   __shared__ Ext sh[SERIAL_THREADS];
   const uint32_t tid = threadIdx.x;
   WaveTx tx;
@@ -770,10 +807,10 @@ __global__ void __launch_bounds__(SERIAL_THREADS) round_finalize_kernel(DevTx* t
 // round_finalize_kernel reads their partials after that launch ends.
 struct QueryFold {
   QueryArrays qa;
-  const uint32_t* count_ptr;
+  JP<const uint32_t> count_ptr;
   uint32_t d, k_data, rho;
-  const uint32_t* tw_lo;
-  const uint32_t* tw_hi;
+  JP<const uint32_t> tw_lo;
+  JP<const uint32_t> tw_hi;
 };
 
 constexpr uint32_t QUERY_BLOCKS = 4;
@@ -868,27 +905,41 @@ __device__ void query_fold_block(const QueryFold& qf, const Ext r, uint32_t qb, 
 // workgroups, the last ones: the query terms of step 0 as partials g..g+3
 // (coeffs_q_kernel), or query_fold_block with the same r (its terms as
 // partials g..g+3 in fold_coeffs_q_kernel; none after the last fold).
-__global__ void __launch_bounds__(SC_THREADS) coeffs_q_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
-                                                          uint64_t n, Ext* __restrict__ partials, QueryFold qf) {
+__global__ void __launch_bounds__(SC_THREADS) coeffs_q_kernel(JP<const Ext> f_, JP<const Ext> w_,
+                                                          uint64_t n, JP<Ext> partials_, QueryFold qf) {
+  const Ext* __restrict__ f = f_;
+  const Ext* __restrict__ w = w_;
+  Ext* __restrict__ partials = partials_;
   __shared__ Ext sh[SC_THREADS];
   const uint32_t nf = gridDim.x - QUERY_BLOCKS;
   if (blockIdx.x >= nf)
     query_terms_partial(qf.qa, *qf.count_ptr, blockIdx.x - nf, qf.d, qf.k_data, 0, partials, blockIdx.x, sh);
   else coeffs_body(f, w, n, partials, blockIdx.x, nf, sh);
 }
-__global__ void __launch_bounds__(SC_THREADS) fold_coeffs_q_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
-                                                               uint64_t n, const Ext* __restrict__ r_ptr,
-                                                               Ext* __restrict__ fo, Ext* __restrict__ wo,
-                                                               Ext* __restrict__ partials, QueryFold qf) {
+__global__ void __launch_bounds__(SC_THREADS) fold_coeffs_q_kernel(JP<const Ext> f_, JP<const Ext> w_,
+                                                               uint64_t n, JP<const Ext> r_ptr_,
+                                                               JP<Ext> fo_, JP<Ext> wo_,
+                                                               JP<Ext> partials_, QueryFold qf) {
+  const Ext* __restrict__ f = f_;
+  const Ext* __restrict__ w = w_;
+  const Ext* __restrict__ r_ptr = r_ptr_;
+  Ext* __restrict__ fo = fo_;
+  Ext* __restrict__ wo = wo_;
+  Ext* __restrict__ partials = partials_;
   __shared__ Ext sh[SC_THREADS];
   const Ext r = ld_ext(r_ptr);
   const uint32_t nf = gridDim.x - QUERY_BLOCKS;
   if (blockIdx.x >= nf) query_fold_block(qf, r, blockIdx.x - nf, partials, blockIdx.x, sh);
   else fold_coeffs_body(f, w, n, r, fo, wo, partials, blockIdx.x, nf, sh);
 }
-__global__ void __launch_bounds__(SC_THREADS) fold_q_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
-                                                        uint64_t n, const Ext* __restrict__ r_ptr,
-                                                        Ext* __restrict__ fo, Ext* __restrict__ wo, QueryFold qf) {
+__global__ void __launch_bounds__(SC_THREADS) fold_q_kernel(JP<const Ext> f_, JP<const Ext> w_,
+                                                        uint64_t n, JP<const Ext> r_ptr_,
+                                                        JP<Ext> fo_, JP<Ext> wo_, QueryFold qf) {
+  const Ext* __restrict__ f = f_;
+  const Ext* __restrict__ w = w_;
+  const Ext* __restrict__ r_ptr = r_ptr_;
+  Ext* __restrict__ fo = fo_;
+  Ext* __restrict__ wo = wo_;
   __shared__ Ext sh[SC_THREADS];
   const Ext r = ld_ext(r_ptr);
   const uint32_t nf = gridDim.x - QUERY_BLOCKS;

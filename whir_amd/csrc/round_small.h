@@ -39,14 +39,14 @@ inline bool small_round_fits(int kc, int n_ood, int n_stir) {
 }
 
 struct SmallRoundArgs {
-  const uint32_t* root;      // this round's commitment root (8 words)
-  Ext* f_cur;                // round polynomial, 2^kc extension elements
-  Ext* w_cur;                // its sumcheck weights
-  Ext* f_alt;                // ping-pong buffers for the folds
-  Ext* w_alt;
+  JP<const uint32_t> root;      // this round's commitment root (8 words)
+  JP<Ext> f_cur;                // round polynomial, 2^kc extension elements
+  JP<Ext> w_cur;                // its sumcheck weights
+  JP<Ext> f_alt;                // ping-pong buffers for the folds
+  JP<Ext> w_alt;
   uint32_t kc, n_ood, n_stir, kfd, f, rs_idx, row_words;
-  const uint32_t* prev_cw;   // previous round's codeword rows (row_words words each)
-  const uint32_t* prev_tree; // its tree layers (depth kfd)
+  JP<const uint32_t> prev_cw;   // previous round's codeword rows (row_words words each)
+  JP<const uint32_t> prev_tree; // its tree layers (depth kfd)
 };
 
 // Sums of two Ext per thread over the workgroup, into wave 0 (the transcript's
@@ -149,8 +149,11 @@ __device__ __forceinline__ void sumcheck_block(WaveTx& tx, JobScalars* js, Ext* 
   }
 }
 
-__global__ void __launch_bounds__(SR_THREADS) small_round_kernel(DevTx* t, uint32_t* log, JobScalars* js,
+__global__ void __launch_bounds__(SR_THREADS) small_round_kernel(JP<DevTx> t_, JP<uint32_t> log_, JP<JobScalars> js_,
                                                                   SmallRoundArgs a) {
+  DevTx* t = t_;
+  uint32_t* log = log_;
+  JobScalars* js = js_;
   __shared__ Ext s_eq[SR_MAX_OOD][2][64];   // split eq tables of the OOD points
   __shared__ Ext s_pt[SR_MAX_LOG_N];
   __shared__ Ext s_wave[2 * SR_WAVES];
@@ -280,9 +283,16 @@ __global__ void __launch_bounds__(SR_THREADS) small_round_kernel(DevTx* t, uint3
 // transcript and folded arrays as coeffs_kernel, then f times round_finalize
 // (fold_coeffs_kernel between them) and fold_kernel, in one launch instead of
 // 2f + 1 whose ~5 us floors are most of their time at this size.
-__global__ void __launch_bounds__(SR_THREADS) sumcheck_tail_kernel(DevTx* t, uint32_t* log, JobScalars* js, Ext* f,
-                                                                   Ext* w, Ext* fo, Ext* wo, uint32_t n, uint32_t nf,
+__global__ void __launch_bounds__(SR_THREADS) sumcheck_tail_kernel(JP<DevTx> t_, JP<uint32_t> log_, JP<JobScalars> js_, JP<Ext> f_,
+                                                                   JP<Ext> w_, JP<Ext> fo_, JP<Ext> wo_, uint32_t n, uint32_t nf,
                                                                    uint32_t rs_idx) {
+  DevTx* t = t_;
+  uint32_t* log = log_;
+  JobScalars* js = js_;
+  Ext* f = f_;
+  Ext* w = w_;
+  Ext* fo = fo_;
+  Ext* wo = wo_;
   __shared__ Ext s_wave[2 * SR_WAVES];
   __shared__ Ext s_x;
   WaveTx tx;
@@ -294,10 +304,16 @@ __global__ void __launch_bounds__(SR_THREADS) sumcheck_tail_kernel(DevTx* t, uin
 // The final phase (whir.rs:314-355 and test.rs:233): write the final
 // polynomial (n extension elements), draw the last STIR indexes, write their
 // openings of the last tree as hints, draw the checkpoint.
-__global__ void __launch_bounds__(SERIAL_THREADS) final_phase_kernel(DevTx* t, uint32_t* log, JobScalars* js,
-                                                                     const Ext* poly, uint32_t n, uint32_t n_stir,
-                                                                     uint32_t kfd, const uint32_t* cw,
-                                                                     uint32_t row_words, const uint32_t* tree) {
+__global__ void __launch_bounds__(SERIAL_THREADS) final_phase_kernel(JP<DevTx> t_, JP<uint32_t> log_, JP<JobScalars> js_,
+                                                                     JP<const Ext> poly_, uint32_t n, uint32_t n_stir,
+                                                                     uint32_t kfd, JP<const uint32_t> cw_,
+                                                                     uint32_t row_words, JP<const uint32_t> tree_) {
+  DevTx* t = t_;
+  uint32_t* log = log_;
+  JobScalars* js = js_;
+  const Ext* poly = poly_;
+  const uint32_t* cw = cw_;
+  const uint32_t* tree = tree_;
   __shared__ uint32_t s_v[MAX_STIR];
   __shared__ uint32_t s_scan[SERIAL_THREADS];
   __shared__ uint32_t s_idx[MAX_STIR];

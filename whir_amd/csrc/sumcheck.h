@@ -35,8 +35,8 @@ struct StackBlocks {
   uint32_t log[MAX_CLASSES];
   uint32_t seg0[MAX_CLASSES];
   uint64_t start[MAX_CLASSES + 1];
-  const uint32_t* seg_start;
-  const uint32_t* seg_claims;
+  JP<const uint32_t> seg_start;
+  JP<const uint32_t> seg_claims;
 };
 
 
@@ -149,10 +149,15 @@ __device__ void build_eq_pair(Ext* out0, const Ext* z0, int m0, Ext scale0, Ext*
 // its poly at desc[c0 + y].poly_off (desc == nullptr: offset 0) and writes its
 // partials after the previous claims' gridDim.x * W.
 template <int F_LOG>
-__global__ void __launch_bounds__(SC_THREADS) compress_hi_kernel(const uint32_t* __restrict__ poly,
-                                                             const Ext* __restrict__ eq0, const Ext* __restrict__ eq1,
-                                                             uint32_t m0, Ext* __restrict__ partials,
-                                                             const ClaimDesc* __restrict__ desc, uint32_t c0) {
+__global__ void __launch_bounds__(SC_THREADS) compress_hi_kernel(JP<const uint32_t> poly_,
+                                                             JP<const Ext> eq0_, JP<const Ext> eq1_,
+                                                             uint32_t m0, JP<Ext> partials_,
+                                                             JP<const ClaimDesc> desc_, uint32_t c0) {
+  const uint32_t* __restrict__ poly = poly_;
+  const Ext* __restrict__ eq0 = eq0_;
+  const Ext* __restrict__ eq1 = eq1_;
+  Ext* __restrict__ partials = partials_;
+  const ClaimDesc* __restrict__ desc = desc_;
   if (desc) poly += desc[c0 + blockIdx.y].poly_off;
   partials += (uint64_t)blockIdx.y * gridDim.x * (1u << F_LOG);
   constexpr int W = 1 << F_LOG;
@@ -219,9 +224,13 @@ __global__ void __launch_bounds__(SC_THREADS) compress_hi_kernel(const uint32_t*
 }
 
 // ---- evaluation of an extension polynomial (SplitEq::eval_ext, eq.rs:179-250) ----
-__global__ void __launch_bounds__(SC_THREADS) eval_ext_kernel(const Ext* __restrict__ poly, const Ext* __restrict__ eq0,
-                                                          const Ext* __restrict__ eq1, uint32_t m0,
-                                                          Ext* __restrict__ partials) {
+__global__ void __launch_bounds__(SC_THREADS) eval_ext_kernel(JP<const Ext> poly_, JP<const Ext> eq0_,
+                                                          JP<const Ext> eq1_, uint32_t m0,
+                                                          JP<Ext> partials_) {
+  const Ext* __restrict__ poly = poly_;
+  const Ext* __restrict__ eq0 = eq0_;
+  const Ext* __restrict__ eq1 = eq1_;
+  Ext* __restrict__ partials = partials_;
   __shared__ Ext sh[SC_THREADS];
   const uint64_t h1 = blockIdx.x;
   const uint64_t n0 = 1ull << m0;
@@ -240,12 +249,19 @@ __global__ void __launch_bounds__(SC_THREADS) eval_ext_kernel(const Ext* __restr
 // m0); the others are on one stacked poly's block (prover.rs:190-199), found
 // through `sb` (n_class == 0: none).
 template <int F_LOG>
-__global__ void __launch_bounds__(SC_THREADS) expr_build_kernel(const uint32_t* __restrict__ poly,
-                                                            const Ext* __restrict__ eq_rs, const Ext* __restrict__ pool,
-                                                            const ClaimDesc* __restrict__ desc, uint32_t full0,
+__global__ void __launch_bounds__(SC_THREADS) expr_build_kernel(JP<const uint32_t> poly_,
+                                                            JP<const Ext> eq_rs_, JP<const Ext> pool_,
+                                                            JP<const ClaimDesc> desc_, uint32_t full0,
                                                             uint32_t n_full, uint32_t m0, uint64_t n,
-                                                            Ext* __restrict__ epoly, Ext* __restrict__ ew,
-                                                            StackBlocks sb, const Ext* __restrict__ cscale) {
+                                                            JP<Ext> epoly_, JP<Ext> ew_,
+                                                            StackBlocks sb, JP<const Ext> cscale_) {
+  const uint32_t* __restrict__ poly = poly_;
+  const Ext* __restrict__ eq_rs = eq_rs_;
+  const Ext* __restrict__ pool = pool_;
+  const ClaimDesc* __restrict__ desc = desc_;
+  Ext* __restrict__ epoly = epoly_;
+  Ext* __restrict__ ew = ew_;
+  const Ext* __restrict__ cscale = cscale_;
   constexpr int W = 1 << F_LOG;
   __shared__ Ext s_eq[W];
   if (threadIdx.x < W) s_eq[threadIdx.x] = eq_rs[threadIdx.x];
@@ -314,11 +330,16 @@ __global__ void __launch_bounds__(SC_THREADS) expr_build_kernel(const uint32_t* 
 // Montgomery products taken two per REDC (2 p^2 < p 2^32) and summed unreduced
 // in 64 bits, so a term costs ~5 VALU instructions per coefficient.
 template <int S, bool UNI>
-__global__ void __launch_bounds__(SC_THREADS) combine_kernel(Ext* __restrict__ w, uint64_t n, uint32_t m0,
-                                                         const Ext* __restrict__ eq_pool, uint32_t n_eq,
-                                                         uint64_t eq_stride, const uint32_t* __restrict__ powL,
-                                                         const Ext* __restrict__ powR, const uint32_t* n_pow_ptr,
+__global__ void __launch_bounds__(SC_THREADS) combine_kernel(JP<Ext> w_, uint64_t n, uint32_t m0,
+                                                         JP<const Ext> eq_pool_, uint32_t n_eq,
+                                                         uint64_t eq_stride, JP<const uint32_t> powL_,
+                                                         JP<const Ext> powR_, JP<const uint32_t> n_pow_ptr_,
                                                          uint32_t m1) {
+  Ext* __restrict__ w = w_;
+  const Ext* __restrict__ eq_pool = eq_pool_;
+  const uint32_t* __restrict__ powL = powL_;
+  const Ext* __restrict__ powR = powR_;
+  const uint32_t* n_pow_ptr = n_pow_ptr_;
   constexpr uint32_t XB = SC_THREADS / S;  // outputs per workgroup, a multiple of 64
   __shared__ Ext s_part[S > 1 ? SC_THREADS : 1];
   const uint32_t sub = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / XB));
@@ -366,7 +387,7 @@ __global__ void __launch_bounds__(SC_THREADS) combine_kernel(Ext* __restrict__ w
 // Host: launch combine_kernel for 2^(m0 + m1) = n outputs; S = 4 when there
 // are pow terms (n_pow_ptr set) or more than four eq terms, one lane per output
 // otherwise.
-inline void launch_combine(hipStream_t s, Ext* w, uint64_t n, uint32_t m0, const Ext* eq_pool, uint32_t n_eq,
+inline void launch_combine(hipStream_t s, uint32_t jobs, Ext* w, uint64_t n, uint32_t m0, const Ext* eq_pool, uint32_t n_eq,
                            uint64_t eq_stride, const uint32_t* powL, const Ext* powR, const uint32_t* n_pow_ptr,
                            uint32_t m1) {
   const bool uni = m0 >= 6;
@@ -374,26 +395,31 @@ inline void launch_combine(hipStream_t s, Ext* w, uint64_t n, uint32_t m0, const
   const unsigned grid = (unsigned)((n + SC_THREADS / S - 1) / (SC_THREADS / S));
   if (S == 4) {
     if (uni)
-      hipLaunchKernelGGL((combine_kernel<4, true>), dim3(grid), dim3(SC_THREADS), 0, s, w, n, m0, eq_pool, n_eq,
+      hipLaunchKernelGGL((combine_kernel<4, true>), dim3(grid, 1, jobs), dim3(SC_THREADS), 0, s, w, n, m0, eq_pool, n_eq,
                          eq_stride, powL, powR, n_pow_ptr, m1);
     else
-      hipLaunchKernelGGL((combine_kernel<4, false>), dim3(grid), dim3(SC_THREADS), 0, s, w, n, m0, eq_pool, n_eq,
+      hipLaunchKernelGGL((combine_kernel<4, false>), dim3(grid, 1, jobs), dim3(SC_THREADS), 0, s, w, n, m0, eq_pool, n_eq,
                          eq_stride, powL, powR, n_pow_ptr, m1);
   } else {
     if (uni)
-      hipLaunchKernelGGL((combine_kernel<1, true>), dim3(grid), dim3(SC_THREADS), 0, s, w, n, m0, eq_pool, n_eq,
+      hipLaunchKernelGGL((combine_kernel<1, true>), dim3(grid, 1, jobs), dim3(SC_THREADS), 0, s, w, n, m0, eq_pool, n_eq,
                          eq_stride, powL, powR, n_pow_ptr, m1);
     else
-      hipLaunchKernelGGL((combine_kernel<1, false>), dim3(grid), dim3(SC_THREADS), 0, s, w, n, m0, eq_pool, n_eq,
+      hipLaunchKernelGGL((combine_kernel<1, false>), dim3(grid, 1, jobs), dim3(SC_THREADS), 0, s, w, n, m0, eq_pool, n_eq,
                          eq_stride, powL, powR, n_pow_ptr, m1);
   }
 }
 
 // powL[j][e] = var_j^e for e < 2^m0; powR[j][e] = scale_j * var_j^(e * 2^m0) for e < 2^m1.
-__global__ void __launch_bounds__(SC_THREADS) pow_tables_kernel(const uint32_t* __restrict__ vars,
-                                                            const Ext* __restrict__ scales, const uint32_t* count_ptr,
-                                                            uint32_t m0, uint32_t m1, uint32_t* __restrict__ powL,
-                                                            Ext* __restrict__ powR) {
+__global__ void __launch_bounds__(SC_THREADS) pow_tables_kernel(JP<const uint32_t> vars_,
+                                                            JP<const Ext> scales_, JP<const uint32_t> count_ptr_,
+                                                            uint32_t m0, uint32_t m1, JP<uint32_t> powL_,
+                                                            JP<Ext> powR_) {
+  const uint32_t* __restrict__ vars = vars_;
+  const Ext* __restrict__ scales = scales_;
+  const uint32_t* count_ptr = count_ptr_;
+  uint32_t* __restrict__ powL = powL_;
+  Ext* __restrict__ powR = powR_;
   const uint32_t count = *count_ptr;
   const uint64_t n0 = 1ull << m0, n1 = 1ull << m1;
   const uint64_t per = n0 + n1;
@@ -434,8 +460,11 @@ __device__ __forceinline__ void coeffs_body(const Ext* __restrict__ f, const Ext
   }
 }
 
-__global__ void __launch_bounds__(SC_THREADS) coeffs_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
-                                                        uint64_t n, Ext* __restrict__ partials) {
+__global__ void __launch_bounds__(SC_THREADS) coeffs_kernel(JP<const Ext> f_, JP<const Ext> w_,
+                                                        uint64_t n, JP<Ext> partials_) {
+  const Ext* __restrict__ f = f_;
+  const Ext* __restrict__ w = w_;
+  Ext* __restrict__ partials = partials_;
   __shared__ Ext sh[SC_THREADS];
   coeffs_body(f, w, n, partials, blockIdx.x, gridDim.x, sh);
 }
@@ -470,10 +499,16 @@ __device__ __forceinline__ void fold_coeffs_body(const Ext* __restrict__ f, cons
   }
 }
 
-__global__ void __launch_bounds__(SC_THREADS) fold_coeffs_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
-                                                             uint64_t n, const Ext* __restrict__ r_ptr,
-                                                             Ext* __restrict__ fo, Ext* __restrict__ wo,
-                                                             Ext* __restrict__ partials) {
+__global__ void __launch_bounds__(SC_THREADS) fold_coeffs_kernel(JP<const Ext> f_, JP<const Ext> w_,
+                                                             uint64_t n, JP<const Ext> r_ptr_,
+                                                             JP<Ext> fo_, JP<Ext> wo_,
+                                                             JP<Ext> partials_) {
+  const Ext* __restrict__ f = f_;
+  const Ext* __restrict__ w = w_;
+  const Ext* __restrict__ r_ptr = r_ptr_;
+  Ext* __restrict__ fo = fo_;
+  Ext* __restrict__ wo = wo_;
+  Ext* __restrict__ partials = partials_;
   __shared__ Ext sh[SC_THREADS];
   fold_coeffs_body(f, w, n, ld_ext(r_ptr), fo, wo, partials, blockIdx.x, gridDim.x, sh);
 }
@@ -488,9 +523,14 @@ __device__ __forceinline__ void fold_body(const Ext* __restrict__ f, const Ext* 
   st_ext(wo + i, eadd(b0, emul(r, esub(b1, b0))));
 }
 
-__global__ void __launch_bounds__(SC_THREADS) fold_kernel(const Ext* __restrict__ f, const Ext* __restrict__ w,
-                                                      uint64_t n, const Ext* __restrict__ r_ptr, Ext* __restrict__ fo,
-                                                      Ext* __restrict__ wo) {
+__global__ void __launch_bounds__(SC_THREADS) fold_kernel(JP<const Ext> f_, JP<const Ext> w_,
+                                                      uint64_t n, JP<const Ext> r_ptr_, JP<Ext> fo_,
+                                                      JP<Ext> wo_) {
+  const Ext* __restrict__ f = f_;
+  const Ext* __restrict__ w = w_;
+  const Ext* __restrict__ r_ptr = r_ptr_;
+  Ext* __restrict__ fo = fo_;
+  Ext* __restrict__ wo = wo_;
   fold_body(f, w, n, ld_ext(r_ptr), fo, wo, blockIdx.x);
 }
 

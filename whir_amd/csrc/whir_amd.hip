@@ -255,7 +255,7 @@ static bool ntt_plan_for(const NttConfig& cfg, uint64_t n_in, int log_nn, int lo
 
 // `force`: run with this configuration instead of the tuned or default one.
 static int ntt_enqueue(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_nn, int log_c, uint32_t* out,
-                       uint32_t* tmp, hipStream_t s, const NttConfig* force = nullptr) {
+                       uint32_t* tmp, hipStream_t s, const NttConfig* force = nullptr, uint32_t jobs = 1) {
   const NttConfig cfg = force ? *force : ntt_config(c, n_in, log_nn, log_c);
   NttPlan pl;
   if (!ntt_plan_for(cfg, n_in, log_nn, log_c, pl)) return WHIR_EINVAL;
@@ -272,7 +272,7 @@ static int ntt_enqueue(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_n
                          : (pl.lv == 2 ? ntt_kernel_for<4>(l) : pl.lv == 1 ? ntt_kernel_for<2>(l)
                                                                            : ntt_kernel_for<1>(l));
     if (!fn || ps.lds > 65536) return WHIR_EINVAL;
-    hipLaunchKernelGGL(fn, dim3(ps.grid), dim3(ps.threads), ps.lds, s, ps.a);
+    hipLaunchKernelGGL(fn, dim3(ps.grid, 1, jobs), dim3(ps.threads), ps.lds, s, ps.a);
   }
   return WHIR_OK;
 }
@@ -298,10 +298,10 @@ extern "C" int whir_dft_batch(whir_ctx* c, const uint32_t* d_in, uint64_t n_in_r
 // ============================ Merkle ============================
 
 static int leaves_enqueue(const uint32_t* rows, uint64_t n_rows, uint32_t row_words, uint32_t* leaves,
-                          hipStream_t s, const MerkleConfig& mc) {
-  const dim3 grid((unsigned)((n_rows + MERKLE_THREADS - 1) / MERKLE_THREADS)), block(MERKLE_THREADS);
+                          hipStream_t s, const MerkleConfig& mc, uint32_t jobs) {
+  const dim3 grid((unsigned)((n_rows + MERKLE_THREADS - 1) / MERKLE_THREADS), 1, jobs), block(MERKLE_THREADS);
   if (n_rows <= mc.pair_max && row_words % 16 == 0) {
-    const dim3 grid2((unsigned)((2 * n_rows + MERKLE_THREADS - 1) / MERKLE_THREADS));
+    const dim3 grid2((unsigned)((2 * n_rows + MERKLE_THREADS - 1) / MERKLE_THREADS), 1, jobs);
     switch (row_words) {
 #define LEAF_PAIR_CASE(W) \
   case W: hipLaunchKernelGGL(merkle_leaves_pair_kernel<W>, grid2, block, 0, s, rows, n_rows, leaves); return WHIR_OK;
@@ -328,10 +328,10 @@ static int leaves_enqueue(const uint32_t* rows, uint64_t n_rows, uint32_t row_wo
 // node count (tree_top_claim_kernel runs it with the transcript step).
 static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, uint32_t* layers, hipStream_t s,
                           const MerkleConfig& mc, hipEvent_t* leaf_ev = nullptr, uint32_t** top = nullptr,
-                          uint32_t* top_n = nullptr) {
+                          uint32_t* top_n = nullptr, uint32_t jobs = 1) {
   const uint64_t H = 1ull << log_h;
   if (leaf_ev) HIP_TRY(hipEventRecord(leaf_ev[0], s));
-  int rc = leaves_enqueue(rows, H, row_words, layers, s, mc);
+  int rc = leaves_enqueue(rows, H, row_words, layers, s, mc, jobs);
   if (rc) return rc;
   if (leaf_ev) HIP_TRY(hipEventRecord(leaf_ev[1], s));
   uint32_t* cur = layers;
@@ -339,10 +339,10 @@ static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, u
   while (n > std::max<uint64_t>(mc.coop_from, COOP_MAX_NODES)) {
     const uint64_t half = n / 2;
     if (half <= mc.pair_max)
-      hipLaunchKernelGGL(merkle_layer_pair_kernel, dim3((unsigned)((2 * half + MERKLE_THREADS - 1) / MERKLE_THREADS)),
+      hipLaunchKernelGGL(merkle_layer_pair_kernel, dim3((unsigned)((2 * half + MERKLE_THREADS - 1) / MERKLE_THREADS), 1, jobs),
                          dim3(MERKLE_THREADS), 0, s, cur, half, cur + n * 8);
     else
-      hipLaunchKernelGGL(merkle_layer_kernel, dim3((unsigned)((half + MERKLE_THREADS - 1) / MERKLE_THREADS)),
+      hipLaunchKernelGGL(merkle_layer_kernel, dim3((unsigned)((half + MERKLE_THREADS - 1) / MERKLE_THREADS), 1, jobs),
                          dim3(MERKLE_THREADS), 0, s, cur, half, cur + n * 8);
     cur += n * 8;
     n = half;
@@ -353,7 +353,7 @@ static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, u
     const uint64_t kk = std::min<uint64_t>(n, COOP_MAX_NODES);
     const uint32_t G = (uint32_t)(n / kk);
     if (G == 1 && top) break;
-    hipLaunchKernelGGL(merkle_coop_subtrees_kernel, dim3(G), dim3(COOP_THREADS), 0, s, cur, (uint32_t)n, G);
+    hipLaunchKernelGGL(merkle_coop_subtrees_kernel, dim3(G, 1, jobs), dim3(COOP_THREADS), 0, s, cur, (uint32_t)n, G);
     while (n > G) {  // advance to the G-node layer
       cur += n * 8;
       n /= 2;
@@ -425,6 +425,7 @@ constexpr uint32_t LOG_TAIL = 4;
 struct whir_prover {
   whir_ctx* ctx = nullptr;
   Plan plan;
+  uint32_t jobs = 1;  // proofs run in lockstep (gridDim.z) on workspace copies at a fixed stride
   void* arena = nullptr;
   size_t arena_bytes = 0;
   // carved buffers
@@ -1055,16 +1056,21 @@ extern "C" size_t whir_prover_proof_capacity(const whir_prover* P) { return P ? 
 extern "C" uint32_t* whir_prover_input(whir_prover* P) { return P ? P->input : nullptr; }
 
 static unsigned blocks_for(uint64_t n, unsigned per = SC_THREADS) { return (unsigned)((n + per - 1) / per); }
+// A launch grid over the prover's jobs (gridDim.z).
+static dim3 zg(dim3 g, uint32_t jobs) {
+  g.z = jobs;
+  return g;
+}
 
 template <int FL>
 static void launch_compress_hi(dim3 grid, hipStream_t s, const uint32_t* poly, const Ext* eq0, const Ext* eq1,
-                               uint32_t m0, Ext* partials, const ClaimDesc* desc, uint32_t c0) {
-  hipLaunchKernelGGL(compress_hi_kernel<FL>, grid, dim3(SC_THREADS), 0, s, poly, eq0, eq1, m0, partials, desc, c0);
+                               uint32_t m0, Ext* partials, const ClaimDesc* desc, uint32_t c0, uint32_t jobs) {
+  hipLaunchKernelGGL(compress_hi_kernel<FL>, zg(grid, jobs), dim3(SC_THREADS), 0, s, poly, eq0, eq1, m0, partials, desc, c0);
 }
 template <int FL>
 static void launch_expr(unsigned grid, hipStream_t s, const uint32_t* poly, const Ext* eq_rs, const whir_prover* P,
                         uint32_t m0, uint64_t n, Ext* ep, Ext* ew) {
-  hipLaunchKernelGGL(expr_build_kernel<FL>, dim3(grid), dim3(SC_THREADS), 0, s, poly, eq_rs, (const Ext*)P->svo_pool,
+  hipLaunchKernelGGL(expr_build_kernel<FL>, zg(dim3(grid), P->jobs), dim3(SC_THREADS), 0, s, poly, eq_rs, (const Ext*)P->svo_pool,
                      (const ClaimDesc*)P->desc, P->full0, P->n_full, m0, n, ep, ew, P->sb,
                      (const Ext*)P->claim_scale);
 }
@@ -1109,19 +1115,19 @@ static int enqueue_commit(whir_prover* P, const uint32_t* poly, hipStream_t s, c
   const int k = pl.k, f = pl.f;
   const uint32_t W = 1u << f;
   int rc;
-  if (import) hipLaunchKernelGGL(tx_import_kernel, dim3(1), dim3(64), 0, s, P->tx, P->js, *import);
+  if (import) hipLaunchKernelGGL(tx_import_kernel, zg(dim3(1), P->jobs), dim3(64), 0, s, P->tx, P->js, *import);
   const int log_h0 = k + pl.rate - f;
   if (P->timers) HIP_TRY(hipEventRecord(P->ev[WHIR_TIMER_DFT_BASE][0], s));
-  if ((rc = ntt_enqueue(c, poly, 1ull << (k - f), log_h0, f, P->cw0, P->ntt_tmp, s))) return rc;
+  if ((rc = ntt_enqueue(c, poly, 1ull << (k - f), log_h0, f, P->cw0, P->ntt_tmp, s, nullptr, P->jobs))) return rc;
   if (P->timers) HIP_TRY(hipEventRecord(P->ev[WHIR_TIMER_DFT_BASE][1], s));
   uint32_t* top = nullptr;
   uint32_t top_n = 0;
   if ((rc = merkle_enqueue(P->cw0, log_h0, W, P->tree0, s, merkle_config(c, log_h0, W),
-                           P->timers ? P->ev[WHIR_TIMER_LEAVES_BASE] : nullptr, &top, &top_n)))
+                           P->timers ? P->ev[WHIR_TIMER_LEAVES_BASE] : nullptr, &top, &top_n, P->jobs)))
     return rc;
   P->timer_pending = P->timers;
   // the tree's top levels, transcript.write(root) and the first group's draw
-  hipLaunchKernelGGL(tree_top_claim_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, top, top_n, P->tx, P->log, P->js,
+  hipLaunchKernelGGL(tree_top_claim_kernel, zg(dim3(1), P->jobs), dim3(SERIAL_THREADS), 0, s, top, top_n, P->tx, P->log, P->js,
                      first, import ? 0u : (uint32_t)pl.log_words);
   return WHIR_OK;
 }
@@ -1136,9 +1142,9 @@ static int enqueue_groups(whir_prover* P, const uint32_t* poly, hipStream_t s, i
     const ClaimBegin cb = group_begin(P, g);
     const whir_prover::Group& G = P->groups[g];
     FSWITCH(f, launch_compress_hi<FL>(dim3(1u << cb.m1, G.count), s, poly, cb.eq0, cb.eq1, cb.m0, P->partials,
-                                      P->desc, G.c0));
+                                      P->desc, G.c0, P->jobs));
     const ClaimBegin next = g + 1 < g1 ? group_begin(P, g + 1) : ClaimBegin{};
-    hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, cb.slot, G.count,
+    hipLaunchKernelGGL(claim_end_kernel, zg(dim3(1), P->jobs), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, cb.slot, G.count,
                        P->partials, 1u << cb.m1, (uint32_t)f, next);
   }
   return WHIR_OK;
@@ -1167,7 +1173,7 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
   const int n_groups = (int)P->groups.size();
   const uint32_t m0 = P->m0_svo;
   // ---- SVO rounds + expression (prover.rs:279-353, 184-229) ----
-  hipLaunchKernelGGL(svo_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots, P->desc,
+  hipLaunchKernelGGL(svo_kernel, zg(dim3(1), P->jobs), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots, P->desc,
                      (uint32_t)P->n_svo_claims, P->group_start, (uint32_t)n_groups, (uint32_t)f, P->pe_buf,
                      P->gpe_buf, P->claim_scale);
   uint64_t n = 1ull << (k - f);
@@ -1183,11 +1189,12 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
     uint32_t* cw = P->cw[r & 1];
     uint32_t* tree = P->tree[r & 1];
     // commit_ext (whir.rs:38-61): 2^(kc-f) rows of 2^f extension elements
-    if ((rc = ntt_enqueue(c, (const uint32_t*)P->ep[cur], 1ull << (rp.kc - f), rp.log_rows, f + 2, cw, P->ntt_tmp, s)))
+    if ((rc = ntt_enqueue(c, (const uint32_t*)P->ep[cur], 1ull << (rp.kc - f), rp.log_rows, f + 2, cw, P->ntt_tmp, s,
+                          nullptr, P->jobs)))
       return rc;
     const bool small = r > 0 && small_round_fits(rp.kc, rp.n_ood, rp.n_stir);
     if ((rc = merkle_enqueue(cw, rp.log_rows, 4 * W, tree, s, merkle_config(c, rp.log_rows, 4 * W), nullptr,
-                             small ? nullptr : &top, &top_n)))
+                             small ? nullptr : &top, &top_n, P->jobs)))
       return rc;
     const uint64_t Hr = 1ull << rp.log_rows;
     const uint32_t* root = tree + (2 * Hr - 2) * 8;
@@ -1196,7 +1203,7 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
       SmallRoundArgs sa{root, P->ep[cur], P->ew[cur], P->ep[cur ^ 1], P->ew[cur ^ 1], (uint32_t)rp.kc,
                         (uint32_t)rp.n_ood, (uint32_t)rp.n_stir, (uint32_t)rp.kfd, (uint32_t)f, rs_idx, 4 * W,
                         P->cw[(r - 1) & 1], P->tree[(r - 1) & 1]};
-      hipLaunchKernelGGL(small_round_kernel, dim3(1), dim3(SR_THREADS), 0, s, P->tx, P->log, P->js, sa);
+      hipLaunchKernelGGL(small_round_kernel, zg(dim3(1), P->jobs), dim3(SR_THREADS), 0, s, P->tx, P->log, P->js, sa);
       n >>= f;
       cur ^= f & 1;
       rs_idx += f;
@@ -1219,14 +1226,14 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
       return cb;
     };
     // the tree's top levels, transcript.write(root) and the first OOD draw
-    hipLaunchKernelGGL(tree_top_claim_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, top, top_n, P->tx, P->log, P->js,
+    hipLaunchKernelGGL(tree_top_claim_kernel, zg(dim3(1), P->jobs), dim3(SERIAL_THREADS), 0, s, top, top_n, P->tx, P->log, P->js,
                        rp.n_ood > 0 ? ood_claim(0) : ClaimBegin{}, 0u);
     for (int o = 0; o < rp.n_ood; ++o) {
       const ClaimBegin cb = ood_claim(o);
-      hipLaunchKernelGGL(eval_ext_kernel, dim3(1u << om1), dim3(SC_THREADS), 0, s, P->ep[cur], cb.eq0, cb.eq1, om0,
+      hipLaunchKernelGGL(eval_ext_kernel, zg(dim3(1u << om1), P->jobs), dim3(SC_THREADS), 0, s, P->ep[cur], cb.eq0, cb.eq1, om0,
                          P->partials);
       if (o + 1 < rp.n_ood)
-        hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, cb.slot, 1u,
+        hipLaunchKernelGGL(claim_end_kernel, zg(dim3(1), P->jobs), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, cb.slot, 1u,
                            P->partials, 1u << om1, 0u, ood_claim(o + 1));
     }
     // STIR draws (whir.rs:237-241) and the fold's alpha (prover.rs:377-384),
@@ -1246,7 +1253,7 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
     sb.m1 = om1;
     sb.qa = P->qa;
     sb.query_scales = r > 0 ? 1u : 0u;  // later rounds' openings launch reads them
-    hipLaunchKernelGGL(stir_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, sb);
+    hipLaunchKernelGGL(stir_begin_kernel, zg(dim3(1), P->jobs), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, sb);
     // STIR openings (whir.rs:242-293); later rounds also build the pow tables
     // of combine_pows_scaled (prover.rs:419-425); STIR_EXTRA_BLOCKS more
     // workgroups scale by alpha what the combination reads next
@@ -1254,19 +1261,19 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
                         r == 0 ? P->qa.scale : nullptr};
     const dim3 qgrid((unsigned)rp.n_stir + STIR_EXTRA_BLOCKS);
     if (r == 0) {
-      hipLaunchKernelGGL(stir_query_kernel, qgrid, dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log, P->stir_idx,
+      hipLaunchKernelGGL(stir_query_kernel, zg(qgrid, P->jobs), dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log, P->stir_idx,
                          P->stir_count, P->cw0, W, P->tree0, (uint32_t)rp.kfd, 1, (uint32_t)f, P->qa, 0u, 0u,
                          (uint32_t*)nullptr, (Ext*)nullptr, ss);
     } else {
-      hipLaunchKernelGGL(stir_query_kernel, qgrid, dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log, P->stir_idx,
+      hipLaunchKernelGGL(stir_query_kernel, zg(qgrid, P->jobs), dim3(SERIAL_THREADS), 0, s, P->tx, P->js, P->log, P->stir_idx,
                          P->stir_count, P->cw[(r - 1) & 1], 4 * W, P->tree[(r - 1) & 1], (uint32_t)rp.kfd, 2,
                          (uint32_t)f, P->qa, om0, om1, P->powL, P->powR, ss);
     }
-    launch_combine(s, P->ew[cur], n, om0, P->ood_pool, (uint32_t)rp.n_ood, P->ood_stride, P->powL, P->powR,
+    launch_combine(s, P->jobs, P->ew[cur], n, om0, P->ood_pool, (uint32_t)rp.n_ood, P->ood_stride, P->powL, P->powR,
                    r > 0 ? P->stir_count : nullptr, om1);
     if (r > 0 && n <= (1ull << SC_TAIL_MAX_LOG_N)) {
       // the round's f sumcheck rounds and folds in one workgroup (round_small.h)
-      hipLaunchKernelGGL(sumcheck_tail_kernel, dim3(1), dim3(SR_THREADS), 0, s, P->tx, P->log, P->js, P->ep[cur],
+      hipLaunchKernelGGL(sumcheck_tail_kernel, zg(dim3(1), P->jobs), dim3(SR_THREADS), 0, s, P->tx, P->log, P->js, P->ep[cur],
                          P->ew[cur], P->ep[cur ^ 1], P->ew[cur ^ 1], (uint32_t)n, (uint32_t)f, rs_idx);
       n >>= f;
       cur ^= f & 1;
@@ -1285,31 +1292,31 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
       if (t == 0) {
         g = std::min<unsigned>(1024, blocks_for(n / 2));
         if (r == 0)
-          hipLaunchKernelGGL(coeffs_q_kernel, dim3(g + qx), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
+          hipLaunchKernelGGL(coeffs_q_kernel, zg(dim3(g + qx), P->jobs), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
                              P->partials, qfold(0));
         else
-          hipLaunchKernelGGL(coeffs_kernel, dim3(g), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
+          hipLaunchKernelGGL(coeffs_kernel, zg(dim3(g), P->jobs), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
                              P->partials);
       } else {
         g = std::min<unsigned>(1024, blocks_for(n / 4));
         if (r == 0)
-          hipLaunchKernelGGL(fold_coeffs_q_kernel, dim3(g + qx), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
+          hipLaunchKernelGGL(fold_coeffs_q_kernel, zg(dim3(g + qx), P->jobs), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
                              P->js->rs + (rs_idx - 1), P->ep[cur ^ 1], P->ew[cur ^ 1], P->partials, qfold(t - 1));
         else
-          hipLaunchKernelGGL(fold_coeffs_kernel, dim3(g), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
+          hipLaunchKernelGGL(fold_coeffs_kernel, zg(dim3(g), P->jobs), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
                              P->js->rs + (rs_idx - 1), P->ep[cur ^ 1], P->ew[cur ^ 1], P->partials);
         n /= 2;
         cur ^= 1;
       }
-      hipLaunchKernelGGL(round_finalize_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js,
+      hipLaunchKernelGGL(round_finalize_kernel, zg(dim3(1), P->jobs), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js,
                          P->partials, g + qx, rs_idx);
       ++rs_idx;
     }
     if (r == 0)
-      hipLaunchKernelGGL(fold_q_kernel, dim3(blocks_for(n / 2) + QUERY_BLOCKS), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur],
+      hipLaunchKernelGGL(fold_q_kernel, zg(dim3(blocks_for(n / 2) + QUERY_BLOCKS), P->jobs), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur],
                          n, P->js->rs + (rs_idx - 1), P->ep[cur ^ 1], P->ew[cur ^ 1], qfold(f - 1));
     else
-      hipLaunchKernelGGL(fold_kernel, dim3(blocks_for(n / 2)), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
+      hipLaunchKernelGGL(fold_kernel, zg(dim3(blocks_for(n / 2)), P->jobs), dim3(SC_THREADS), 0, s, P->ep[cur], P->ew[cur], n,
                          P->js->rs + (rs_idx - 1), P->ep[cur ^ 1], P->ew[cur ^ 1]);
     n /= 2;
     cur ^= 1;
@@ -1317,16 +1324,16 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
       // combine_pows_scaled(weights, ws, scales) at k - 2f variables (prover.rs:556)
       const int mm = rp.kc - f;
       const uint32_t pm0 = mm / 2, pm1 = mm - mm / 2;
-      hipLaunchKernelGGL(pow_tables_kernel, dim3(blocks_for((uint64_t)rp.n_stir * ((1ull << pm0) + (1ull << pm1)))),
+      hipLaunchKernelGGL(pow_tables_kernel, zg(dim3(blocks_for((uint64_t)rp.n_stir * ((1ull << pm0) + (1ull << pm1)))), P->jobs),
                          dim3(SC_THREADS), 0, s, P->qa.ws, P->qa.scale, P->stir_count, pm0, pm1, P->powL, P->powR);
-      launch_combine(s, P->ew[cur], n, pm0, nullptr, 0u, 0, P->powL, P->powR, P->stir_count, pm1);
+      launch_combine(s, P->jobs, P->ew[cur], n, pm0, nullptr, 0u, 0, P->powL, P->powR, P->stir_count, pm1);
     }
   }
   // ---- final polynomial, queries and the test's checkpoint (whir.rs:314-355, test.rs:233) ----
   {
     const int lr = pl.n_rounds - 1;
     const bool base = pl.n_rounds == 0;
-    hipLaunchKernelGGL(final_phase_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->ep[cur],
+    hipLaunchKernelGGL(final_phase_kernel, zg(dim3(1), P->jobs), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->ep[cur],
                        (uint32_t)n, (uint32_t)pl.final_n_stir, (uint32_t)pl.final_kfd,
                        base ? P->cw0 : P->cw[lr & 1], base ? W : 4 * W, base ? P->tree0 : P->tree[lr & 1]);
   }
@@ -1546,13 +1553,13 @@ extern "C" int whir_prover_eval(whir_prover* P, const uint32_t* point, void* str
           gp.z[j].c[q] = w;
         }
     }
-    hipLaunchKernelGGL(claim_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, cb, gp,
+    hipLaunchKernelGGL(claim_begin_kernel, zg(dim3(1), P->jobs), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, cb, gp,
                        point ? 1u : 0u);
   }
   // this claim alone: its compression with the group's eq tables, then its value
   FSWITCH(f, launch_compress_hi<FL>(dim3(1u << cb.m1, 1), s, P->session_poly, cb.eq0, cb.eq1, cb.m0, P->partials,
-                                    P->desc, (uint32_t)c));
-  hipLaunchKernelGGL(claim_end_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots + c, 1u,
+                                    P->desc, (uint32_t)c, P->jobs));
+  hipLaunchKernelGGL(claim_end_kernel, zg(dim3(1), P->jobs), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots + c, 1u,
                      P->partials, 1u << cb.m1, (uint32_t)f, ClaimBegin{});
   HIP_TRY(hipGetLastError());
   P->next_claim = c + 1;
@@ -1566,7 +1573,7 @@ extern "C" int whir_prover_open(whir_prover* P, void* stream) {
   const int ng = (int)P->groups.size();
   int rc = WHIR_OK;
   if (P->n_user_groups < ng) {  // the OOD virtual claims (whir.rs:187-192), drawn
-    hipLaunchKernelGGL(claim_begin_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js,
+    hipLaunchKernelGGL(claim_begin_kernel, zg(dim3(1), P->jobs), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js,
                        group_begin(P, P->n_user_groups), GivenPoint{}, 0u);
     rc = enqueue_groups(P, P->session_poly, s, P->n_user_groups, ng);
   }
@@ -1633,4 +1640,5 @@ extern "C" int whir_device_synchronize(void) {
 }
 
 #include "server.h"
+#include "batch.h"
 #include "components.h"
