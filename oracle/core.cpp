@@ -10,7 +10,7 @@ namespace oracle {
 
 F two_adic_generator(unsigned bits) {
   // Plonky3 koala-bear: GENERATOR = 3, two-adicity 24.
-  return fpow(F::from_u32(3), (uint64_t)(P - 1) >> bits);
+  return fpow(F::from_u32(GENERATOR_CANON), (uint64_t)(P - 1) >> bits);
 }
 
 E emul(const E& a, const E& b) {
@@ -108,6 +108,17 @@ E StdRng::next_e() {
 // Canonical values of the internal diagonal V (Plonky3 koala-bear poseidon2.rs).
 // Each entry is sign * num / 2^sh.
 struct DiagEntry { int sign; uint32_t num; int sh; };
+#if defined(ORACLE_BABYBEAR)
+// Plonky3 baby-bear poseidon2.rs (recalled): parity unpinned.
+static const DiagEntry V16[16] = {
+    {-1, 2, 0}, {1, 1, 0}, {1, 2, 0}, {1, 1, 1}, {1, 3, 0}, {1, 4, 0}, {-1, 1, 1}, {-1, 3, 0},
+    {-1, 4, 0}, {1, 1, 8}, {1, 1, 2}, {1, 1, 3}, {1, 1, 27}, {-1, 1, 8}, {-1, 1, 4}, {-1, 1, 27}};
+static const DiagEntry V24[24] = {
+    {-1, 2, 0}, {1, 1, 0}, {1, 2, 0}, {1, 1, 1}, {1, 3, 0}, {1, 4, 0}, {-1, 1, 1}, {-1, 3, 0},
+    {-1, 4, 0}, {1, 1, 8}, {1, 1, 2}, {1, 1, 3}, {1, 1, 4}, {1, 1, 7}, {1, 1, 9}, {1, 1, 27},
+    {-1, 1, 8}, {-1, 1, 2}, {-1, 1, 3}, {-1, 1, 4}, {-1, 1, 5}, {-1, 1, 6}, {-1, 1, 7}, {-1, 1, 27}};
+constexpr int ROUNDS_P16 = 13, ROUNDS_P24 = 21;
+#else
 static const DiagEntry V16[16] = {
     {-1, 2, 0}, {1, 1, 0}, {1, 2, 0}, {1, 1, 1}, {1, 3, 0}, {1, 4, 0}, {-1, 1, 1}, {-1, 3, 0},
     {-1, 4, 0}, {1, 1, 8}, {1, 1, 3}, {1, 1, 24}, {-1, 1, 8}, {-1, 1, 3}, {-1, 1, 4}, {-1, 1, 24}};
@@ -115,6 +126,8 @@ static const DiagEntry V24[24] = {
     {-1, 2, 0}, {1, 1, 0}, {1, 2, 0}, {1, 1, 1}, {1, 3, 0}, {1, 4, 0}, {-1, 1, 1}, {-1, 3, 0},
     {-1, 4, 0}, {1, 1, 8}, {1, 1, 2}, {1, 1, 3}, {1, 1, 4}, {1, 1, 5}, {1, 1, 6}, {1, 1, 24},
     {-1, 1, 8}, {-1, 1, 3}, {-1, 1, 4}, {-1, 1, 5}, {-1, 1, 6}, {-1, 1, 7}, {-1, 1, 9}, {-1, 1, 24}};
+constexpr int ROUNDS_P16 = 20, ROUNDS_P24 = 23;
+#endif
 
 void internal_diag_canonical(int width, std::vector<F>& out) {
   const DiagEntry* v = width == 16 ? V16 : V24;
@@ -129,7 +142,7 @@ void internal_diag_canonical(int width, std::vector<F>& out) {
 Poseidon2 Poseidon2::make(int width, uint64_t seed) {
   Poseidon2 p;
   p.width = width;
-  p.rounds_p = width == 16 ? 20 : 23;
+  p.rounds_p = width == 16 ? ROUNDS_P16 : ROUNDS_P24;
   StdRng rng = StdRng::seed_from_u64(seed);
   p.ext_initial.resize(4 * width);
   p.ext_terminal.resize(4 * width);
@@ -161,20 +174,29 @@ void Poseidon2::internal_layer(F* s) const {
   for (int i = 0; i < width; ++i) s[i] = fadd(fmul(s[i], diag[i]), sum);
 }
 
-static inline F cube(F x) { return fmul(fsq(x), x); }
+// The S-box x^d, d the smallest exponent with gcd(d, p - 1) = 1: 3 for
+// KoalaBear, 7 for BabyBear.
+#if defined(ORACLE_BABYBEAR)
+static inline F sbox(F x) {
+  const F x2 = fsq(x), x3 = fmul(x2, x);
+  return fmul(x3, fsq(x2));
+}
+#else
+static inline F sbox(F x) { return fmul(fsq(x), x); }
+#endif
 
 void Poseidon2::permute(F* s) const {
   mds_light(s);
   for (int r = 0; r < 4; ++r) {
-    for (int i = 0; i < width; ++i) s[i] = cube(fadd(s[i], ext_initial[r * width + i]));
+    for (int i = 0; i < width; ++i) s[i] = sbox(fadd(s[i], ext_initial[r * width + i]));
     mds_light(s);
   }
   for (int r = 0; r < rounds_p; ++r) {
-    s[0] = cube(fadd(s[0], internal[r]));
+    s[0] = sbox(fadd(s[0], internal[r]));
     internal_layer(s);
   }
   for (int r = 0; r < 4; ++r) {
-    for (int i = 0; i < width; ++i) s[i] = cube(fadd(s[i], ext_terminal[r * width + i]));
+    for (int i = 0; i < width; ++i) s[i] = sbox(fadd(s[i], ext_terminal[r * width + i]));
     mds_light(s);
   }
 }

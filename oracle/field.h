@@ -20,7 +20,15 @@
 
 namespace oracle {
 
+#if defined(ORACLE_BABYBEAR)
+// BabyBear (p = 15 * 2^27 + 1), the reference's second field
+// (crates/transcript/src/lib.rs:389-396): GENERATOR 31, two-adicity 27, x^4 = 11.
+constexpr uint32_t P = 0x78000001u;
+constexpr uint32_t GENERATOR_CANON = 31;
+#else
 constexpr uint32_t P = 0x7f000001u;
+constexpr uint32_t GENERATOR_CANON = 3;
+#endif
 // -p^{-1} mod 2^32, used by the Montgomery reduction below.
 constexpr uint32_t P_NEG_INV = []() constexpr {
   uint32_t inv = 1;
@@ -72,8 +80,12 @@ inline F fone() { return F::from_u32(1); }
 // two_adic_generator(bits) = 3^((p-1)/2^bits).
 F two_adic_generator(unsigned bits);
 
-// ---- degree-4 extension, x^4 = W with W = 3 ----
+// ---- degree-4 extension, x^4 = W with W = 3 (BabyBear: 11) ----
+#if defined(ORACLE_BABYBEAR)
+constexpr uint32_t W_CANON = 11;
+#else
 constexpr uint32_t W_CANON = 3;
+#endif
 
 struct E {
   F c[4];

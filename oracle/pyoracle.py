@@ -16,7 +16,28 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
+FIELD = "koalabear"
 P = 0x7F000001
+_FIELDS = {"koalabear": ("liboracle.so", 0x7F000001), "babybear": ("liboracle_bb.so", 0x78000001)}
+_by_field = {}
+
+
+def for_field(name):
+    """This module's functions over the oracle built for `name` ("koalabear",
+    the default, or "babybear": the same restatement over BabyBear)."""
+    import importlib.util
+    import sys
+
+    if name == FIELD:
+        return sys.modules[__name__]
+    if name not in _by_field:
+        libname, p = _FIELDS[name]
+        spec = importlib.util.spec_from_file_location(f"oracle.pyoracle_{name}", __file__)
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        mod.FIELD, mod.P, mod._LIB_PATH, mod._lib = name, p, os.path.join(_HERE, libname), None
+        _by_field[name] = mod
+    return _by_field[name]
 UD, JB, CB = 0, 1, 2
 
 u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -166,8 +187,11 @@ def rand_ext(seed, n):
 
 
 # ---------------- poseidon2 ----------------
+ROUNDS_P = {"koalabear": {16: 20, 24: 23}, "babybear": {16: 13, 24: 21}}
+
+
 def poseidon2_constants(width, seed=1000):
-    rp = 20 if width == 16 else 23
+    rp = ROUNDS_P[FIELD][width]
     a = np.zeros(4 * width, np.uint32)
     b = np.zeros(4 * width, np.uint32)
     c = np.zeros(rp, np.uint32)

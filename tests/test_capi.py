@@ -5,6 +5,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -22,10 +24,11 @@ def test_header_declares_entry_points():
         assert must in names
 
 
-def test_library_exports_every_declared_symbol():
-    import whir_amd
+@pytest.mark.parametrize("field", ["koalabear", "babybear"])
+def test_library_exports_every_declared_symbol(field):
+    from whir_amd._lib import library_paths
 
-    lib = ctypes.CDLL(whir_amd.library_path)
+    lib = ctypes.CDLL(library_paths[field])
     for name in declared_symbols():
         assert hasattr(lib, name), name
 
@@ -41,13 +44,12 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     import whir_amd._lib as L
 
     monkeypatch.setattr(L, "library_path", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(L, "library_paths", {"babybear": str(tmp_path / "absent_bb.so")})
     monkeypatch.setattr(L, "_lib", None)
-    try:
-        L.lib()
-    except RuntimeError as e:
-        assert "no CPU fallback" in str(e)
-    else:
-        raise AssertionError("expected a RuntimeError")
+    monkeypatch.setattr(L, "_libs", {})
+    for field in ("koalabear", "babybear"):
+        with pytest.raises(RuntimeError, match="no CPU fallback"):
+            L.lib(field)
 
 
 def test_poly_rand_is_the_oracles_seeded_polynomial(oracle):
@@ -57,5 +59,10 @@ def test_poly_rand_is_the_oracles_seeded_polynomial(oracle):
 
     import whir_amd
 
+    bb = oracle.for_field("babybear")
     for seed, k in ((1, 10), (2, 0), (1000, 13), (7, 16)):
         assert np.array_equal(whir_amd.poly_rand(seed, k), oracle.rand_poly(seed, k)), (seed, k)
+        assert np.array_equal(whir_amd.poly_rand(seed, k, field="babybear"), bb.rand_poly(seed, k)), (seed, k)
+    # the two fields' libraries are distinct builds: rejection sampling and
+    # Montgomery words differ with the modulus
+    assert not np.array_equal(whir_amd.poly_rand(1, 10), whir_amd.poly_rand(1, 10, field="babybear"))

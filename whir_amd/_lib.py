@@ -5,6 +5,8 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.path.join(_HERE, "libwhir_amd.so")
+# the same C ABI built for each field the reference instantiates
+library_paths = {"koalabear": library_path, "babybear": os.path.join(_HERE, "libwhir_amd_bb.so")}
 
 STATUS = {
     0: "WHIR_OK",
@@ -146,23 +148,30 @@ SIGNATURES = [
 ]
 
 _lib = None
+_libs = {}
 
 
-def lib():
-    """The loaded library. Raises if it has not been built: there is no fallback."""
+def lib(field="koalabear"):
+    """The loaded library of `field`. Raises if it has not been built: there is
+    no fallback."""
     global _lib
-    if _lib is None:
-        if not os.path.exists(library_path):
+    if field == "koalabear" and _lib is not None:
+        return _lib
+    if field not in _libs:
+        path = library_path if field == "koalabear" else library_paths[field]
+        if not os.path.exists(path):
             raise RuntimeError(
-                f"{library_path} is missing; build it with `python whir_amd/build.py` "
+                f"{path} is missing; build it with `python whir_amd/build.py` "
                 "(or __graft_entry__.build()). The WHIR path has no CPU fallback.")
-        L = ctypes.CDLL(library_path)
+        L = ctypes.CDLL(path)
         for name, res, args in SIGNATURES:
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        _lib = L
-    return _lib
+        _libs[field] = L
+    if field == "koalabear":
+        _lib = _libs[field]
+    return _libs[field]
 
 
 def check(rc, where):

@@ -49,26 +49,28 @@ class Context:
     """whir_ctx: device, Poseidon2 constants (seed 1000 by default), twiddles.
     With autotune, workspaces created on it tune their NTT and Merkle shapes."""
 
-    def __init__(self, device=0, poseidon_seed=1000, autotune=False):
+    def __init__(self, device=0, poseidon_seed=1000, autotune=False, field="koalabear"):
+        self.field = field
+        self.L = lib(field)
         p = ctypes.c_void_p()
-        check(lib().whir_ctx_create(device, poseidon_seed, ctypes.byref(p)), "whir_ctx_create")
+        check(self.L.whir_ctx_create(device, poseidon_seed, ctypes.byref(p)), "whir_ctx_create")
         self.ptr = p.value
         if autotune:
-            check(lib().whir_ctx_set_autotune(self.ptr, 1), "whir_ctx_set_autotune")
+            check(self.L.whir_ctx_set_autotune(self.ptr, 1), "whir_ctx_set_autotune")
 
     def tuning(self):
         """The autotuner's choices so far (list of dicts)."""
         import json
 
         n = ctypes.c_size_t()
-        check(lib().whir_ctx_tuning(self.ptr, None, 0, ctypes.byref(n)), "whir_ctx_tuning")
+        check(self.L.whir_ctx_tuning(self.ptr, None, 0, ctypes.byref(n)), "whir_ctx_tuning")
         buf = ctypes.create_string_buffer(n.value + 1)
-        check(lib().whir_ctx_tuning(self.ptr, buf, n.value + 1, ctypes.byref(n)), "whir_ctx_tuning")
+        check(self.L.whir_ctx_tuning(self.ptr, buf, n.value + 1, ctypes.byref(n)), "whir_ctx_tuning")
         return json.loads(buf.value.decode())
 
     def close(self):
         if self.ptr:
-            lib().whir_ctx_destroy(self.ptr)
+            self.L.whir_ctx_destroy(self.ptr)
             self.ptr = None
 
     def __del__(self):
@@ -97,8 +99,8 @@ def dft_batch(mat, log_h, ctx=None):
     din = DeviceBuffer(m.nbytes)
     din.upload(m)
     dout = DeviceBuffer((width << log_h) * 4)
-    check(lib().whir_dft_batch(ctx.ptr, din.ptr, n_in, log_h, width, dout.ptr, None), "whir_dft_batch")
-    check(lib().whir_device_synchronize(), "sync")
+    check(ctx.L.whir_dft_batch(ctx.ptr, din.ptr, n_in, log_h, width, dout.ptr, None), "whir_dft_batch")
+    check(ctx.L.whir_device_synchronize(), "sync")
     return dout.download(np.uint32, width << log_h).reshape(1 << log_h, width)
 
 
@@ -113,7 +115,7 @@ def merkle_commit(rows, ctx=None):
     dr.upload(r)
     dl = DeviceBuffer((2 * h - 1) * 32)
     root = np.zeros(8, np.uint32)
-    check(lib().whir_merkle_commit(ctx.ptr, dr.ptr, log_h, w, dl.ptr, root.ctypes.data_as(_u32p), None),
+    check(ctx.L.whir_merkle_commit(ctx.ptr, dr.ptr, log_h, w, dl.ptr, root.ctypes.data_as(_u32p), None),
           "whir_merkle_commit")
     layers = dl.download(np.uint32, (2 * h - 1) * 8).reshape(2 * h - 1, 8)
     return root, layers
@@ -126,8 +128,8 @@ def poseidon2_permute(states, width, cooperative=False, ctx=None):
     d = DeviceBuffer(s.nbytes)
     d.upload(s)
     w = -16 if cooperative else width
-    check(lib().whir_poseidon2_permute(ctx.ptr, w, d.ptr, s.shape[0], None), "whir_poseidon2_permute")
-    check(lib().whir_device_synchronize(), "sync")
+    check(ctx.L.whir_poseidon2_permute(ctx.ptr, w, d.ptr, s.shape[0], None), "whir_poseidon2_permute")
+    check(ctx.L.whir_device_synchronize(), "sync")
     return d.download(np.uint32, s.size).reshape(s.shape)
 
 
@@ -160,7 +162,7 @@ def merkle_open(layers, rows, log_h, indices, ctx=None):
     di, idx = _dev(np.asarray(indices, np.uint32))
     per = row_words + 8 * log_h
     do = DeviceBuffer(max(4, len(idx) * per * 4))
-    out = _run(lib().whir_merkle_open(ctx.ptr, dl.ptr, log_h, dr.ptr, row_words, di.ptr, len(idx), do.ptr, None),
+    out = _run(ctx.L.whir_merkle_open(ctx.ptr, dl.ptr, log_h, dr.ptr, row_words, di.ptr, len(idx), do.ptr, None),
                "whir_merkle_open", do, len(idx) * per)
     return out.reshape(len(idx), per)
 
@@ -172,7 +174,7 @@ def eq_table(z, scale, ctx=None):
     k = zz.shape[0]
     sc, scp = _ext4(scale)
     do = DeviceBuffer((4 << k) * 4)
-    return _run(lib().whir_eq_table(ctx.ptr, dz.ptr, k, scp, do.ptr, None), "whir_eq_table", do,
+    return _run(ctx.L.whir_eq_table(ctx.ptr, dz.ptr, k, scp, do.ptr, None), "whir_eq_table", do,
                 4 << k).reshape(-1, 4)
 
 
@@ -182,7 +184,7 @@ def eval_ext(poly, z, ctx=None):
     dp, p = _dev(np.asarray(poly).reshape(-1, 4))
     dz, zz = _dev(np.asarray(z).reshape(-1, 4))
     do = DeviceBuffer(16)
-    return _run(lib().whir_eval_ext(ctx.ptr, dp.ptr, zz.shape[0], dz.ptr, do.ptr, None), "whir_eval_ext", do, 4)
+    return _run(ctx.L.whir_eval_ext(ctx.ptr, dp.ptr, zz.shape[0], dz.ptr, do.ptr, None), "whir_eval_ext", do, 4)
 
 
 def compress_hi(poly, k, z, ctx=None):
@@ -192,7 +194,7 @@ def compress_hi(poly, k, z, ctx=None):
     dz, zz = _dev(np.asarray(z).reshape(-1, 4))
     m = zz.shape[0]
     do = DeviceBuffer((4 << (k - m)) * 4)
-    return _run(lib().whir_compress_hi(ctx.ptr, dp.ptr, k, dz.ptr, m, do.ptr, None), "whir_compress_hi", do,
+    return _run(ctx.L.whir_compress_hi(ctx.ptr, dp.ptr, k, dz.ptr, m, do.ptr, None), "whir_compress_hi", do,
                 4 << (k - m)).reshape(-1, 4)
 
 
@@ -203,7 +205,7 @@ def compress_lo(poly, k, z, ctx=None):
     dz, zz = _dev(np.asarray(z).reshape(-1, 4))
     m = zz.shape[0]
     do = DeviceBuffer((4 << (k - m)) * 4)
-    return _run(lib().whir_compress_lo(ctx.ptr, dp.ptr, k, dz.ptr, m, do.ptr, None), "whir_compress_lo", do,
+    return _run(ctx.L.whir_compress_lo(ctx.ptr, dp.ptr, k, dz.ptr, m, do.ptr, None), "whir_compress_lo", do,
                 4 << (k - m)).reshape(-1, 4)
 
 
@@ -214,7 +216,7 @@ def sumcheck_coeffs(f, w, ctx=None):
     dw, _ = _dev(np.asarray(w).reshape(-1, 4))
     k = int(ff.shape[0]).bit_length() - 1
     do = DeviceBuffer(32)
-    return _run(lib().whir_sumcheck_coeffs(ctx.ptr, df.ptr, dw.ptr, k, do.ptr, None), "whir_sumcheck_coeffs", do,
+    return _run(ctx.L.whir_sumcheck_coeffs(ctx.ptr, df.ptr, dw.ptr, k, do.ptr, None), "whir_sumcheck_coeffs", do,
                 8).reshape(2, 4)
 
 
@@ -225,7 +227,7 @@ def fix_lo_var(poly, r, ctx=None):
     k = int(p.shape[0]).bit_length() - 1
     rr, rp = _ext4(r)
     do = DeviceBuffer((4 << (k - 1)) * 4)
-    return _run(lib().whir_fix_lo_var(ctx.ptr, dp.ptr, k, rp, do.ptr, None), "whir_fix_lo_var", do,
+    return _run(ctx.L.whir_fix_lo_var(ctx.ptr, dp.ptr, k, rp, do.ptr, None), "whir_fix_lo_var", do,
                 4 << (k - 1)).reshape(-1, 4)
 
 
@@ -237,7 +239,7 @@ def combine_eqs(w, points, alpha, ctx=None):
     pts = np.asarray(points, np.uint32).reshape(-1, k, 4)
     dpt, _ = _dev(pts)
     aa, ap = _ext4(alpha)
-    return _run(lib().whir_combine_eqs(ctx.ptr, dw.ptr, k, dpt.ptr, pts.shape[0], ap, None), "whir_combine_eqs",
+    return _run(ctx.L.whir_combine_eqs(ctx.ptr, dw.ptr, k, dpt.ptr, pts.shape[0], ap, None), "whir_combine_eqs",
                 dw, ww.size).reshape(-1, 4)
 
 
@@ -248,7 +250,7 @@ def combine_pows(w, variables, alpha, shift, ctx=None):
     k = int(ww.shape[0]).bit_length() - 1
     dv, vv = _dev(np.asarray(variables, np.uint32))
     aa, ap = _ext4(alpha)
-    return _run(lib().whir_combine_pows(ctx.ptr, dw.ptr, k, dv.ptr, vv.size, ap, shift, None), "whir_combine_pows",
+    return _run(ctx.L.whir_combine_pows(ctx.ptr, dw.ptr, k, dv.ptr, vv.size, ap, shift, None), "whir_combine_pows",
                 dw, ww.size).reshape(-1, 4)
 
 
@@ -259,25 +261,25 @@ class Prover:
         self.ctx = ctx or default_context()
         self.params = params
         p = ctypes.c_void_p()
-        check(lib().whir_prover_create(self.ctx.ptr, ctypes.byref(params), ctypes.byref(p)), "whir_prover_create")
+        check(self.ctx.L.whir_prover_create(self.ctx.ptr, ctypes.byref(params), ctypes.byref(p)), "whir_prover_create")
         self.ptr = p.value
-        self.capacity = lib().whir_prover_proof_capacity(self.ptr)
-        self.input_ptr = lib().whir_prover_input(self.ptr)
+        self.capacity = self.ctx.L.whir_prover_proof_capacity(self.ptr)
+        self.input_ptr = self.ctx.L.whir_prover_input(self.ptr)
 
     def upload(self, poly):
         a = _u32(poly)
         assert a.size == 1 << self.params.k
-        check(lib().whir_memcpy_h2d(self.input_ptr, a.ctypes.data, a.nbytes), "whir_memcpy_h2d")
+        check(self.ctx.L.whir_memcpy_h2d(self.input_ptr, a.ctypes.data, a.nbytes), "whir_memcpy_h2d")
 
     def enqueue(self, d_poly=None, stream=None):
-        check(lib().whir_prover_enqueue(self.ptr, d_poly or self.input_ptr, stream), "whir_prover_enqueue")
+        check(self.ctx.L.whir_prover_enqueue(self.ptr, d_poly or self.input_ptr, stream), "whir_prover_enqueue")
 
     def result(self):
         buf = np.zeros(self.capacity, np.uint8)
         ln = ctypes.c_size_t()
         cp = np.zeros(1, np.uint32)
         root = np.zeros(8, np.uint32)
-        check(lib().whir_prover_result(self.ptr, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
+        check(self.ctx.L.whir_prover_result(self.ptr, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
                                        cp.ctypes.data_as(_u32p), root.ctypes.data_as(_u32p)), "whir_prover_result")
         return bytes(buf[: ln.value]), int(cp[0]), root
 
@@ -297,25 +299,25 @@ class Prover:
                 st.output_buffer[i] = int(challenger["output_buffer"][i])
             st.input_len = int(challenger["input_len"])
             st.output_len = int(challenger["output_len"])
-        check(lib().whir_prover_commit(self.ptr, d_poly or self.input_ptr, ctypes.byref(st) if st else None, stream),
+        check(self.ctx.L.whir_prover_commit(self.ptr, d_poly or self.input_ptr, ctypes.byref(st) if st else None, stream),
               "whir_prover_commit")
 
     def eval(self, point=None, stream=None):
         """The next evaluation claim: at `point` (k_poly x 4 Montgomery words) or,
         with None, at expand(k_poly, draw())."""
         if point is None:
-            check(lib().whir_prover_eval(self.ptr, None, stream), "whir_prover_eval")
+            check(self.ctx.L.whir_prover_eval(self.ptr, None, stream), "whir_prover_eval")
         else:
             pt = _u32(point).reshape(-1)
-            check(lib().whir_prover_eval(self.ptr, pt.ctypes.data_as(_u32p), stream), "whir_prover_eval")
+            check(self.ctx.L.whir_prover_eval(self.ptr, pt.ctypes.data_as(_u32p), stream), "whir_prover_eval")
 
     def open(self, stream=None):
-        check(lib().whir_prover_open(self.ptr, stream), "whir_prover_open")
+        check(self.ctx.L.whir_prover_open(self.ptr, stream), "whir_prover_open")
 
     def challenger(self):
         """The transcript state open handed back, as whir_challenger fields."""
         st = whir_challenger()
-        check(lib().whir_prover_challenger(self.ptr, ctypes.byref(st)), "whir_prover_challenger")
+        check(self.ctx.L.whir_prover_challenger(self.ptr, ctypes.byref(st)), "whir_prover_challenger")
         return {"sponge_state": np.array(st.sponge_state[:], np.uint32),
                 "input_buffer": np.array(st.input_buffer[:], np.uint32),
                 "output_buffer": np.array(st.output_buffer[:], np.uint32),
@@ -324,26 +326,26 @@ class Prover:
     TIMER_LEAVES_BASE, TIMER_DFT_BASE = 0, 1
 
     def set_graph(self, enable=True):
-        check(lib().whir_prover_set_graph(self.ptr, int(enable)), "whir_prover_set_graph")
+        check(self.ctx.L.whir_prover_set_graph(self.ptr, int(enable)), "whir_prover_set_graph")
 
     def done(self):
-        rc = lib().whir_prover_query(self.ptr)
+        rc = self.ctx.L.whir_prover_query(self.ptr)
         if rc not in (0, 7):
             check(rc, "whir_prover_query")
         return rc == 0
 
     def set_timers(self, enable=True):
-        check(lib().whir_prover_set_timers(self.ptr, int(enable)), "whir_prover_set_timers")
+        check(self.ctx.L.whir_prover_set_timers(self.ptr, int(enable)), "whir_prover_set_timers")
 
     def timer(self, which):
         ms, n, work = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
-        check(lib().whir_prover_timer(self.ptr, which, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(work)),
+        check(self.ctx.L.whir_prover_timer(self.ptr, which, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(work)),
               "whir_prover_timer")
         return ms.value, n.value, work.value
 
     def close(self):
         if self.ptr:
-            lib().whir_prover_destroy(self.ptr)
+            self.ctx.L.whir_prover_destroy(self.ptr)
             self.ptr = None
 
     def __del__(self):
@@ -371,14 +373,14 @@ class StackedProver(Prover):
         st = whir_stack(len(kp), kp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(cp),
                         cp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cf.ctypes.data_as(_u8p))
         p = ctypes.c_void_p()
-        check(lib().whir_prover_create_stacked(self.ctx.ptr, ctypes.byref(params), ctypes.byref(st),
+        check(self.ctx.L.whir_prover_create_stacked(self.ctx.ptr, ctypes.byref(params), ctypes.byref(st),
                                                ctypes.byref(p)), "whir_prover_create_stacked")
         self.ptr = p.value
-        self.capacity = lib().whir_prover_proof_capacity(self.ptr)
-        self.input_ptr = lib().whir_prover_input(self.ptr)
+        self.capacity = self.ctx.L.whir_prover_proof_capacity(self.ptr)
+        self.input_ptr = self.ctx.L.whir_prover_input(self.ptr)
 
     def offset(self, poly_id):
-        return lib().whir_prover_stack_offset(self.ptr, poly_id)
+        return self.ctx.L.whir_prover_stack_offset(self.ptr, poly_id)
 
     def upload(self, polys):
         """Copies the polys to the device and lays them out with the zero tail."""
@@ -398,7 +400,7 @@ class StackedProver(Prover):
             ptrs = self._staged[1]
         else:
             ptrs = (ctypes.c_void_p * len(d_polys))(*d_polys)
-        check(lib().whir_prover_enqueue_stacked(self.ptr, ptrs, stream), "whir_prover_enqueue_stacked")
+        check(self.ctx.L.whir_prover_enqueue_stacked(self.ptr, ptrs, stream), "whir_prover_enqueue_stacked")
 
 
 class Batch:
@@ -410,27 +412,27 @@ class Batch:
         self.params = params
         self.jobs = int(jobs)
         p = ctypes.c_void_p()
-        check(lib().whir_batch_create(self.ctx.ptr, ctypes.byref(params), self.jobs, ctypes.byref(p)),
+        check(self.ctx.L.whir_batch_create(self.ctx.ptr, ctypes.byref(params), self.jobs, ctypes.byref(p)),
               "whir_batch_create")
         self.ptr = p.value
-        self.capacity = lib().whir_batch_proof_capacity(self.ptr)
+        self.capacity = self.ctx.L.whir_batch_proof_capacity(self.ptr)
         if graph:
-            check(lib().whir_batch_set_graph(self.ptr, 1), "whir_batch_set_graph")
+            check(self.ctx.L.whir_batch_set_graph(self.ptr, 1), "whir_batch_set_graph")
 
     def input_ptr(self, job):
-        return lib().whir_batch_input(self.ptr, job)
+        return self.ctx.L.whir_batch_input(self.ptr, job)
 
     def upload(self, job, poly):
         a = _u32(poly)
         assert a.size == 1 << self.params.k
-        check(lib().whir_memcpy_h2d(self.input_ptr(job), a.ctypes.data, a.nbytes), "whir_memcpy_h2d")
+        check(self.ctx.L.whir_memcpy_h2d(self.input_ptr(job), a.ctypes.data, a.nbytes), "whir_memcpy_h2d")
 
     def enqueue(self, d_polys=None, stream=None):
         ptrs = None if d_polys is None else (ctypes.c_void_p * self.jobs)(*d_polys)
-        check(lib().whir_batch_enqueue(self.ptr, ptrs, stream), "whir_batch_enqueue")
+        check(self.ctx.L.whir_batch_enqueue(self.ptr, ptrs, stream), "whir_batch_enqueue")
 
     def done(self):
-        rc = lib().whir_batch_query(self.ptr)
+        rc = self.ctx.L.whir_batch_query(self.ptr)
         if rc not in (0, 7):
             check(rc, "whir_batch_query")
         return rc == 0
@@ -440,13 +442,13 @@ class Batch:
         ln = ctypes.c_size_t()
         cp = np.zeros(1, np.uint32)
         root = np.zeros(8, np.uint32)
-        check(lib().whir_batch_result(self.ptr, job, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
+        check(self.ctx.L.whir_batch_result(self.ptr, job, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
                                       cp.ctypes.data_as(_u32p), root.ctypes.data_as(_u32p)), "whir_batch_result")
         return bytes(buf[: ln.value]), int(cp[0]), root
 
     def close(self):
         if self.ptr:
-            lib().whir_batch_destroy(self.ptr)
+            self.ctx.L.whir_batch_destroy(self.ptr)
             self.ptr = None
 
     def __del__(self):
@@ -467,30 +469,30 @@ class MultiBatch:
         self.n = len(params_list)
         arr = (whir_params * self.n)(*params_list)
         p = ctypes.c_void_p()
-        check(lib().whir_multi_create(self.ctx.ptr, arr, self.n, streams, ctypes.byref(p)), "whir_multi_create")
+        check(self.ctx.L.whir_multi_create(self.ctx.ptr, arr, self.n, streams, ctypes.byref(p)), "whir_multi_create")
         self.ptr = p.value
-        check(lib().whir_multi_set_graph(self.ptr, int(graph)), "whir_multi_set_graph")
+        check(self.ctx.L.whir_multi_set_graph(self.ptr, int(graph)), "whir_multi_set_graph")
 
     def input_ptr(self, job):
-        return lib().whir_multi_input(self.ptr, job)
+        return self.ctx.L.whir_multi_input(self.ptr, job)
 
     def enqueue(self, d_polys=None, stream=None):
         ptrs = None if d_polys is None else (ctypes.c_void_p * self.n)(*d_polys)
-        check(lib().whir_multi_enqueue(self.ptr, ptrs, stream), "whir_multi_enqueue")
+        check(self.ctx.L.whir_multi_enqueue(self.ptr, ptrs, stream), "whir_multi_enqueue")
 
     def result(self, job):
-        cap = lib().whir_multi_proof_capacity(self.ptr, job)
+        cap = self.ctx.L.whir_multi_proof_capacity(self.ptr, job)
         buf = np.zeros(cap, np.uint8)
         ln = ctypes.c_size_t()
         cp = np.zeros(1, np.uint32)
         root = np.zeros(8, np.uint32)
-        check(lib().whir_multi_result(self.ptr, job, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
+        check(self.ctx.L.whir_multi_result(self.ptr, job, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
                                       cp.ctypes.data_as(_u32p), root.ctypes.data_as(_u32p)), "whir_multi_result")
         return bytes(buf[: ln.value]), int(cp[0]), root
 
     def close(self):
         if self.ptr:
-            lib().whir_multi_destroy(self.ptr)
+            self.ctx.L.whir_multi_destroy(self.ptr)
             self.ptr = None
 
     def __del__(self):
@@ -530,11 +532,12 @@ class Whir:
         return pr.result()
 
 
-def poly_rand(seed, k):
+def poly_rand(seed, k, field="koalabear"):
     """Poly::<F>::rand(&mut rng(seed), k) (crates/poly/src/poly.rs:47-52): the
-    polynomial the reference's tests prove for `seed`, as Montgomery words."""
+    polynomial the reference's tests prove for `seed`, as Montgomery words of
+    `field`."""
     out = np.empty(1 << k, np.uint32)
-    check(lib().whir_poly_rand(seed, k, out.ctypes.data_as(_u32p)), "whir_poly_rand")
+    check(lib(field).whir_poly_rand(seed, k, out.ctypes.data_as(_u32p)), "whir_poly_rand")
     return out
 
 
@@ -552,38 +555,38 @@ class Server:
     def __init__(self, max_slots=6, hbm_budget_bytes=64 << 30, ctx=None):
         self.ctx = ctx or default_context()
         p = ctypes.c_void_p()
-        check(lib().whir_server_create(self.ctx.ptr, max_slots, hbm_budget_bytes, ctypes.byref(p)),
+        check(self.ctx.L.whir_server_create(self.ctx.ptr, max_slots, hbm_budget_bytes, ctypes.byref(p)),
               "whir_server_create")
         self.ptr = p.value
         self._ids = (ctypes.c_int64 * 256)()
 
     def submit(self, p, d_poly):
         jid = ctypes.c_int64()
-        check(lib().whir_server_submit(self.ptr, ctypes.byref(p), d_poly, ctypes.byref(jid)), "whir_server_submit")
+        check(self.ctx.L.whir_server_submit(self.ptr, ctypes.byref(p), d_poly, ctypes.byref(jid)), "whir_server_submit")
         return jid.value
 
     def poll(self):
         n = ctypes.c_int()
-        check(lib().whir_server_poll(self.ptr, self._ids, 256, ctypes.byref(n)), "whir_server_poll")
+        check(self.ctx.L.whir_server_poll(self.ptr, self._ids, 256, ctypes.byref(n)), "whir_server_poll")
         return [self._ids[i] for i in range(n.value)]
 
     def take(self, job_id, capacity):
         buf = np.zeros(capacity, np.uint8)
         ln = ctypes.c_size_t()
         cp = np.zeros(1, np.uint32)
-        check(lib().whir_server_take(self.ptr, job_id, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
+        check(self.ctx.L.whir_server_take(self.ptr, job_id, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
                                      cp.ctypes.data_as(_u32p)), "whir_server_take")
         return bytes(buf[: ln.value]), int(cp[0])
 
     def stats(self):
         r, q, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
-        check(lib().whir_server_stats(self.ptr, ctypes.byref(r), ctypes.byref(q), ctypes.byref(b)),
+        check(self.ctx.L.whir_server_stats(self.ptr, ctypes.byref(r), ctypes.byref(q), ctypes.byref(b)),
               "whir_server_stats")
         return r.value, q.value, b.value
 
     def close(self):
         if self.ptr:
-            lib().whir_server_destroy(self.ptr)
+            self.ctx.L.whir_server_destroy(self.ptr)
             self.ptr = None
 
     def __del__(self):

@@ -1,4 +1,5 @@
-"""Builds the HIP library whir_amd/libwhir_amd.so for gfx950 with hipcc.
+"""Builds the HIP libraries for gfx950 with hipcc: whir_amd/libwhir_amd.so
+(KoalaBear) and whir_amd/libwhir_amd_bb.so (the same sources for BabyBear).
 
 All kernels live in one translation unit (csrc/whir_amd.hip) because they share
 the __constant__ Poseidon2 tables. Packed-FP32 instruction selection is
@@ -20,6 +21,8 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libwhir_amd.so")
+# the same sources built for the reference's second field (BabyBear)
+LIB_BB = os.path.join(HERE, "libwhir_amd_bb.so")
 STATS = os.path.join(HERE, "kernel_stats.json")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("WHIR_ARCH", "gfx950")
@@ -58,6 +61,19 @@ def count_valu(asm, prefix):
 
 
 def build(verbose=False, extra_flags=()):
+    """Both libraries: KoalaBear (with the VALU counts bench.py reads) and BabyBear."""
+    build_field("koalabear", verbose, extra_flags)
+    build_field("babybear", verbose, extra_flags)
+    return LIB
+
+
+def build_field(field, verbose=False, extra_flags=()):
+    if field == "babybear":
+        return _build_one(LIB_BB, ["-DWHIR_FIELD_BABYBEAR", *extra_flags], verbose, stats=False)
+    return _build_one(LIB, list(extra_flags), verbose, stats=True)
+
+
+def _build_one(lib_path, extra_flags, verbose, stats):
     src = os.path.join(CSRC, "whir_amd.hip")
     with tempfile.TemporaryDirectory(prefix="whir_build_") as tmp:
         out = os.path.join(tmp, "libwhir_amd.so")
@@ -66,17 +82,18 @@ def build(verbose=False, extra_flags=()):
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd, cwd=tmp)
         asm_files = [f for f in os.listdir(tmp) if f.endswith(".s") and ARCH in f]
-        stats = {"arch": ARCH}
-        if asm_files:
+        counts = {"arch": ARCH}
+        if stats and asm_files:
             with open(os.path.join(tmp, asm_files[0])) as fh:
                 asm = fh.read()
             for key, sym in COUNTED.items():
-                stats[key + "_valu_per_lane"] = count_valu(asm, sym)
-        shutil.copyfile(out, LIB + ".tmp")
-        os.replace(LIB + ".tmp", LIB)
-    with open(STATS, "w") as fh:
-        json.dump(stats, fh, indent=1)
-    return LIB
+                counts[key + "_valu_per_lane"] = count_valu(asm, sym)
+        shutil.copyfile(out, lib_path + ".tmp")
+        os.replace(lib_path + ".tmp", lib_path)
+    if stats:
+        with open(STATS, "w") as fh:
+            json.dump(counts, fh, indent=1)
+    return lib_path
 
 
 if __name__ == "__main__":

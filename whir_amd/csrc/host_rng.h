@@ -3,7 +3,7 @@
 // (crates/whir/src/test.rs:195-196, crates/common/src/test.rs:1-4):
 //   * StdRng (rand 0.10) = ChaCha12, key from rand_core's PCG32 seed_from_u64
 //     expansion, 64-bit block counter, stream 0, keystream read as LE u32 words;
-//   * StandardUniform<KoalaBear>: v = next_u32() >> 1, rejected unless v < p,
+//   * StandardUniform<KoalaBear | BabyBear>: v = next_u32() >> 1, rejected unless v < p,
 //     taken as the Montgomery representative;
 //   * order: 4 initial external rounds (W each), 4 terminal, R_P internal.
 #pragma once
@@ -28,10 +28,11 @@ class ChaChaRng {
     if (pos_ == 16) refill();
     return buf_[pos_++];
   }
-  uint32_t next_koalabear() {
+  // StandardUniform of the build's field (kb::P): the Montgomery representative
+  uint32_t next_field() {
     for (;;) {
       const uint32_t v = next_u32() >> 1;
-      if (v < 0x7f000001u) return v;
+      if (v < kb::P) return v;
     }
   }
 

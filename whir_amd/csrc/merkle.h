@@ -9,7 +9,9 @@
 #include "field.h"
 #include "poseidon2.h"
 #include "tx.h"
+#if !defined(WHIR_FIELD_BABYBEAR)
 #include "p2pair.h"
+#endif
 
 namespace kb {
 
@@ -73,6 +75,7 @@ __global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_kernel(JP<const u
   po[1] = make_uint4(o[4], o[5], o[6], o[7]);
 }
 
+#if !defined(WHIR_FIELD_BABYBEAR)  // two lanes per permutation: KoalaBear lazy arithmetic (p2pair.h)
 // Two lanes per row (p2pair.h): lane 0 takes words 0..11 of each 16-word chunk,
 // lane 1 words 12..15 (its words 0..3; its words 4..11 are the capacity).
 // ROW_WORDS is a multiple of 16.
@@ -141,6 +144,8 @@ __global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_pair_kernel(JP<co
     po[1] = make_uint4(s[4], s[5], s[6], s[7]);
   }
 }
+
+#endif
 
 // Upper layers with cooperative compressions (16 lanes per P16, tx.h), which
 // cut the per-layer latency of the narrow top of the tree. Under split-half

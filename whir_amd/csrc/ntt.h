@@ -55,8 +55,8 @@ struct Omega16 {
 };
 constexpr Omega16 make_omega16() {
   Omega16 t{};
-  const uint32_t g3 = (uint32_t)(((uint64_t)3 << 32) % P);  // 3 in Montgomery form
-  const uint32_t w16 = cx_pow(g3, (uint64_t)(P - 1) >> 4);  // two_adic_generator(4)
+  const uint32_t g = (uint32_t)(((uint64_t)GENERATOR << 32) % P);  // the generator in Montgomery form
+  const uint32_t w16 = cx_pow(g, (uint64_t)(P - 1) >> 4);          // two_adic_generator(4)
   uint32_t x = ONE;
   for (int i = 0; i < 16; ++i) {
     t.w[i] = x;

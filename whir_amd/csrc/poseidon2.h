@@ -32,6 +32,70 @@
 
 namespace kb {
 
+KB_HD uint64_t mad64(uint32_t a, uint32_t b, uint64_t c) { return (uint64_t)a * b + c; }
+
+// a * K + c in one v_mad_u64_u32 with K an inline constant (1 or 2); written
+// as inline ALU asm so that the compiler keeps one instruction per term instead
+// of factoring the sums into 64 x 32-bit products.
+template <int K>
+KB_HD uint64_t mad_k(uint32_t a, uint64_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t d;
+  asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(d) : "v"(a), "i"(K), "v"(c) : "vcc");
+  return d;
+#else
+  return (uint64_t)a * K + c;
+#endif
+}
+// a * K (zero-extended to 64 bits) in one instruction.
+template <int K>
+KB_HD uint64_t mul_k(uint32_t a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t d;
+  asm("v_mad_u64_u32 %0, vcc, %1, %2, 0" : "=v"(d) : "v"(a), "i"(K) : "vcc");
+  return d;
+#else
+  return (uint64_t)a * K;
+#endif
+}
+// c + u for a wave-uniform u (an SGPR operand), one instruction.
+KB_HD uint64_t add_u(uint64_t c, uint32_t u) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t d;
+  asm("v_mad_u64_u32 %0, vcc, %1, 1, %2" : "=v"(d) : "s"(u), "v"(c) : "vcc");
+  return d;
+#else
+  return c + u;
+#endif
+}
+// a * b + c for a wave-uniform b (an SGPR operand), one instruction.
+KB_HD uint64_t mad_u(uint32_t a, uint32_t b, uint64_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint64_t d;
+  asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c) : "vcc");
+  return d;
+#else
+  return (uint64_t)a * b + c;
+#endif
+}
+// T * R^-1 mod p, in [0, T / R + p) for T < p R.
+KB_HD uint32_t redc(uint64_t T) {
+  const uint32_t m = (uint32_t)T * NPINV;
+  return (uint32_t)(mad64(m, P, T) >> 32);
+}
+KB_HD uint32_t mulr(uint32_t a, uint32_t b) { return redc((uint64_t)a * b); }  // needs a * b < p R
+KB_HD uint32_t canon2(uint32_t t) {  // [0, 2p) -> [0, p)
+  const uint32_t u = t - P;
+  return t < u ? t : u;
+}
+
+}  // namespace kb
+
+#if defined(WHIR_FIELD_BABYBEAR)
+#include "poseidon2_bb.h"
+#else
+namespace kb {
+
 // The plain round constants (Montgomery words, the reference's draw order);
 // host side only: the kernels use the derived Poseidon2Lazy tables.
 struct Poseidon2Consts {
@@ -95,63 +159,6 @@ template <int W>
 __device__ __forceinline__ const Poseidon2Lazy<W>& lazy_consts() {
   if constexpr (W == 16) return c_p2l.w16;
   else return c_p2l.w24;
-}
-
-KB_HD uint64_t mad64(uint32_t a, uint32_t b, uint64_t c) { return (uint64_t)a * b + c; }
-
-// a * K + c in one v_mad_u64_u32 with K an inline constant (1 or 2); written
-// as inline ALU asm so that the compiler keeps one instruction per term instead
-// of factoring the sums into 64 x 32-bit products.
-template <int K>
-KB_HD uint64_t mad_k(uint32_t a, uint64_t c) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  uint64_t d;
-  asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(d) : "v"(a), "i"(K), "v"(c) : "vcc");
-  return d;
-#else
-  return (uint64_t)a * K + c;
-#endif
-}
-// a * K (zero-extended to 64 bits) in one instruction.
-template <int K>
-KB_HD uint64_t mul_k(uint32_t a) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  uint64_t d;
-  asm("v_mad_u64_u32 %0, vcc, %1, %2, 0" : "=v"(d) : "v"(a), "i"(K) : "vcc");
-  return d;
-#else
-  return (uint64_t)a * K;
-#endif
-}
-// c + u for a wave-uniform u (an SGPR operand), one instruction.
-KB_HD uint64_t add_u(uint64_t c, uint32_t u) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  uint64_t d;
-  asm("v_mad_u64_u32 %0, vcc, %1, 1, %2" : "=v"(d) : "s"(u), "v"(c) : "vcc");
-  return d;
-#else
-  return c + u;
-#endif
-}
-// a * b + c for a wave-uniform b (an SGPR operand), one instruction.
-KB_HD uint64_t mad_u(uint32_t a, uint32_t b, uint64_t c) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  uint64_t d;
-  asm("v_mad_u64_u32 %0, vcc, %1, %2, %3" : "=v"(d) : "v"(a), "s"(b), "v"(c) : "vcc");
-  return d;
-#else
-  return (uint64_t)a * b + c;
-#endif
-}
-// T * R^-1 mod p, in [0, T / R + p) for T < p R.
-KB_HD uint32_t redc(uint64_t T) {
-  const uint32_t m = (uint32_t)T * NPINV;
-  return (uint32_t)(mad64(m, P, T) >> 32);
-}
-KB_HD uint32_t mulr(uint32_t a, uint32_t b) { return redc((uint64_t)a * b); }  // needs a * b < p R
-KB_HD uint32_t canon2(uint32_t t) {  // [0, 2p) -> [0, p)
-  const uint32_t u = t - P;
-  return t < u ? t : u;
 }
 
 // internal_lazy's sum without x[0] as two interleaved chains, sa = x1 + x3 +
@@ -482,3 +489,4 @@ __device__ __forceinline__ void compress(const uint32_t (&l)[8], const uint32_t 
 }
 
 }  // namespace kb
+#endif

@@ -85,7 +85,7 @@ struct ClaimScratch {
 };
 
 // Extension squaring spread over the four lanes of a quad: lane q holds and
-// returns coefficient q. With x^4 = W = 3, coefficient q of a^2 is
+// returns coefficient q. With x^4 = W, coefficient q of a^2 is
 //   sum_{i <= q} a_i a_(q-i)  +  W sum_{i > q} a_i a_(q-i+4),
 // each sum below 4 p^2, so one reduce_sum4 each; the factors come from DPP
 // quad broadcasts and rotations (no memory). The same canonical words as esqr
@@ -103,7 +103,7 @@ __device__ __forceinline__ uint32_t esqr_quad(uint32_t a, uint32_t q) {
   const uint64_t U = (uint64_t)a0 * a + (uint64_t)a1 * u1 + (uint64_t)a2 * u2 + (uint64_t)a3 * u3;
   const uint64_t V = (uint64_t)a1 * v1 + (uint64_t)a2 * v2 + (uint64_t)a3 * v3;
   const uint32_t u = reduce_sum4(U), v = reduce_sum4(V);
-  return add(u, add(dbl(v), v));
+  return add(u, mul_w(v));
 }
 
 // pt[i] = z^(2^i) for i < k (expand, whir.rs:227-229 and test.rs:225-226),

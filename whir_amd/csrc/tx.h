@@ -41,6 +41,66 @@ constexpr int DPP_QUAD_XOR1 = 1 | (0 << 2) | (3 << 4) | (2 << 6);
 constexpr int DPP_QUAD_XOR2 = 2 | (3 << 2) | (0 << 4) | (1 << 6);
 constexpr int DPP_ROR(int n) { return 0x120 + n; }
 
+#if defined(WHIR_FIELD_BABYBEAR)
+// Cooperative Poseidon2-16 over BabyBear (plain Montgomery arithmetic on
+// canonical words): lane i (mod 16) holds state word i; the MDS-light chunk
+// and column sums and the internal layer's row sum are DPP moves and modular
+// additions inside each 16-lane row.
+struct CoopP16 {
+  uint32_t ei[4], et[4];  // this lane's external round constants (initial, terminal)
+  uint32_t d;             // V_lane (Montgomery)
+
+  __device__ __forceinline__ void load() {
+    const Poseidon2Plain<16>& C = c_p2b.w16;
+    const uint32_t i = threadIdx.x & 15;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      ei[r] = C.ext_init[r][i];
+      et[r] = C.ext_term[r][i];
+    }
+    d = C.diag[i];
+  }
+};
+
+__device__ __forceinline__ uint32_t mds16_coop_plain(uint32_t x) {
+  uint32_t t = add(x, dpp<DPP_QUAD_XOR1>(x));
+  t = add(t, dpp<DPP_QUAD_XOR2>(t));  // the chunk's sum
+  const uint32_t y = add(add(t, x), dbl(dpp<DPP_QUAD_NEXT>(x)));
+  uint32_t u = add(y, dpp<DPP_ROR(4)>(y));
+  u = add(u, dpp<DPP_ROR(8)>(u));  // the chunks' words at this position
+  return add(y, u);
+}
+
+// In and out: the lane's state word in [0, p).
+__device__ __forceinline__ void permute16_coop(uint32_t& s, const CoopP16& K) {
+  const Poseidon2Plain<16>& C = c_p2b.w16;
+  const bool lane0 = (threadIdx.x & 15) == 0;
+  uint32_t x = mds16_coop_plain(s);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) x = mds16_coop_plain(sbox7(add(x, K.ei[r])));
+#pragma unroll 1
+  for (int r = 0; r < Poseidon2Plain<16>::RP; ++r) {
+    const uint32_t c = sbox7(add(x, C.internal[r]));
+    x = lane0 ? c : x;
+    uint32_t sum = add(x, dpp<DPP_ROR(8)>(x));
+    sum = add(sum, dpp<DPP_ROR(4)>(sum));
+    sum = add(sum, dpp<DPP_ROR(2)>(sum));
+    sum = add(sum, dpp<DPP_ROR(1)>(sum));
+    x = add(mul(x, K.d), sum);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) x = mds16_coop_plain(sbox7(add(x, K.et[r])));
+  s = x;
+}
+
+using CoopLane = CoopP16;
+
+__device__ __noinline__ uint32_t permute16_coop_shared(uint32_t s, CoopLane L) {
+  permute16_coop(s, L);
+  return s;
+}
+
+#else
 // Cooperative Poseidon2-16: lane i (mod 16) holds state word i, with the
 // lazy reduction and scaled state of permute_lazy (poseidon2.h). The per-lane
 // constants are loaded into registers once per kernel (CoopP16::load), so a
@@ -167,6 +227,8 @@ __device__ __noinline__ uint32_t permute16_coop_shared(uint32_t s, CoopLane L) {
   permute16_coop(s, K);
   return s;
 }
+
+#endif  // WHIR_FIELD_BABYBEAR
 
 // One wavefront runs the transcript in uniform control flow. The sponge state
 // and both rate buffers are lane-distributed so nothing is indexed dynamically:

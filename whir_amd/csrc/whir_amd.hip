@@ -63,7 +63,11 @@ struct NttConfig {
 // cooperative LDS-resident subtrees of up to COOP_MAX_NODES nodes per launch
 // (the latency-bound top of the tree).
 struct MerkleConfig {
+#if defined(WHIR_FIELD_BABYBEAR)
+  uint64_t pair_max = 0;  // no two-lane kernels in this build
+#else
   uint64_t pair_max = 1ull << 15;
+#endif
   uint64_t coop_from = 1ull << 15;
 };
 
@@ -96,23 +100,46 @@ static MerkleConfig merkle_config(whir_ctx* c, int log_h, uint32_t row_words) {
   return it == c->merkle_tuned.end() ? c->merkle : it->second;
 }
 
+#if defined(WHIR_FIELD_BABYBEAR)
+// Poseidon2BabyBear<16|24>::new_from_rng_128(StdRng::seed_from_u64(seed)): per
+// width a fresh generator, 4 initial external rounds of W constants, 4
+// terminal, R_P internal (the same draw order as KoalaBear's).
+template <int W>
+static void plain_consts_draw(uint64_t seed, Poseidon2Plain<W>& C) {
+  whir_host::ChaChaRng r(seed);
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < W; ++j) C.ext_init[i][j] = r.next_field();
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < W; ++j) C.ext_term[i][j] = r.next_field();
+  for (int i = 0; i < Poseidon2Plain<W>::RP; ++i) C.internal[i] = r.next_field();
+  plain_diag_build<W>(C);
+}
+
+static int upload_constants(uint64_t seed) {
+  static Poseidon2PlainConsts pc;
+  plain_consts_draw<16>(seed, pc.w16);
+  plain_consts_draw<24>(seed, pc.w24);
+  HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_p2b), &pc, sizeof(pc)));
+  return WHIR_OK;
+}
+#else
 static int upload_constants(uint64_t seed) {
   Poseidon2Consts c;
   {
     whir_host::ChaChaRng r(seed);
     for (int i = 0; i < 4; ++i)
-      for (int j = 0; j < 16; ++j) c.ext16_init[i][j] = r.next_koalabear();
+      for (int j = 0; j < 16; ++j) c.ext16_init[i][j] = r.next_field();
     for (int i = 0; i < 4; ++i)
-      for (int j = 0; j < 16; ++j) c.ext16_term[i][j] = r.next_koalabear();
-    for (int i = 0; i < 20; ++i) c.int16[i] = r.next_koalabear();
+      for (int j = 0; j < 16; ++j) c.ext16_term[i][j] = r.next_field();
+    for (int i = 0; i < 20; ++i) c.int16[i] = r.next_field();
   }
   {
     whir_host::ChaChaRng r(seed);
     for (int i = 0; i < 4; ++i)
-      for (int j = 0; j < 24; ++j) c.ext24_init[i][j] = r.next_koalabear();
+      for (int j = 0; j < 24; ++j) c.ext24_init[i][j] = r.next_field();
     for (int i = 0; i < 4; ++i)
-      for (int j = 0; j < 24; ++j) c.ext24_term[i][j] = r.next_koalabear();
-    for (int i = 0; i < 23; ++i) c.int24[i] = r.next_koalabear();
+      for (int j = 0; j < 24; ++j) c.ext24_term[i][j] = r.next_field();
+    for (int i = 0; i < 23; ++i) c.int24[i] = r.next_field();
   }
   {
     static Poseidon2LazyConsts lz;
@@ -122,6 +149,7 @@ static int upload_constants(uint64_t seed) {
   }
   return WHIR_OK;
 }
+#endif
 
 extern "C" int whir_ctx_create(int device, uint64_t seed, whir_ctx** out) {
   if (!out) return WHIR_EINVAL;
@@ -300,6 +328,7 @@ extern "C" int whir_dft_batch(whir_ctx* c, const uint32_t* d_in, uint64_t n_in_r
 static int leaves_enqueue(const uint32_t* rows, uint64_t n_rows, uint32_t row_words, uint32_t* leaves,
                           hipStream_t s, const MerkleConfig& mc, uint32_t jobs) {
   const dim3 grid((unsigned)((n_rows + MERKLE_THREADS - 1) / MERKLE_THREADS), 1, jobs), block(MERKLE_THREADS);
+#if !defined(WHIR_FIELD_BABYBEAR)
   if (n_rows <= mc.pair_max && row_words % 16 == 0) {
     const dim3 grid2((unsigned)((2 * n_rows + MERKLE_THREADS - 1) / MERKLE_THREADS), 1, jobs);
     switch (row_words) {
@@ -310,6 +339,7 @@ static int leaves_enqueue(const uint32_t* rows, uint64_t n_rows, uint32_t row_wo
       default: break;
     }
   }
+#endif
   switch (row_words) {
 #define LEAF_CASE(W) \
   case W: hipLaunchKernelGGL(merkle_leaves_kernel<W>, grid, block, 0, s, rows, n_rows, leaves); break;
@@ -338,10 +368,12 @@ static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, u
   uint64_t n = H;
   while (n > std::max<uint64_t>(mc.coop_from, COOP_MAX_NODES)) {
     const uint64_t half = n / 2;
+#if !defined(WHIR_FIELD_BABYBEAR)
     if (half <= mc.pair_max)
       hipLaunchKernelGGL(merkle_layer_pair_kernel, dim3((unsigned)((2 * half + MERKLE_THREADS - 1) / MERKLE_THREADS), 1, jobs),
                          dim3(MERKLE_THREADS), 0, s, cur, half, cur + n * 8);
     else
+#endif
       hipLaunchKernelGGL(merkle_layer_kernel, dim3((unsigned)((half + MERKLE_THREADS - 1) / MERKLE_THREADS), 1, jobs),
                          dim3(MERKLE_THREADS), 0, s, cur, half, cur + n * 8);
     cur += n * 8;
@@ -872,10 +904,15 @@ static std::string merkle_signature(int log_h, uint32_t row_words, const MerkleC
 static std::vector<MerkleConfig> merkle_candidates(int log_h, uint32_t row_words) {
   std::vector<MerkleConfig> out;
   std::vector<std::string> seen;
-  for (int pm : {15, 14, 16, 13})
+#if defined(WHIR_FIELD_BABYBEAR)
+  const int pair_logs[] = {-1};  // no two-lane kernels in this build
+#else
+  const int pair_logs[] = {15, 14, 16, 13};
+#endif
+  for (int pm : pair_logs)
     for (int cf : {15, 13, 14, 16, 17}) {
       MerkleConfig mc;
-      mc.pair_max = 1ull << pm;
+      mc.pair_max = pm < 0 ? 0 : 1ull << pm;
       mc.coop_from = 1ull << cf;
       const std::string sig = merkle_signature(log_h, row_words, mc);
       if (std::find(seen.begin(), seen.end(), sig) != seen.end()) continue;
@@ -1608,7 +1645,7 @@ extern "C" int whir_poly_rand(uint64_t seed, uint32_t k, uint32_t* out) {
   if (!out || k > 32) return WHIR_EINVAL;
   whir_host::ChaChaRng r(seed);
   const uint64_t n = 1ull << k;
-  for (uint64_t i = 0; i < n; ++i) out[i] = r.next_koalabear();
+  for (uint64_t i = 0; i < n; ++i) out[i] = r.next_field();
   return WHIR_OK;
 }
 
