@@ -187,6 +187,12 @@ int whir_combine_eqs(whir_ctx* ctx, uint32_t* d_w, uint32_t k, const uint32_t* d
 int whir_combine_pows(whir_ctx* ctx, uint32_t* d_w, uint32_t k, const uint32_t* d_vars, uint32_t n,
                       const uint32_t alpha[4], uint64_t shift, void* stream);
 
+/* Replaces combine_pows_scaled (crates/whir/src/sumcheck/combine.rs:440-468):
+ * d_w[i] += sum_j scale_j vars_j^i over n base-field vars with n extension
+ * scales given on the device (d_scales: n x 4 words). */
+int whir_combine_pows_scaled(whir_ctx* ctx, uint32_t* d_w, uint32_t k, const uint32_t* d_vars, uint32_t n,
+                             const uint32_t* d_scales, void* stream);
+
 /* ---- whole prover (Whir::commit + claims + Whir::open) ---- */
 
 int whir_prover_create(whir_ctx* ctx, const whir_params* params, whir_prover** out);

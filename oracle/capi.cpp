@@ -174,6 +174,18 @@ void oracle_combine_pows(uint32_t* out, int k, const uint32_t* vars, int n, cons
   combine_pows(o, v, get_e(alpha), shift);
   memcpy(out, o.data(), o.size() * 16);
 }
+void oracle_combine_pows_scaled(uint32_t* out, int k, const uint32_t* vars, int n, const uint32_t* scales) {
+  std::vector<E> o((size_t)1 << k);
+  memcpy(o.data(), out, o.size() * 16);
+  std::vector<F> v(n);
+  std::vector<E> sc(n);
+  for (int i = 0; i < n; ++i) {
+    v[i] = F{vars[i]};
+    sc[i] = get_e(scales + 4 * i);
+  }
+  combine_pows_scaled(o, v, sc);
+  memcpy(out, o.data(), o.size() * 16);
+}
 void oracle_eval_univariate(const uint32_t* poly, size_t n, const uint32_t* x, uint32_t* out) {
   std::vector<E> p(n);
   memcpy(p.data(), poly, n * 16);

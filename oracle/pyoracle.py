@@ -91,6 +91,7 @@ def lib():
         L.oracle_fix_lo_var.argtypes = [u32p, ctypes.c_int, u32p]
         L.oracle_combine_eqs.argtypes = [u32p, ctypes.c_int, u32p, ctypes.c_int, u32p]
         L.oracle_combine_pows.argtypes = [u32p, ctypes.c_int, u32p, ctypes.c_int, u32p, ctypes.c_int]
+        L.oracle_combine_pows_scaled.argtypes = [u32p, ctypes.c_int, u32p, ctypes.c_int, u32p]
         L.oracle_eval_univariate.argtypes = [u32p, ctypes.c_size_t, u32p, u32p]
         L.oracle_n_ood_queries.argtypes = [ctypes.c_int] * 4
         L.oracle_n_stir_queries.argtypes = [ctypes.c_int] * 3
@@ -338,6 +339,16 @@ def combine_pows(out, vars_, alpha, shift):
     v = _u32(vars_)
     a = _u32(alpha)
     lib().oracle_combine_pows(_p(o), k, _p(v), v.size, _p(a), shift)
+    return o
+
+
+def combine_pows_scaled(out, vars_, scales):
+    o = _u32(out).reshape(-1, 4).copy()
+    k = int(np.log2(o.shape[0]))
+    v = _u32(vars_)
+    sc = _u32(scales).reshape(-1, 4)
+    assert sc.shape[0] == v.size
+    lib().oracle_combine_pows_scaled(_p(o), k, _p(v), v.size, _p(sc))
     return o
 
 

@@ -92,6 +92,17 @@ def test_combine_pows(gpu_ctx, oracle, k, n, shift):
     assert np.array_equal(got, oracle.combine_pows(w, vs, alpha, shift))
 
 
+@pytest.mark.parametrize("k,n", [(1, 1), (6, 5), (13, 40), (16, 133)])
+def test_combine_pows_scaled(gpu_ctx, oracle, k, n):
+    import whir_amd
+
+    rng = np.random.default_rng(650 + k * 13 + n)
+    w, vs, scales = _ext(rng, 1 << k), _base(rng, n), _ext(rng, n)
+    print(f"combine_pows_scaled k={k} n={n}", flush=True)
+    got = whir_amd.api.combine_pows_scaled(w, vs, scales, ctx=gpu_ctx)
+    assert np.array_equal(got, oracle.combine_pows_scaled(w, vs, scales))
+
+
 @pytest.mark.parametrize("log_h,row_words", [(1, 16), (5, 64), (10, 16), (13, 8)])
 def test_merkle_open_paths_verify(gpu_ctx, oracle, log_h, row_words):
     import whir_amd

@@ -165,3 +165,14 @@ def test_combine_eqs_and_pows(oracle):
                     acc = eadd(acc, escale(a, pow(v, x, P)))
                 want.append(acc)
             assert got == want
+            # combine_pows_scaled (combine.rs:440-468) with arbitrary scales
+            scales = rand_ext_list(rnd, n)
+            got = arr_ext(oracle.combine_pows_scaled(ext_arr(base), np.array([to_m(v) for v in vars_], np.uint32),
+                                                     ext_arr(scales)))
+            want = []
+            for x in range(1 << k):
+                acc = base[x]
+                for v, sc in zip(vars_, scales):
+                    acc = eadd(acc, escale(sc, pow(v, x, P)))
+                want.append(acc)
+            assert got == want

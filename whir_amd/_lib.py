@@ -86,6 +86,7 @@ SIGNATURES = [
     ("whir_combine_eqs", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _u32p, _vp]),
     ("whir_combine_pows", ctypes.c_int,
      [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _u32p, ctypes.c_uint64, _vp]),
+    ("whir_combine_pows_scaled", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp]),
     ("whir_prover_create", ctypes.c_int, [_vp, ctypes.POINTER(whir_params), ctypes.POINTER(_vp)]),
     ("whir_prover_destroy", ctypes.c_int, [_vp]),
     ("whir_prover_proof_capacity", ctypes.c_size_t, [_vp]),

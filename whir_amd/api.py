@@ -254,6 +254,17 @@ def combine_pows(w, variables, alpha, shift, ctx=None):
                 dw, ww.size).reshape(-1, 4)
 
 
+def combine_pows_scaled(w, variables, scales, ctx=None):
+    """w[i] += sum_j scales_j vars_j^i; w is 2^k x 4 words, scales n x 4."""
+    ctx = ctx or default_context()
+    dw, ww = _dev(np.asarray(w).reshape(-1, 4))
+    k = int(ww.shape[0]).bit_length() - 1
+    dv, vv = _dev(np.asarray(variables, np.uint32))
+    ds, _ = _dev(np.asarray(scales).reshape(-1, 4))
+    return _run(ctx.L.whir_combine_pows_scaled(ctx.ptr, dw.ptr, k, dv.ptr, vv.size, ds.ptr, None),
+                "whir_combine_pows_scaled", dw, ww.size).reshape(-1, 4)
+
+
 class Prover:
     """Device workspace for one proof shape (whir_prover)."""
 

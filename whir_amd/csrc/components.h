@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(SC_THREADS) fix_lo_var_kernel(const Ext* __res
 __global__ void pow_scales_kernel(ExtArg alpha, uint64_t shift, uint32_t n, Ext* scales, uint32_t* count) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j == 0) *count = n;
-  if (j < n) st_ext(scales + j, epow(ext_of(alpha), shift + j));
+  if (j < n && scales) st_ext(scales + j, epow(ext_of(alpha), shift + j));  // no scales: count only
 }
 
 // MerkleTree::query (poseidon_packed.rs:145-233): per index, the row
@@ -311,32 +311,51 @@ extern "C" int whir_combine_eqs(whir_ctx* c, uint32_t* d_w, uint32_t k, const ui
   return WHIR_OK;
 }
 
-extern "C" int whir_combine_pows(whir_ctx* c, uint32_t* d_w, uint32_t k, const uint32_t* d_vars, uint32_t n,
-                                 const uint32_t* alpha, uint64_t shift, void* stream) {
-  if (!c || !d_w || (!d_vars && n) || !alpha || k < 1 || k > 30 || n > 4096) return WHIR_EINVAL;
-  if (n == 0) return WHIR_OK;
-  hipStream_t s = pick_stream(c, stream);
+// d_w[i] += sum_j scale_j vars_j^i: split pow tables (powL = vars^e for the low
+// m0 bits of i, powR = scale vars^(e 2^m0) for the high m1 bits) and the
+// combine kernel. With `alpha` set the scales are alpha^(shift + j)
+// (combine_pows), else the caller's n device scales (combine_pows_scaled).
+static int combine_pows_enqueue(hipStream_t s, uint32_t* d_w, uint32_t k, const uint32_t* d_vars, uint32_t n,
+                                const uint32_t* alpha, uint64_t shift, const Ext* d_scales) {
   const uint32_t m0 = k / 2, m1 = k - m0;
-  Ext* scales;
+  Ext* scales = nullptr;
   uint32_t* count;
   uint32_t* powL;
   Ext* powR;
-  HIP_TRY(hipMallocAsync((void**)&scales, (uint64_t)n * 16, s));
   HIP_TRY(hipMallocAsync((void**)&count, 4, s));
+  if (alpha) HIP_TRY(hipMallocAsync((void**)&scales, (uint64_t)n * 16, s));
   HIP_TRY(hipMallocAsync((void**)&powL, ((uint64_t)n << m0) * 4, s));
   HIP_TRY(hipMallocAsync((void**)&powR, ((uint64_t)n << m1) * 16, s));
-  hipLaunchKernelGGL(pow_scales_kernel, dim3((n + 255) / 256), dim3(256), 0, s, ext_arg(alpha), shift, n, scales,
-                     count);
+  if (alpha)
+    hipLaunchKernelGGL(pow_scales_kernel, dim3((n + 255) / 256), dim3(256), 0, s, ext_arg(alpha), shift, n, scales,
+                       count);
+  else
+    hipLaunchKernelGGL(pow_scales_kernel, dim3(1), dim3(1), 0, s, ext_arg(nullptr), 0ull, n, (Ext*)nullptr, count);
   hipLaunchKernelGGL(pow_tables_kernel, dim3(blocks_for((uint64_t)n * ((1ull << m0) + (1ull << m1)))),
-                     dim3(SC_THREADS), 0, s, d_vars, (const Ext*)scales, (const uint32_t*)count, m0, m1, powL, powR);
+                     dim3(SC_THREADS), 0, s, d_vars, alpha ? (const Ext*)scales : d_scales, (const uint32_t*)count,
+                     m0, m1, powL, powR);
   const uint64_t N = 1ull << k;
   launch_combine(s, 1u, (Ext*)d_w, N, m0, nullptr, 0u, 0, powL, powR, count, m1);
   HIP_TRY(hipFreeAsync(powR, s));
   HIP_TRY(hipFreeAsync(powL, s));
+  if (scales) HIP_TRY(hipFreeAsync(scales, s));
   HIP_TRY(hipFreeAsync(count, s));
-  HIP_TRY(hipFreeAsync(scales, s));
   HIP_TRY(hipGetLastError());
   return WHIR_OK;
+}
+
+extern "C" int whir_combine_pows(whir_ctx* c, uint32_t* d_w, uint32_t k, const uint32_t* d_vars, uint32_t n,
+                                 const uint32_t* alpha, uint64_t shift, void* stream) {
+  if (!c || !d_w || (!d_vars && n) || !alpha || k < 1 || k > 30 || n > 4096) return WHIR_EINVAL;
+  if (n == 0) return WHIR_OK;
+  return combine_pows_enqueue(pick_stream(c, stream), d_w, k, d_vars, n, alpha, shift, nullptr);
+}
+
+extern "C" int whir_combine_pows_scaled(whir_ctx* c, uint32_t* d_w, uint32_t k, const uint32_t* d_vars, uint32_t n,
+                                        const uint32_t* d_scales, void* stream) {
+  if (!c || !d_w || ((!d_vars || !d_scales) && n) || k < 1 || k > 30 || n > 4096) return WHIR_EINVAL;
+  if (n == 0) return WHIR_OK;
+  return combine_pows_enqueue(pick_stream(c, stream), d_w, k, d_vars, n, nullptr, 0, (const Ext*)d_scales);
 }
 
 extern "C" int whir_merkle_open(whir_ctx* c, const uint32_t* d_layers, uint32_t log_h, const uint32_t* d_rows,
