@@ -32,7 +32,8 @@ typedef enum {
   WHIR_EINVAL = 4,      /* shape or parameter outside what the reference accepts */
   WHIR_EOOM = 5,        /* device allocation failed */
   WHIR_EDEVICE = 6,     /* HIP runtime error */
-  WHIR_EBUSY = 7        /* the queried proof is still running */
+  WHIR_EBUSY = 7,       /* the queried proof is still running */
+  WHIR_ECHECK = 8       /* a device self-check failed (whir_prover_set_checks) */
 } whir_status;
 
 /* SecurityAssumption (crates/whir/src/params.rs:6-21). */
@@ -63,8 +64,9 @@ int whir_ctx_destroy(whir_ctx* ctx);
 
 /* NTT tile shape of this context (no counterpart in the reference, whose DFT
  * has no device tiling): passes of at most 2^max_log_pass rows (1..8; raised
- * to what three passes need), workgroups of up to 2^log_threads threads (6..8).
- * Results do not depend on it. Defaults 8, 8. */
+ * to what three passes need), workgroups of up to 2^log_threads threads (6..9;
+ * 512 threads only for the three-stage passes of a two-pass transform, 256 for
+ * the others). Results do not depend on it. Defaults 8, 8. */
 int whir_ctx_set_ntt_tuning(whir_ctx* ctx, int max_log_pass, int log_threads);
 /* NTT pass layout of this context (no counterpart in the reference): for a
  * transform that needs three passes, first_pass_log in 2..4 makes the first
@@ -76,7 +78,7 @@ int whir_ctx_set_ntt_shape(whir_ctx* ctx, int first_pass_log, int log_vec);
 /* NTT pass count of this context (no counterpart in the reference): with
  * two_pass = 1 a transform of 2^17..2^20 rows runs in two passes of up to 1024
  * rows (three register stages each), with 0 in three passes of <= 256 rows.
- * Results do not depend on it. Default 1. */
+ * Results do not depend on it. Default 0. */
 int whir_ctx_set_ntt_passes(whir_ctx* ctx, int two_pass);
 
 /* Merkle tree shape of this context (no counterpart in the reference, whose
@@ -285,6 +287,14 @@ int whir_prover_enqueue_stacked(whir_prover* prover, const uint32_t* const* d_po
  * enqueue and replay it afterwards; the graph proves the prover's own input
  * buffer (other d_poly are first copied into it on the stream). */
 int whir_prover_set_graph(whir_prover* prover, int enable);
+/* Device self-checks of each proof, the reference's debug assertions on the
+ * prover path (MerkleTree::query's opening re-verification, crates/merkle/src/
+ * poseidon_packed.rs:158-230; the sumcheck invariant sum == prod(), crates/whir/
+ * src/sumcheck/expr.rs:68, for the rounds on the grid after round 0): extra
+ * kernels beside the proof's own, the proof bytes unchanged. With checks on, a
+ * result call reports WHIR_ECHECK (the proof still copied out) when one failed.
+ * Off by default. */
+int whir_prover_set_checks(whir_prover* prover, int enable);
 /* WHIR_OK once the last enqueued proof has finished, WHIR_EBUSY while running. */
 int whir_prover_query(whir_prover* prover);
 /* Device bytes held by this prover / needed by a prover of this shape. */
@@ -348,6 +358,8 @@ int whir_batch_destroy(whir_batch* batch);
 uint32_t* whir_batch_input(whir_batch* batch, int job);
 /* Capture the batch's proof as a hipGraph on the first enqueue, replay it after. */
 int whir_batch_set_graph(whir_batch* batch, int enable);
+/* whir_prover_set_checks for every job of the batch. */
+int whir_batch_set_checks(whir_batch* batch, int enable);
 /* Enqueues every job's proof on `stream` (d_polys: NULL, or per job a device
  * polynomial copied into the job's input first, NULL entries keep the input). */
 int whir_batch_enqueue(whir_batch* batch, const uint32_t* const* d_polys, void* stream);

@@ -17,6 +17,7 @@ STATUS = {
     5: "WHIR_EOOM",
     6: "WHIR_EDEVICE",
     7: "WHIR_EBUSY",
+    8: "WHIR_ECHECK",
 }
 
 
@@ -103,6 +104,7 @@ SIGNATURES = [
     ("whir_prover_stack_offset", ctypes.c_int64, [_vp, ctypes.c_int32]),
     ("whir_prover_enqueue_stacked", ctypes.c_int, [_vp, ctypes.POINTER(_vp), _vp]),
     ("whir_prover_set_graph", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("whir_prover_set_checks", ctypes.c_int, [_vp, ctypes.c_int]),
     ("whir_prover_query", ctypes.c_int, [_vp]),
     ("whir_prover_device_bytes", ctypes.c_size_t, [_vp]),
     ("whir_prover_bytes_for", ctypes.c_size_t, [ctypes.POINTER(whir_params)]),
@@ -124,6 +126,7 @@ SIGNATURES = [
     ("whir_batch_destroy", ctypes.c_int, [_vp]),
     ("whir_batch_input", _vp, [_vp, ctypes.c_int]),
     ("whir_batch_set_graph", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("whir_batch_set_checks", ctypes.c_int, [_vp, ctypes.c_int]),
     ("whir_batch_enqueue", ctypes.c_int, [_vp, ctypes.POINTER(_vp), _vp]),
     ("whir_batch_query", ctypes.c_int, [_vp]),
     ("whir_batch_result", ctypes.c_int,

@@ -339,6 +339,11 @@ class Prover:
     def set_graph(self, enable=True):
         check(self.ctx.L.whir_prover_set_graph(self.ptr, int(enable)), "whir_prover_set_graph")
 
+    def set_checks(self, enable=True):
+        """Device self-checks of each proof; result() then raises WhirError
+        (WHIR_ECHECK) when one failed."""
+        check(self.ctx.L.whir_prover_set_checks(self.ptr, int(enable)), "whir_prover_set_checks")
+
     def done(self):
         rc = self.ctx.L.whir_prover_query(self.ptr)
         if rc not in (0, 7):
@@ -429,6 +434,9 @@ class Batch:
         self.capacity = self.ctx.L.whir_batch_proof_capacity(self.ptr)
         if graph:
             check(self.ctx.L.whir_batch_set_graph(self.ptr, 1), "whir_batch_set_graph")
+
+    def set_checks(self, enable=True):
+        check(self.ctx.L.whir_batch_set_checks(self.ptr, int(enable)), "whir_batch_set_checks")
 
     def input_ptr(self, job):
         return self.ctx.L.whir_batch_input(self.ptr, job)

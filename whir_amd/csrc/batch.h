@@ -103,6 +103,19 @@ extern "C" int whir_batch_set_graph(whir_batch* B, int enable) {
   return WHIR_OK;
 }
 
+extern "C" int whir_batch_set_checks(whir_batch* B, int enable) {
+  if (!B) return WHIR_EINVAL;
+  if (B->P->checks != (enable != 0)) {  // a captured graph has the old launch sequence
+    if (B->done) HIP_TRY(hipEventSynchronize(B->done));
+    if (B->gexec) HIP_TRY(hipGraphExecDestroy(B->gexec));
+    if (B->graph) HIP_TRY(hipGraphDestroy(B->graph));
+    B->gexec = nullptr;
+    B->graph = nullptr;
+  }
+  B->P->checks = enable != 0;
+  return WHIR_OK;
+}
+
 static int batch_enqueue_proofs(whir_batch* B, hipStream_t s) {
   whir_prover* P = B->P;
   int rc = prover_enqueue(P, P->input, s);
@@ -178,7 +191,7 @@ extern "C" int whir_batch_result(whir_batch* B, int job, uint8_t* proof, size_t 
     if (bytes > cap) return WHIR_ETRANSCRIPT;
     memcpy(proof, log, bytes);
   }
-  return WHIR_OK;
+  return P->checks && tail[3] ? WHIR_ECHECK : WHIR_OK;
 }
 
 extern "C" size_t whir_batch_proof_capacity(const whir_batch* B) { return B ? B->P->plan.log_words * 4 : 0; }

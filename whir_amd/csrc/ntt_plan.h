@@ -93,7 +93,8 @@ inline void ntt_pass_setup(const NttPlan& pl, int p, uint64_t n_in, int log_thre
   a.log_n = (uint32_t)pl.log_n;
   a.log_m = (uint32_t)(pl.log_n - used);
   const int log_lines = pl.log_n + pl.log_b - pl.lv - l;
-  a.log_qv = (uint32_t)std::max(0, std::min(log_threads - lr2, log_lines));
+  const int lt = ps.kind == NTT_PASS_TRI ? log_threads : std::min(log_threads, 8);  // ntt_kernel: <= 256 threads
+  a.log_qv = (uint32_t)std::max(0, std::min(lt - lr2, log_lines));
   a.first = p == 0;
   a.twiddle_out = p + 1 < pl.P;
   a.natural = pl.P > 1 && p + 1 == pl.P;

@@ -35,6 +35,7 @@ struct JobScalars {
   uint32_t stir_count;
   uint32_t checkpoint;
   uint32_t pad;
+  Ext chk_prod, chk_c0, chk_c2;  // self-checks: <f, w>, c0, c2 of the last checked arrays
 };
 
 // Per-query state (structure of arrays, MAX_STIR entries each).

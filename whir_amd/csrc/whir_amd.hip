@@ -21,6 +21,7 @@
 #include "sumcheck.h"
 #include "protocol.h"
 #include "round_small.h"
+#include "checks.h"
 #include "host_params.h"
 #include "host_rng.h"
 #include "stack.h"
@@ -210,7 +211,7 @@ extern "C" int whir_ctx_set_ntt_shape(whir_ctx* c, int first_pass_log, int log_v
 }
 
 extern "C" int whir_ctx_set_ntt_tuning(whir_ctx* c, int max_log_pass, int log_threads) {
-  if (!c || max_log_pass < 1 || max_log_pass > 8 || log_threads < 6 || log_threads > 8) return WHIR_EINVAL;
+  if (!c || max_log_pass < 1 || max_log_pass > 8 || log_threads < 6 || log_threads > 9) return WHIR_EINVAL;
   c->ntt.max_logl = max_log_pass;
   c->ntt.log_threads = log_threads;
   return WHIR_OK;
@@ -450,8 +451,9 @@ extern "C" int whir_poseidon2_permute(whir_ctx* c, int width, uint32_t* d_states
 
 // ============================ prover ============================
 
-// Status words after a proof log's capacity (written by final_phase_kernel):
-// words used, overflow flag, checkpoint draw, spare.
+// Status words after a proof log's capacity: words used, overflow flag,
+// checkpoint draw (written by final_phase_kernel), failed self-checks
+// (whir_prover_set_checks; zeroed when the checks are on).
 constexpr uint32_t LOG_TAIL = 4;
 
 struct whir_prover {
@@ -520,6 +522,7 @@ struct whir_prover {
   // stream of the last enqueue; optional hipGraph of the whole proof
   hipStream_t last_stream = nullptr;
   bool graph_mode = false;
+  bool checks = false;  // device self-checks (checks.h)
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -865,7 +868,7 @@ static std::vector<NttConfig> ntt_candidates(whir_ctx* c, uint64_t n_in, int log
   std::vector<std::string> seen;
   for (int two : {0, 1})
     for (int fl : {4, 3, 0})
-      for (int lt : {8, 7})
+      for (int lt : {8, 7, 9})  // 9: 512-thread three-stage passes (others are capped at 256 threads)
         for (int lv : {2, 1}) {
           NttConfig cfg;
           cfg.two_pass = two;
@@ -1209,6 +1212,14 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
   int rc;
   const int n_groups = (int)P->groups.size();
   const uint32_t m0 = P->m0_svo;
+  uint32_t* fail = P->log + P->plan.log_words + 3;  // failed self-checks (tail word 3)
+  if (P->checks) hipLaunchKernelGGL(check_reset_kernel, zg(dim3(1), P->jobs), dim3(64), 0, s, fail);
+  // the sumcheck self-check of the rounds on the grid after round 0 (round 0's
+  // weights have query terms that live in the per-query tables, not in ew)
+  auto sumcheck_check = [&](int cur_, uint64_t n_, const Ext* r_) {
+    hipLaunchKernelGGL(sumcheck_check_kernel, zg(dim3(1), P->jobs), dim3(CHECK_THREADS), 0, s, P->ep[cur_],
+                       P->ew[cur_], n_, r_, P->js, fail);
+  };
   // ---- SVO rounds + expression (prover.rs:279-353, 184-229) ----
   hipLaunchKernelGGL(svo_kernel, zg(dim3(1), P->jobs), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js, P->slots, P->desc,
                      (uint32_t)P->n_svo_claims, P->group_start, (uint32_t)n_groups, (uint32_t)f, P->pe_buf,
@@ -1306,6 +1317,12 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
                          P->stir_count, P->cw[(r - 1) & 1], 4 * W, P->tree[(r - 1) & 1], (uint32_t)rp.kfd, 2,
                          (uint32_t)f, P->qa, om0, om1, P->powL, P->powR, ss);
     }
+    if (P->checks)  // the openings just written, before the next transcript kernel advances the log
+      hipLaunchKernelGGL(opening_check_kernel,
+                         zg(dim3((unsigned)((rp.n_stir + OPEN_CHECK_THREADS - 1) / OPEN_CHECK_THREADS)), P->jobs),
+                         dim3(OPEN_CHECK_THREADS), 0, s, P->tx, P->log, P->stir_idx, P->stir_count,
+                         r == 0 ? P->cw0 : P->cw[(r - 1) & 1], r == 0 ? W : 4 * W,
+                         r == 0 ? P->tree0 : P->tree[(r - 1) & 1], (uint32_t)rp.kfd, fail);
     launch_combine(s, P->jobs, P->ew[cur], n, om0, P->ood_pool, (uint32_t)rp.n_ood, P->ood_stride, P->powL, P->powR,
                    r > 0 ? P->stir_count : nullptr, om1);
     if (r > 0 && n <= (1ull << SC_TAIL_MAX_LOG_N)) {
@@ -1324,6 +1341,8 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
     };
     // (and its query terms of the next step as QUERY_BLOCKS more partials)
     const unsigned qx = r == 0 ? QUERY_BLOCKS : 0u;
+    const bool check_sc = P->checks && r > 0;
+    if (check_sc) sumcheck_check(cur, n, nullptr);
     for (int t = 0; t < f; ++t) {
       unsigned g;
       if (t == 0) {
@@ -1344,6 +1363,7 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
                              P->js->rs + (rs_idx - 1), P->ep[cur ^ 1], P->ew[cur ^ 1], P->partials);
         n /= 2;
         cur ^= 1;
+        if (check_sc) sumcheck_check(cur, n, P->js->rs + (rs_idx - 1));
       }
       hipLaunchKernelGGL(round_finalize_kernel, zg(dim3(1), P->jobs), dim3(SERIAL_THREADS), 0, s, P->tx, P->log, P->js,
                          P->partials, g + qx, rs_idx);
@@ -1357,6 +1377,7 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
                          P->js->rs + (rs_idx - 1), P->ep[cur ^ 1], P->ew[cur ^ 1]);
     n /= 2;
     cur ^= 1;
+    if (check_sc) sumcheck_check(cur, n, P->js->rs + (rs_idx - 1));
     if (r == 0) {
       // combine_pows_scaled(weights, ws, scales) at k - 2f variables (prover.rs:556)
       const int mm = rp.kc - f;
@@ -1499,6 +1520,19 @@ extern "C" int whir_prover_result(whir_prover* P, uint8_t* proof, size_t cap, si
     if (bytes > cap) return WHIR_ETRANSCRIPT;
     memcpy(proof, P->host_log, bytes);
   }
+  return P->checks && tail[3] ? WHIR_ECHECK : WHIR_OK;
+}
+
+extern "C" int whir_prover_set_checks(whir_prover* P, int enable) {
+  if (!P) return WHIR_EINVAL;
+  if (P->checks != (enable != 0)) {  // a captured proof graph has the old launch sequence
+    if (P->done) HIP_TRY(hipEventSynchronize(P->done));
+    if (P->gexec) HIP_TRY(hipGraphExecDestroy(P->gexec));
+    if (P->graph) HIP_TRY(hipGraphDestroy(P->graph));
+    P->gexec = nullptr;
+    P->graph = nullptr;
+  }
+  P->checks = enable != 0;
   return WHIR_OK;
 }
 
