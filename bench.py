@@ -25,9 +25,9 @@ The JSON line also carries
     whir_amd/kernel_stats.json) per second against the same peak. `traffic`
     comes from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary.
   * roofline_dft: the base RS encode (three NTT passes), timed the same way.
-    It issues 4.3 G VALU lane instructions (rocprofv3 SQ_INSTS_VALU, read from
+    It issues 4.9 G VALU lane instructions (rocprofv3 SQ_INSTS_VALU, read from
     the same summary) against 0.74 GB of HBM traffic, so its bound is the VALU
-    issue rate (110 us at peak) rather than HBM (92 us at 8 TB/s); both
+    issue rate (125 us at peak) rather than HBM (92 us at 8 TB/s); both
     views are in the object.
   * cpu_baseline: the CPU oracle (a multithreaded C++ restatement of the
     reference prover, "port") proving the same polynomial on the host cores
@@ -66,7 +66,7 @@ FOLDING, RATE, IR, SOUNDNESS, SECURITY, N_POINTS = 4, 1, 3, 1, 123, 1
 METRIC = "commit+prove ms @2^24 vars; batched proofs/s & p99 vs plain GPU, bit-exact"
 VALU_PEAK_T = 256 * 4 * 16 * 2.4e9 / 1e12  # T lane-instructions / s
 HBM_PEAK_GBS = 8000.0
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "merkle_leaves16_k24_pmc.txt")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "whir_prove_k24_pmc.txt")
 
 
 MODMULS_PER_P24 = 8 * 24 * 2 + 23 * 2 + 23 * 24  # 982: cubes of full and partial rounds, internal diagonal
@@ -462,11 +462,20 @@ def main():
         "issued_valu_frac": round(issued / VALU_PEAK_T, 3) if issued else None,
     }
     dft_s = dft_ms / dft_n / 1e3
-    dft_traffic = read_traffic("ntt_base_traffic_bytes")
-    dft_valu = read_traffic("ntt_base_valu_lane_instr")
+    # the PMC summary's counts are for the default plan of the base transform
+    # (three passes, a radix-16 register pass first, two words per line);
+    # with another tuned plan only the timing-based fields apply
+    base_plan = next((t for t in (ctx.tuning() if not args.no_autotune else [])
+                      if t["kind"] == "ntt" and t["log_rows"] == args.k - FOLDING + RATE
+                      and t["log_cols"] == FOLDING), None)
+    default_plan = base_plan is None or (base_plan["two_pass"] == 0 and base_plan["first_pass_log"] == 4
+                                         and base_plan["log_vec"] == 1)
+    dft_traffic = read_traffic("ntt_base_traffic_bytes") if default_plan else None
+    dft_valu = read_traffic("ntt_base_valu_lane_instr") if default_plan else None
     dft_valu_t = dft_valu / dft_s / 1e12 if dft_valu else None
     roofline_dft = {
-        "kernel": "base RS encode (3 ntt_kernel passes, coset-form LDE 2^20 -> 2^21 rows x 16 columns)",
+        "kernel": "base RS encode (coset-form LDE 2^20 -> 2^21 rows x 16 columns; "
+                  + ("3 NTT passes)" if default_plan else f"tuned plan {base_plan})"),
         "bound": "valu",
         "achieved": round(dft_valu_t, 2) if dft_valu_t else None,
         "peak": round(VALU_PEAK_T, 2),

@@ -72,7 +72,7 @@ int whir_ctx_set_ntt_tuning(whir_ctx* ctx, int max_log_pass, int log_threads);
  * transform that needs three passes, first_pass_log in 2..4 makes the first
  * pass an LDS-free radix-2^first_pass_log register pass (0: three even passes);
  * log_vec in 0..2 sets the words per lane and line (1, 2, 4). Results do not
- * depend on it. Defaults 4, 2. */
+ * depend on it. Defaults 4, 1. */
 int whir_ctx_set_ntt_shape(whir_ctx* ctx, int first_pass_log, int log_vec);
 
 /* NTT pass count of this context (no counterpart in the reference): with

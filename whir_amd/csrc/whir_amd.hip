@@ -54,7 +54,7 @@ static __device__ __align__(256) unsigned char scratch[WHIR_NOALLOC_SCRATCH_BYTE
 // 2^max_lv words per lane and line; transforms of 2^17..2^20 rows in two passes
 // of up to 1024 rows (ntt3_kernel) when two_pass is set.
 struct NttConfig {
-  int max_logl = 8, log_threads = 8, first_log = 4, max_lv = 2, two_pass = 0;
+  int max_logl = 8, log_threads = 8, first_log = 4, max_lv = 1, two_pass = 0;
 };
 // How one Merkle tree runs: two lanes per permutation (p2pair.h) for leaf
 // layers and compression layers of at most pair_max rows / compressions
