@@ -6,6 +6,7 @@
 #pragma once
 #include "field.h"
 #include "poseidon2.h"
+#include "tx.h"
 
 namespace kb {
 
@@ -47,13 +48,26 @@ __device__ __forceinline__ Ext ld_ext(const Ext* p) {
 __device__ __forceinline__ void st_ext(Ext* p, const Ext& e) {
   *reinterpret_cast<uint4*>(p) = make_uint4(e.c[0], e.c[1], e.c[2], e.c[3]);
 }
-// Sum over the 64 lanes of a wavefront (butterfly: every lane gets the sum).
+// Sum over the 64 lanes of a wavefront, returned to every lane, with DPP
+// moves feeding the adds (no LDS crossbar traffic): rotations inside each
+// 16-lane row give every lane its row's sum; row_bcast:15 adds row 0's sum
+// into row 1 and row 2's into row 3, row_bcast:31 adds lane 31 (rows 0 + 1)
+// into rows 2 and 3, so lane 63 holds the total, which readlane broadcasts.
+// Rows outside a move's row mask read the move's "old" operand, 0. Every
+// lane of the wavefront must be active.
+constexpr int DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+__device__ __forceinline__ uint32_t wave_sum_word(uint32_t x) {
+  x = add(x, dpp<DPP_ROR(8)>(x));
+  x = add(x, dpp<DPP_ROR(4)>(x));
+  x = add(x, dpp<DPP_ROR(2)>(x));
+  x = add(x, dpp<DPP_ROR(1)>(x));
+  x = add(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_BCAST15, 0xA, 0xF, false));
+  x = add(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, DPP_ROW_BCAST31, 0xC, 0xF, false));
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
 __device__ __forceinline__ Ext wave_sum_ext(Ext v) {
 #pragma unroll
-  for (int m = 32; m > 0; m >>= 1) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) v.c[c] = add(v.c[c], (uint32_t)__shfl_xor((int)v.c[c], m));
-  }
+  for (int c = 0; c < 4; ++c) v.c[c] = wave_sum_word(v.c[c]);
   return v;
 }
 
