@@ -107,10 +107,25 @@ constexpr int bitrev(int i, int bits) {
   return r;
 }
 
+// Register slot i after stage s of dft_regs holds a value that stage s + 1
+// only multiplies by a twiddle (the upper half of a butterfly with j != 0);
+// after the last stage, bit i of the caller's LAZY_OUT says so.
+constexpr bool dft_lazy(int LR, int s, int i, uint32_t lazy_out) {
+  if (s + 1 == LR) return ((lazy_out >> i) & 1u) != 0;
+  const int half = 1 << (s + 1);
+  return (i % (2 * half)) >= half && (i % half) != 0;
+}
+
 // x <- DFT_R(x) (natural order in and out, R = 2^LR <= 16): radix-2 DIT on the
 // bit-reversed registers; twiddles w_{2^(s+1)}^j are compile-time constants and
-// j = 0 multiplies are skipped (17 multiplies for R = 16).
-template <int LR, int V>
+// j = 0 multiplies are skipped (17 multiplies for R = 16). Inputs are in
+// [0, p). A sum or difference that is next only multiplied by a twiddle is
+// left unreduced in [0, 2p) (u + t, or u - t + p): the Montgomery product of
+// a factor < 2p and a twiddle < p is below p 2^32 and comes out in [0, p), so
+// such values skip the reduction of the add or sub (2 or 1 instructions).
+// LAZY_OUT marks the outputs the caller multiplies next; the others are in
+// [0, p).
+template <int LR, int V, uint32_t LAZY_OUT = 0>
 KB_HD void dft_regs(Vec<V> (&x)[1 << LR]) {
   constexpr int R = 1 << LR;
   Vec<V> y[R];
@@ -125,18 +140,32 @@ KB_HD void dft_regs(Vec<V> (&x)[1 << LR]) {
       for (int j = 0; j < half; ++j) {
         const int i0 = g + j, i1 = g + j + half;
         const uint32_t w = kOmega16.w[j << (3 - s)];
+        const bool lazy0 = dft_lazy(LR, s, i0, LAZY_OUT), lazy1 = dft_lazy(LR, s, i1, LAZY_OUT);
 #pragma unroll
         for (int c = 0; c < V; ++c) {
           const uint32_t u = y[i0].v[c];
           const uint32_t t = j == 0 ? y[i1].v[c] : mul(y[i1].v[c], w);
-          y[i0].v[c] = add(u, t);
-          y[i1].v[c] = sub(u, t);
+          y[i0].v[c] = lazy0 ? u + t : add(u, t);
+          y[i1].v[c] = lazy1 ? u - t + P : sub(u, t);
         }
       }
     }
   }
 #pragma unroll
   for (int i = 0; i < R; ++i) x[i] = y[i];
+}
+template <int V>
+KB_HD void vcanon(Vec<V>& x) {  // [0, 2p) -> [0, p)
+#pragma unroll
+  for (int c = 0; c < V; ++c) {
+    const uint32_t u = x.v[c] - P;
+    x.v[c] = x.v[c] < u ? x.v[c] : u;
+  }
+}
+// all outputs but the first (index 0 is never multiplied by a twiddle)
+template <int LR>
+constexpr uint32_t lazy_but_first() {
+  return (uint32_t)((1ull << (1 << LR)) - 2u);
 }
 
 struct NttArgs {
@@ -210,10 +239,12 @@ __global__ void __launch_bounds__(256) ntt_kernel(NttArgs a) {
       }
     }
   }
-  dft_regs<LR1, V>(x);
   if constexpr (LR2 == 0) {
     // register-only pass (an early pass of a multi-pass transform): output t
-    // of this line is x[t]; no exchange, no LDS
+    // of this line is x[t]; no exchange, no LDS. Its time is the strided
+    // loads, not arithmetic: reduced outputs throughout (the lazy variant
+    // measured 63.0 -> 68.5 us for the base LDE's first pass).
+    dft_regs<LR1, V>(x);
     if (a.twiddle_out && m) {
       const uint32_t st = tw24(a.tw_lo, a.tw_hi, m << (24 - log_mprev));  // w_{M_{p-1}}^m
       uint32_t cur = st;
@@ -227,14 +258,15 @@ __global__ void __launch_bounds__(256) ntt_kernel(NttArgs a) {
     for (int ta = 0; ta < R1; ++ta) vstore<V>(a.dst + base + (uint32_t)ta * jstride, x[ta]);
     return;
   }
-  if (R2 > 1 && jb) {  // inner twiddles w_L^(j_b t_a)
-    const uint32_t w1 = tw24(a.tw_lo, a.tw_hi, jb << (24 - LL));
-    uint32_t cur = w1;
+  // outputs 1.. come out in [0, 2p): the twiddle multiply that follows
+  // reduces them; a line that has none reduces them explicitly
+  dft_regs<LR1, V, lazy_but_first<LR1>()>(x);
+  if (R2 > 1 && jb) {  // inner twiddles w_L^(j_b t_a) = tw_hi[(j_b t_a mod L) 2^(12 - LL)] (L <= 4096)
 #pragma unroll
-    for (int ta = 1; ta < R1; ++ta) {
-      vscale<V>(x[ta], cur);
-      cur = mul(cur, w1);
-    }
+    for (int ta = 1; ta < R1; ++ta) vscale<V>(x[ta], a.tw_hi[((jb * (uint32_t)ta) & ((1u << LL) - 1)) << (12 - LL)]);
+  } else {
+#pragma unroll
+    for (int ta = 1; ta < R1; ++ta) vcanon<V>(x[ta]);
   }
   Vec<V>* l4 = reinterpret_cast<Vec<V>*>(lds);
 #pragma unroll
@@ -251,16 +283,23 @@ __global__ void __launch_bounds__(256) ntt_kernel(NttArgs a) {
     Vec<V> z[R2];
 #pragma unroll
     for (int j = 0; j < R2; ++j) z[j] = l4[(ta * R2 + j) * QV + cq];
-    dft_regs<LR2, V>(z);
-    if (a.twiddle_out && m) {
-      const uint32_t sh = 24 - log_mprev;
-      uint32_t cur = tw24(a.tw_lo, a.tw_hi, (m * ta) << sh);
-      const uint32_t st = tw24(a.tw_lo, a.tw_hi, (m << LR1) << sh);
+    if (a.twiddle_out) {  // every output is multiplied next (reduced instead when m = 0)
+      dft_regs<LR2, V, (uint32_t)((1ull << R2) - 1)>(z);
+      if (m) {
+        const uint32_t sh = 24 - log_mprev;
+        uint32_t cur = tw24(a.tw_lo, a.tw_hi, (m * ta) << sh);
+        const uint32_t st = tw24(a.tw_lo, a.tw_hi, (m << LR1) << sh);
 #pragma unroll
-      for (int tb = 0; tb < R2; ++tb) {
-        vscale<V>(z[tb], cur);
-        cur = mul(cur, st);
+        for (int tb = 0; tb < R2; ++tb) {
+          vscale<V>(z[tb], cur);
+          cur = mul(cur, st);
+        }
+      } else {
+#pragma unroll
+        for (int tb = 0; tb < R2; ++tb) vcanon<V>(z[tb]);
       }
+    } else {
+      dft_regs<LR2, V>(z);
     }
 #pragma unroll
     for (int tb = 0; tb < R2; ++tb) {
@@ -342,15 +381,13 @@ struct Ntt3 {
         }
       }
     }
-    dft_regs<LR1, V>(x);
-    if (l.jb) {  // w_L^(jb t1)
-      const uint32_t w1 = tw24(a.tw_lo, a.tw_hi, l.jb << (24 - LL));
-      uint32_t cur = w1;
+    dft_regs<LR1, V, lazy_but_first<LR1>()>(x);
+    if (l.jb) {  // w_L^(jb t1) = tw_hi[(jb t1 mod L) 2^(12 - LL)] (L <= 4096)
       #pragma unroll
-      for (int t1 = 1; t1 < R1; ++t1) {
-        vscale<V>(x[t1], cur);
-        cur = mul(cur, w1);
-      }
+      for (int t1 = 1; t1 < R1; ++t1) vscale<V>(x[t1], a.tw_hi[((l.jb * (uint32_t)t1) & ((1u << LL) - 1)) << (12 - LL)]);
+    } else {
+      #pragma unroll
+      for (int t1 = 1; t1 < R1; ++t1) vcanon<V>(x[t1]);
     }
     #pragma unroll
     for (int t1 = 0; t1 < R1; ++t1) lds[slot(a, ((uint32_t)t1 << LT) + l.jb, l.cq)] = x[t1];
@@ -364,15 +401,13 @@ struct Ntt3 {
       const uint32_t t1 = q >> LR3, j3 = q & (R3 - 1);
       #pragma unroll
       for (int j2 = 0; j2 < R2; ++j2) y[k][j2] = lds[slot(a, (t1 << LT) + ((uint32_t)j2 << LR3) + j3, l.cq)];
-      dft_regs<LR2, V>(y[k]);
-      if (j3) {  // w_T^(j3 t2)
-        const uint32_t w1 = tw24(a.tw_lo, a.tw_hi, j3 << (24 - LT));
-        uint32_t cur = w1;
+      dft_regs<LR2, V, lazy_but_first<LR2>()>(y[k]);
+      if (j3) {  // w_T^(j3 t2) = tw_hi[(j3 t2 mod T) 2^(12 - LT)]
         #pragma unroll
-        for (int t2 = 1; t2 < R2; ++t2) {
-          vscale<V>(y[k][t2], cur);
-          cur = mul(cur, w1);
-        }
+        for (int t2 = 1; t2 < R2; ++t2) vscale<V>(y[k][t2], a.tw_hi[((j3 * (uint32_t)t2) & ((1u << LT) - 1)) << (12 - LT)]);
+      } else {
+        #pragma unroll
+        for (int t2 = 1; t2 < R2; ++t2) vcanon<V>(y[k][t2]);
       }
     }
   }
@@ -399,16 +434,23 @@ struct Ntt3 {
       Vec<V> z[R3];
       #pragma unroll
       for (int j3 = 0; j3 < R3; ++j3) z[j3] = lds[slot(a, (u << LR3) + (uint32_t)j3, l.cq)];
-      dft_regs<LR3, V>(z);
-      if (a.twiddle_out && l.m) {  // w_{M_{p-1}}^(m t)
-        const uint32_t sh = 24 - log_mprev;
-        uint32_t cur = tw24(a.tw_lo, a.tw_hi, (l.m * u) << sh);
-        const uint32_t st = tw24(a.tw_lo, a.tw_hi, (l.m << (LR1 + LR2)) << sh);
-        #pragma unroll
-        for (int t3 = 0; t3 < R3; ++t3) {
-          vscale<V>(z[t3], cur);
-          cur = mul(cur, st);
+      if (a.twiddle_out) {  // w_{M_{p-1}}^(m t); every output is multiplied next (reduced when m = 0)
+        dft_regs<LR3, V, (uint32_t)((1ull << R3) - 1)>(z);
+        if (l.m) {
+          const uint32_t sh = 24 - log_mprev;
+          uint32_t cur = tw24(a.tw_lo, a.tw_hi, (l.m * u) << sh);
+          const uint32_t st = tw24(a.tw_lo, a.tw_hi, (l.m << (LR1 + LR2)) << sh);
+          #pragma unroll
+          for (int t3 = 0; t3 < R3; ++t3) {
+            vscale<V>(z[t3], cur);
+            cur = mul(cur, st);
+          }
+        } else {
+          #pragma unroll
+          for (int t3 = 0; t3 < R3; ++t3) vcanon<V>(z[t3]);
         }
+      } else {
+        dft_regs<LR3, V>(z);
       }
       #pragma unroll
       for (int t3 = 0; t3 < R3; ++t3) {
