@@ -130,3 +130,81 @@ def test_batch_matches_oracle(bb_ctx, bb):
     finally:
         b.close()
     assert got == [bb.whir_prove(*cfg, p)[0] for p in polys]
+
+
+def test_session_with_prefix_and_given_point(bb_ctx, bb):
+    """Whir::commit / ProverStack::eval / Whir::open inside a caller's
+    BabyBear transcript (crates/whir/src/whir.rs:127-180): the GPU's bytes
+    follow the caller's prefix as in the oracle's session proof, the
+    handed-back challenger is the oracle's, and its verifier accepts."""
+    import whir_amd
+    from kbpy import DuplexPy
+
+    k, f, rate, ir, s, sec = 12, 3, 1, 2, 1, 32
+    rng = np.random.default_rng(3)
+    poly = bb.rand_poly(9, k)
+    prefix = _rand(rng, 5)
+    given = [_rand(rng, (k, 4))]
+    claims = [(0, 1), (0, 2)]
+    oproof, ocp, oroot, ost = bb.whir_prove_session(k, f, rate, ir, s, sec, [poly], claims, prefix, given)
+    d = DuplexPy(bb)
+    for x in prefix:
+        d.observe(int(x))
+    print("babybear session k=12", flush=True)
+    pr = whir_amd.Prover(whir_amd.params(k, f, rate, ir, s, sec, 2), bb_ctx)
+    try:
+        pr.upload(poly)
+        pr.commit(None, d.as_state())
+        pr.eval(None)
+        pr.eval(given[0])
+        pr.open()
+        gproof, gcp, groot = pr.result()
+        gst = pr.challenger()
+    finally:
+        pr.close()
+    assert gproof == oproof[4 * len(prefix):] and gcp == ocp and np.array_equal(groot, oroot)
+    assert np.array_equal(np.asarray(gst["sponge_state"]), np.asarray(ost["sponge_state"]))
+    rc, _ = bb.whir_verify_session(k, f, rate, ir, s, sec, [k], claims, oproof[:4 * len(prefix)] + gproof, prefix,
+                                   given)
+    assert rc == 0
+
+
+def test_stacked_proof(bb_ctx, bb):
+    """Several BabyBear polys committed and opened as one (ProverStack,
+    crates/whir/src/sumcheck/prover.rs:31-124) against the oracle's stack."""
+    import whir_amd
+    from tests.stack_specs import MIXED, rand_polys, stacked_k
+
+    k_polys, claims = MIXED[1]
+    f, rate, ir, s, sec = 3, 1, 2, 1, 32
+    k = stacked_k(k_polys)
+    polys = rand_polys(bb, k_polys, 31)
+    print(f"babybear stacked k={k} polys={k_polys}", flush=True)
+    pr = whir_amd.StackedProver(whir_amd.params(k, f, rate, ir, s, sec, len(claims)), k_polys, claims, ctx=bb_ctx)
+    try:
+        pr.upload(polys)
+        pr.enqueue()
+        got = pr.result()
+    finally:
+        pr.close()
+    want = bb.whir_prove_stacked(k, f, rate, ir, s, sec, polys, claims)
+    assert got[0] == want[0] and got[1] == want[1]
+
+
+def test_checked_proof(bb_ctx, bb):
+    """The device self-checks on a BabyBear proof with a grid round after
+    round 0: no check fails and the bytes are the oracle's."""
+    import whir_amd
+
+    cfg = (20, 3, 1, 3, 1, 123, 1)
+    poly = bb.rand_poly(2, 20)
+    print("babybear checked k=20", flush=True)
+    pr = whir_amd.Prover(whir_amd.params(*cfg), bb_ctx)
+    try:
+        pr.set_checks(True)
+        pr.upload(poly)
+        pr.enqueue()
+        got = pr.result()
+    finally:
+        pr.close()
+    assert got[0] == bb.whir_prove(*cfg, poly)[0]
