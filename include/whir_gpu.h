@@ -211,6 +211,10 @@ int whir_prover_enqueue(whir_prover* prover, const uint32_t* d_poly, void* strea
  * its length, the test's checkpoint draw (test.rs:233) and the commitment root. */
 int whir_prover_result(whir_prover* prover, uint8_t* proof, size_t cap, size_t* len, uint32_t* checkpoint,
                        uint32_t* root8);
+/* The same result without a copy: *proof points at the proof in the prover's
+ * pinned host buffer, valid until the next enqueue on the prover. */
+int whir_prover_result_view(whir_prover* prover, const uint8_t** proof, size_t* len, uint32_t* checkpoint,
+                            uint32_t* root8);
 
 /* ---- Whir::commit / ProverStack::eval / Whir::open with the caller's transcript ----
  * The reference threads one transcript through three calls
@@ -368,6 +372,10 @@ int whir_batch_query(whir_batch* batch);
 /* Job `job`'s proof, its checkpoint draw and commitment root (waits). */
 int whir_batch_result(whir_batch* batch, int job, uint8_t* proof, size_t cap, size_t* len, uint32_t* checkpoint,
                       uint32_t* root8);
+/* The same result without a copy: *proof points at the job's proof in the
+ * library's pinned host buffer, valid until the next enqueue on the batch. */
+int whir_batch_result_view(whir_batch* batch, int job, const uint8_t** proof, size_t* len, uint32_t* checkpoint,
+                           uint32_t* root8);
 size_t whir_batch_proof_capacity(const whir_batch* batch);
 size_t whir_batch_device_bytes(const whir_batch* batch);
 
@@ -386,6 +394,8 @@ int whir_multi_set_graph(whir_multi* multi, int mode);
 int whir_multi_enqueue(whir_multi* multi, const uint32_t* const* d_polys, void* stream);
 int whir_multi_result(whir_multi* multi, int job, uint8_t* proof, size_t cap, size_t* len, uint32_t* checkpoint,
                       uint32_t* root8);
+int whir_multi_result_view(whir_multi* multi, int job, const uint8_t** proof, size_t* len, uint32_t* checkpoint,
+                           uint32_t* root8);
 size_t whir_multi_proof_capacity(const whir_multi* multi, int job);
 
 /* ---- device memory helpers (plain HIP allocations) ---- */

@@ -114,3 +114,61 @@ def test_multi_shape_set_matches_oracle(gpu_ctx, oracle, graph):
                 assert got[i] == want[i], (rep, i, cfgs[i])
     finally:
         m.close()
+
+
+def test_copying_results_equal_the_views(gpu_ctx, oracle):
+    """whir_*_result (copy into the caller's buffer) and whir_*_result_view
+    (a pointer into the library's pinned copy) give the same proof, checkpoint
+    and root, for a prover, a batch and a mixed-shape set; a buffer one byte
+    short is refused."""
+    import ctypes
+
+    import whir_amd
+    from whir_amd._lib import check
+
+    L = gpu_ctx.L
+    cfg = (12, 4, 1, 3, 1, 123, 1)
+    polys = [oracle.rand_poly(60 + j, 12) for j in range(2)]
+
+    def copied(fn, *args):
+        ln = ctypes.c_size_t()
+        cp = ctypes.c_uint32()
+        root = np.zeros(8, np.uint32)
+        check(fn(*args, None, 0, ctypes.byref(ln), ctypes.byref(cp), root.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+              "len")
+        buf = (ctypes.c_uint8 * ln.value)()
+        assert fn(*args, buf, ln.value - 1, None, None, None) == 1  # WHIR_ETRANSCRIPT: too small
+        check(fn(*args, buf, ln.value, ctypes.byref(ln), None, None), "copy")
+        return bytes(buf), int(cp.value), root
+
+    pr = whir_amd.Prover(whir_amd.params(*cfg), gpu_ctx)
+    b = whir_amd.Batch(whir_amd.params(*cfg), 2, ctx=gpu_ctx)
+    m = whir_amd.MultiBatch([whir_amd.params(*cfg), whir_amd.params(13, 4, 1, 3, 1, 123, 1)], streams=2, ctx=gpu_ctx)
+    bufs = []
+    try:
+        pr.upload(polys[0])
+        pr.enqueue()
+        v = pr.result()
+        c = copied(L.whir_prover_result, pr.ptr)
+        assert c[0] == v[0] and c[1] == v[1] and np.array_equal(c[2], v[2])
+        for j, p in enumerate(polys):
+            b.upload(j, p)
+        b.enqueue()
+        for j in range(2):
+            v = b.result(j)
+            c = copied(L.whir_batch_result, b.ptr, j)
+            assert c[0] == v[0] and c[1] == v[1] and np.array_equal(c[2], v[2])
+        for p in (polys[0], oracle.rand_poly(70, 13)):
+            d = whir_amd.api.DeviceBuffer(p.nbytes)
+            d.upload(p)
+            bufs.append(d)
+        m.enqueue([d.ptr for d in bufs])
+        for j in range(2):
+            v = m.result(j)
+            c = copied(L.whir_multi_result, m.ptr, j)
+            assert c[0] == v[0] and c[1] == v[1] and np.array_equal(c[2], v[2])
+        assert m.result(0)[0] == oracle.whir_prove(*cfg, polys[0])[0]
+    finally:
+        pr.close()
+        b.close()
+        m.close()

@@ -63,6 +63,7 @@ class whir_challenger(ctypes.Structure):
 _vp = ctypes.c_void_p
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
+_u8pp = ctypes.POINTER(_u8p)
 SIGNATURES = [
     ("whir_ctx_create", ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     ("whir_ctx_destroy", ctypes.c_int, [_vp]),
@@ -95,6 +96,7 @@ SIGNATURES = [
     ("whir_prover_enqueue", ctypes.c_int, [_vp, _vp, _vp]),
     ("whir_prover_result", ctypes.c_int,
      [_vp, _u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
+    ("whir_prover_result_view", ctypes.c_int, [_vp, _u8pp, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
     ("whir_prover_commit", ctypes.c_int, [_vp, _vp, ctypes.POINTER(whir_challenger), _vp]),
     ("whir_prover_eval", ctypes.c_int, [_vp, _u32p, _vp]),
     ("whir_prover_open", ctypes.c_int, [_vp, _vp]),
@@ -131,6 +133,7 @@ SIGNATURES = [
     ("whir_batch_query", ctypes.c_int, [_vp]),
     ("whir_batch_result", ctypes.c_int,
      [_vp, ctypes.c_int, _u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
+    ("whir_batch_result_view", ctypes.c_int, [_vp, ctypes.c_int, _u8pp, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
     ("whir_batch_proof_capacity", ctypes.c_size_t, [_vp]),
     ("whir_batch_device_bytes", ctypes.c_size_t, [_vp]),
     ("whir_multi_create", ctypes.c_int,
@@ -141,6 +144,7 @@ SIGNATURES = [
     ("whir_multi_enqueue", ctypes.c_int, [_vp, ctypes.POINTER(_vp), _vp]),
     ("whir_multi_result", ctypes.c_int,
      [_vp, ctypes.c_int, _u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
+    ("whir_multi_result_view", ctypes.c_int, [_vp, ctypes.c_int, _u8pp, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),
     ("whir_multi_proof_capacity", ctypes.c_size_t, [_vp, ctypes.c_int]),
     ("whir_poly_rand", ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, _u32p]),
     ("whir_device_alloc", ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),

@@ -14,6 +14,17 @@ def _u32(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
 
 
+def _view_result(fn, name, *args):
+    """(proof bytes, checkpoint, root) through a *_result_view entry point:
+    one copy from the library's pinned host buffer into the bytes object."""
+    ptr = _u8p()
+    ln = ctypes.c_size_t()
+    cp = ctypes.c_uint32()
+    root = np.zeros(8, np.uint32)
+    check(fn(*args, ctypes.byref(ptr), ctypes.byref(ln), ctypes.byref(cp), root.ctypes.data_as(_u32p)), name)
+    return ctypes.string_at(ptr, ln.value), int(cp.value), root
+
+
 class DeviceBuffer:
     """A plain hipMalloc allocation owned by Python."""
 
@@ -286,13 +297,7 @@ class Prover:
         check(self.ctx.L.whir_prover_enqueue(self.ptr, d_poly or self.input_ptr, stream), "whir_prover_enqueue")
 
     def result(self):
-        buf = np.zeros(self.capacity, np.uint8)
-        ln = ctypes.c_size_t()
-        cp = np.zeros(1, np.uint32)
-        root = np.zeros(8, np.uint32)
-        check(self.ctx.L.whir_prover_result(self.ptr, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
-                                       cp.ctypes.data_as(_u32p), root.ctypes.data_as(_u32p)), "whir_prover_result")
-        return bytes(buf[: ln.value]), int(cp[0]), root
+        return _view_result(self.ctx.L.whir_prover_result_view, "whir_prover_result_view", self.ptr)
 
     # Whir::commit / ProverStack::eval / Whir::open with the caller's transcript
     # (whir.rs:127-180, prover.rs:64-90)
@@ -457,13 +462,7 @@ class Batch:
         return rc == 0
 
     def result(self, job):
-        buf = np.zeros(self.capacity, np.uint8)
-        ln = ctypes.c_size_t()
-        cp = np.zeros(1, np.uint32)
-        root = np.zeros(8, np.uint32)
-        check(self.ctx.L.whir_batch_result(self.ptr, job, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
-                                      cp.ctypes.data_as(_u32p), root.ctypes.data_as(_u32p)), "whir_batch_result")
-        return bytes(buf[: ln.value]), int(cp[0]), root
+        return _view_result(self.ctx.L.whir_batch_result_view, "whir_batch_result_view", self.ptr, job)
 
     def close(self):
         if self.ptr:
@@ -500,14 +499,7 @@ class MultiBatch:
         check(self.ctx.L.whir_multi_enqueue(self.ptr, ptrs, stream), "whir_multi_enqueue")
 
     def result(self, job):
-        cap = self.ctx.L.whir_multi_proof_capacity(self.ptr, job)
-        buf = np.zeros(cap, np.uint8)
-        ln = ctypes.c_size_t()
-        cp = np.zeros(1, np.uint32)
-        root = np.zeros(8, np.uint32)
-        check(self.ctx.L.whir_multi_result(self.ptr, job, buf.ctypes.data_as(_u8p), buf.size, ctypes.byref(ln),
-                                      cp.ctypes.data_as(_u32p), root.ctypes.data_as(_u32p)), "whir_multi_result")
-        return bytes(buf[: ln.value]), int(cp[0]), root
+        return _view_result(self.ctx.L.whir_multi_result_view, "whir_multi_result_view", self.ptr, job)
 
     def close(self):
         if self.ptr:

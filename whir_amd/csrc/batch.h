@@ -175,23 +175,33 @@ extern "C" int whir_batch_query(whir_batch* B) {
   return WHIR_EDEVICE;
 }
 
-extern "C" int whir_batch_result(whir_batch* B, int job, uint8_t* proof, size_t cap, size_t* len,
-                                 uint32_t* checkpoint, uint32_t* root8) {
+extern "C" int whir_batch_result_view(whir_batch* B, int job, const uint8_t** proof, size_t* len,
+                                      uint32_t* checkpoint, uint32_t* root8) {
   if (!B || !B->result_ok || job < 0 || (uint32_t)job >= B->jobs) return WHIR_EINVAL;
   HIP_TRY(hipEventSynchronize(B->done));
   const whir_prover* P = B->P;
   const uint32_t* log = (const uint32_t*)(B->host + (size_t)job * P->host_log_bytes);
-  const uint32_t* tail = log + P->plan.log_words;  // log_words, overflow, checkpoint
+  const uint32_t* tail = log + P->plan.log_words;  // log_words, overflow, checkpoint, failed checks
   if (tail[1] || tail[0] > P->plan.log_words) return WHIR_ETRANSCRIPT;
-  const size_t bytes = (size_t)tail[0] * 4;
-  if (len) *len = bytes;
+  if (len) *len = (size_t)tail[0] * 4;
   if (checkpoint) *checkpoint = tail[2];
   if (root8) memcpy(root8, log, 32);
+  if (proof) *proof = (const uint8_t*)log;
+  return P->checks && tail[3] ? WHIR_ECHECK : WHIR_OK;
+}
+
+extern "C" int whir_batch_result(whir_batch* B, int job, uint8_t* proof, size_t cap, size_t* len,
+                                 uint32_t* checkpoint, uint32_t* root8) {
+  const uint8_t* src = nullptr;
+  size_t bytes = 0;
+  const int rc = whir_batch_result_view(B, job, &src, &bytes, checkpoint, root8);
+  if (rc && rc != WHIR_ECHECK) return rc;
+  if (len) *len = bytes;
   if (proof) {
     if (bytes > cap) return WHIR_ETRANSCRIPT;
-    memcpy(proof, log, bytes);
+    memcpy(proof, src, bytes);
   }
-  return P->checks && tail[3] ? WHIR_ECHECK : WHIR_OK;
+  return rc;
 }
 
 extern "C" size_t whir_batch_proof_capacity(const whir_batch* B) { return B ? B->P->plan.log_words * 4 : 0; }
@@ -376,6 +386,13 @@ extern "C" int whir_multi_result(whir_multi* M, int job, uint8_t* proof, size_t 
   if (!M || !M->result_ok || job < 0 || job >= (int)M->job_slot.size()) return WHIR_EINVAL;
   return whir_batch_result(M->batches[M->job_slot[job].first], M->job_slot[job].second, proof, cap, len, checkpoint,
                            root8);
+}
+
+extern "C" int whir_multi_result_view(whir_multi* M, int job, const uint8_t** proof, size_t* len,
+                                      uint32_t* checkpoint, uint32_t* root8) {
+  if (!M || !M->result_ok || job < 0 || job >= (int)M->job_slot.size()) return WHIR_EINVAL;
+  return whir_batch_result_view(M->batches[M->job_slot[job].first], M->job_slot[job].second, proof, len,
+                                checkpoint, root8);
 }
 
 extern "C" size_t whir_multi_proof_capacity(const whir_multi* M, int job) {

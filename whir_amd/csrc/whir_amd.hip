@@ -1510,7 +1510,7 @@ extern "C" int whir_prover_result(whir_prover* P, uint8_t* proof, size_t cap, si
     }
     P->timer_pending = false;
   }
-  const uint32_t* tail = P->host_log + P->plan.log_words;  // log_words, overflow, checkpoint
+  const uint32_t* tail = P->host_log + P->plan.log_words;  // log_words, overflow, checkpoint, failed checks
   if (tail[1] || tail[0] > P->plan.log_words) return WHIR_ETRANSCRIPT;
   const size_t bytes = (size_t)tail[0] * 4;
   if (len) *len = bytes;
@@ -1521,6 +1521,14 @@ extern "C" int whir_prover_result(whir_prover* P, uint8_t* proof, size_t cap, si
     memcpy(proof, P->host_log, bytes);
   }
   return P->checks && tail[3] ? WHIR_ECHECK : WHIR_OK;
+}
+
+extern "C" int whir_prover_result_view(whir_prover* P, const uint8_t** proof, size_t* len, uint32_t* checkpoint,
+                                       uint32_t* root8) {
+  const int rc = whir_prover_result(P, nullptr, 0, len, checkpoint, root8);
+  if (rc && rc != WHIR_ECHECK) return rc;
+  if (proof) *proof = (const uint8_t*)P->host_log;
+  return rc;
 }
 
 extern "C" int whir_prover_set_checks(whir_prover* P, int enable) {
