@@ -108,7 +108,8 @@ __device__ __noinline__ uint32_t permute16_coop_shared(uint32_t s, CoopLane L) {
 struct CoopP16 {
   uint32_t la[9];   // lin_add[k][lane]
   uint32_t ia[20];  // int_add[r] on lane 0, 0 elsewhere
-  uint32_t d;       // V_lane as an integer; 2 on lane 0 (applied to 2p - x0)
+  uint32_t d;       // V_lane as an integer; 2 on lane 0, applied to 2p - x0: V_0 = -2, and the small
+                    // product keeps lane 0's REDC near-canonical for the next S-box's square
   uint32_t kappa[20];
   uint32_t fix;
 
