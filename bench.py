@@ -245,6 +245,7 @@ def batched_leg(ctx, slots, reps=3, n_jobs=64, seed=4, streams=12):
         "server_ms": round(ts * 1e3, 2),
         "reps": reps,
         "bit_identical": bool(same),
+        "tuned_plans_for_batches": sum(1 for t in ctx.tuning() if t.get("jobs", 1) > 1),
     }
 
 
@@ -467,7 +468,7 @@ def main():
     # with another tuned plan only the timing-based fields apply
     base_plan = next((t for t in (ctx.tuning() if not args.no_autotune else [])
                       if t["kind"] == "ntt" and t["log_rows"] == args.k - FOLDING + RATE
-                      and t["log_cols"] == FOLDING), None)
+                      and t["log_cols"] == FOLDING and t.get("jobs", 1) == 1), None)
     default_plan = base_plan is None or (base_plan["two_pass"] == 0 and base_plan["first_pass_log"] == 4
                                          and base_plan["log_vec"] == 1)
     dft_traffic = read_traffic("ntt_base_traffic_bytes") if default_plan else None

@@ -91,3 +91,35 @@ def test_autotuned_proofs_match_oracle(oracle):
             assert t["default_us"] <= 0 or t["best_us"] <= t["default_us"] + 1e-6
     finally:
         ctx.close()
+
+
+def test_batches_tuned_with_their_job_count(oracle):
+    """A lockstep batch on an autotuned context is timed with its own job
+    count (its plans kept for that job-count class, beside the single-proof
+    ones; wider Merkle levels are candidates) and every job's proof is still
+    the oracle's."""
+    import whir_amd
+
+    cfg = (18, 4, 1, 3, 1, 123, 1)
+    ctx = whir_amd.Context(0, 1000, autotune=True)
+    try:
+        polys = [oracle.rand_poly(80 + j, 18) for j in range(4)]
+        print("autotuned batch of 4, k=18", flush=True)
+        b = whir_amd.Batch(whir_amd.params(*cfg), 4, ctx=ctx)
+        try:
+            for j, p in enumerate(polys):
+                b.upload(j, p)
+            b.enqueue()
+            got = [b.result(j)[0] for j in range(4)]
+        finally:
+            b.close()
+        tuned = ctx.tuning()
+        assert {t["jobs"] for t in tuned} == {4}
+        assert {t["kind"] for t in tuned} == {"ntt", "merkle"}
+        # the single-proof path on the same context tunes its own plans
+        single = whir_amd.Whir(*cfg[:6], ctx=ctx).prove(polys[0], 1)[0]
+        assert {t["jobs"] for t in ctx.tuning()} == {1, 4}
+    finally:
+        ctx.close()
+    assert got == [oracle.whir_prove(*cfg, p)[0] for p in polys]
+    assert single == got[0]

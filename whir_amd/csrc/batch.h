@@ -87,7 +87,7 @@ extern "C" int whir_batch_create(whir_ctx* c, const whir_params* prm, int jobs, 
     whir_batch_destroy(B);
     return WHIR_EOOM;
   }
-  if (c->autotune) prover_autotune(P);  // single-job timings on job 0's buffers
+  if (c->autotune) prover_autotune(P);  // timed with the batch's job count, kept for that job-count class
   *out = B;
   return WHIR_OK;
 }

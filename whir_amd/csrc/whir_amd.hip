@@ -82,22 +82,34 @@ struct whir_ctx {
   // words), the configuration measured fastest on this device
   bool autotune = false;
   std::mutex tune_mu;
-  std::map<std::tuple<uint64_t, int, int>, NttConfig> ntt_tuned;
-  std::map<std::pair<int, uint32_t>, MerkleConfig> merkle_tuned;
+  // (keys end with the job-count class of the launches: lockstep batches
+  // are tuned with their own job count)
+  std::map<std::tuple<uint64_t, int, int, uint32_t>, NttConfig> ntt_tuned;
+  std::map<std::tuple<int, uint32_t, uint32_t>, MerkleConfig> merkle_tuned;
   std::vector<std::string> tune_log;  // one JSON object per tuned transform / tree
   uint32_t* tw = nullptr;  // tw_lo[4096] || tw_hi[4096]
   const uint32_t* tw_lo() const { return tw; }
   const uint32_t* tw_hi() const { return tw + 4096; }
 };
 
-static NttConfig ntt_config(whir_ctx* c, uint64_t n_in, int log_nn, int log_c) {
+// Job-count class of a launch over `jobs` lockstep proofs: 1, 2, 4, 8 or 16+.
+static uint32_t jobs_class(uint32_t jobs) {
+  uint32_t c = 1;
+  while (c < jobs && c < 16) c <<= 1;
+  return c;
+}
+// The tuned configuration for this size and job count, else the one tuned for
+// single proofs, else the context's default.
+static NttConfig ntt_config(whir_ctx* c, uint64_t n_in, int log_nn, int log_c, uint32_t jobs = 1) {
   std::lock_guard<std::mutex> g(c->tune_mu);
-  auto it = c->ntt_tuned.find(std::make_tuple(n_in, log_nn, log_c));
+  auto it = c->ntt_tuned.find(std::make_tuple(n_in, log_nn, log_c, jobs_class(jobs)));
+  if (it == c->ntt_tuned.end()) it = c->ntt_tuned.find(std::make_tuple(n_in, log_nn, log_c, 1u));
   return it == c->ntt_tuned.end() ? c->ntt : it->second;
 }
-static MerkleConfig merkle_config(whir_ctx* c, int log_h, uint32_t row_words) {
+static MerkleConfig merkle_config(whir_ctx* c, int log_h, uint32_t row_words, uint32_t jobs = 1) {
   std::lock_guard<std::mutex> g(c->tune_mu);
-  auto it = c->merkle_tuned.find(std::make_pair(log_h, row_words));
+  auto it = c->merkle_tuned.find(std::make_tuple(log_h, row_words, jobs_class(jobs)));
+  if (it == c->merkle_tuned.end()) it = c->merkle_tuned.find(std::make_tuple(log_h, row_words, 1u));
   return it == c->merkle_tuned.end() ? c->merkle : it->second;
 }
 
@@ -285,7 +297,7 @@ static bool ntt_plan_for(const NttConfig& cfg, uint64_t n_in, int log_nn, int lo
 // `force`: run with this configuration instead of the tuned or default one.
 static int ntt_enqueue(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_nn, int log_c, uint32_t* out,
                        uint32_t* tmp, hipStream_t s, const NttConfig* force = nullptr, uint32_t jobs = 1) {
-  const NttConfig cfg = force ? *force : ntt_config(c, n_in, log_nn, log_c);
+  const NttConfig cfg = force ? *force : ntt_config(c, n_in, log_nn, log_c, jobs);
   NttPlan pl;
   if (!ntt_plan_for(cfg, n_in, log_nn, log_c, pl)) return WHIR_EINVAL;
   for (int p = 0; p < pl.P; ++p) {
@@ -904,16 +916,23 @@ static std::string merkle_signature(int log_h, uint32_t row_words, const MerkleC
   return sig + "|" + std::to_string(n);
 }
 
-static std::vector<MerkleConfig> merkle_candidates(int log_h, uint32_t row_words) {
+static std::vector<MerkleConfig> merkle_candidates(int log_h, uint32_t row_words, uint32_t jobs = 1) {
   std::vector<MerkleConfig> out;
   std::vector<std::string> seen;
 #if defined(WHIR_FIELD_BABYBEAR)
-  const int pair_logs[] = {-1};  // no two-lane kernels in this build
+  std::vector<int> pair_logs = {-1};  // no two-lane kernels in this build
 #else
-  const int pair_logs[] = {15, 14, 16, 13};
+  std::vector<int> pair_logs = {15, 14, 16, 13};
 #endif
+  std::vector<int> coop_logs = {15, 13, 14, 16, 17};
+  if (jobs > 1) {  // a batch has jobs x the lanes per level: cheaper wide levels further up
+    coop_logs.insert(coop_logs.end(), {12, 11, 10});
+#if !defined(WHIR_FIELD_BABYBEAR)
+    pair_logs.insert(pair_logs.end(), {12, 10});
+#endif
+  }
   for (int pm : pair_logs)
-    for (int cf : {15, 13, 14, 16, 17}) {
+    for (int cf : coop_logs) {
       MerkleConfig mc;
       mc.pair_max = pm < 0 ? 0 : 1ull << pm;
       mc.coop_from = 1ull << cf;
@@ -930,8 +949,8 @@ constexpr uint64_t TUNE_MIN_NTT_WORDS = 1ull << 16;
 constexpr int TUNE_MIN_TREE_LOG = 12;
 
 static void tune_ntt(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_nn, int log_c, uint32_t* out,
-                     uint32_t* tmp) {
-  const auto key = std::make_tuple(n_in, log_nn, log_c);
+                     uint32_t* tmp, uint32_t jobs) {
+  const auto key = std::make_tuple(n_in, log_nn, log_c, jobs_class(jobs));
   {
     std::lock_guard<std::mutex> g(c->tune_mu);
     if (c->ntt_tuned.count(key)) return;
@@ -942,7 +961,8 @@ static void tune_ntt(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_nn,
   NttConfig best = c->ntt;
   double best_ms = -1.0, default_ms = -1.0;
   for (const NttConfig& cfg : cand) {
-    const double ms = time_min_ms(c->stream, [&] { return ntt_enqueue(c, in, n_in, log_nn, log_c, out, tmp, c->stream, &cfg); });
+    const double ms =
+        time_min_ms(c->stream, [&] { return ntt_enqueue(c, in, n_in, log_nn, log_c, out, tmp, c->stream, &cfg, jobs); });
     if (ms < 0) continue;
     if (cfg.two_pass == c->ntt.two_pass && cfg.first_log == c->ntt.first_log && cfg.log_threads == c->ntt.log_threads &&
         cfg.max_lv == c->ntt.max_lv)
@@ -955,29 +975,32 @@ static void tune_ntt(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_nn,
   if (best_ms < 0) return;
   char buf[320];
   snprintf(buf, sizeof(buf),
-           "{\"kind\": \"ntt\", \"n_in\": %llu, \"log_rows\": %d, \"log_cols\": %d, \"candidates\": %zu, "
-           "\"two_pass\": %d, \"first_pass_log\": %d, \"log_threads\": %d, \"log_vec\": %d, \"best_us\": %.1f, "
-           "\"default_us\": %.1f}",
-           (unsigned long long)n_in, log_nn, log_c, cand.size(), best.two_pass, best.first_log, best.log_threads,
-           best.max_lv, best_ms * 1e3, default_ms * 1e3);
+           "{\"kind\": \"ntt\", \"n_in\": %llu, \"log_rows\": %d, \"log_cols\": %d, \"jobs\": %u, "
+           "\"candidates\": %zu, \"two_pass\": %d, \"first_pass_log\": %d, \"log_threads\": %d, \"log_vec\": %d, "
+           "\"best_us\": %.1f, \"default_us\": %.1f}",
+           (unsigned long long)n_in, log_nn, log_c, jobs_class(jobs), cand.size(), best.two_pass, best.first_log,
+           best.log_threads, best.max_lv, best_ms * 1e3, default_ms * 1e3);
   std::lock_guard<std::mutex> g(c->tune_mu);
   c->ntt_tuned[key] = best;
   c->tune_log.push_back(buf);
 }
 
-static void tune_merkle(whir_ctx* c, const uint32_t* rows, int log_h, uint32_t row_words, uint32_t* layers) {
-  const auto key = std::make_pair(log_h, row_words);
+static void tune_merkle(whir_ctx* c, const uint32_t* rows, int log_h, uint32_t row_words, uint32_t* layers,
+                        uint32_t jobs) {
+  const auto key = std::make_tuple(log_h, row_words, jobs_class(jobs));
   {
     std::lock_guard<std::mutex> g(c->tune_mu);
     if (c->merkle_tuned.count(key)) return;
   }
   if (log_h < TUNE_MIN_TREE_LOG) return;
-  const std::vector<MerkleConfig> cand = merkle_candidates(log_h, row_words);
+  const std::vector<MerkleConfig> cand = merkle_candidates(log_h, row_words, jobs);
   if (cand.size() < 2) return;
   MerkleConfig best = c->merkle;
   double best_ms = -1.0, default_ms = -1.0;
   for (const MerkleConfig& mc : cand) {
-    const double ms = time_min_ms(c->stream, [&] { return merkle_enqueue(rows, log_h, row_words, layers, c->stream, mc); });
+    const double ms = time_min_ms(c->stream, [&] {
+      return merkle_enqueue(rows, log_h, row_words, layers, c->stream, mc, nullptr, nullptr, nullptr, jobs);
+    });
     if (ms < 0) continue;
     if (mc.pair_max == c->merkle.pair_max && mc.coop_from == c->merkle.coop_from) default_ms = ms;
     if (best_ms < 0 || ms < best_ms) {
@@ -991,9 +1014,9 @@ static void tune_merkle(whir_ctx* c, const uint32_t* rows, int log_h, uint32_t r
   while ((1ull << cf) < best.coop_from) ++cf;
   char buf[320];
   snprintf(buf, sizeof(buf),
-           "{\"kind\": \"merkle\", \"log_rows\": %d, \"row_words\": %u, \"candidates\": %zu, "
+           "{\"kind\": \"merkle\", \"log_rows\": %d, \"row_words\": %u, \"jobs\": %u, \"candidates\": %zu, "
            "\"pair_max_log\": %d, \"coop_from_log\": %d, \"best_us\": %.1f, \"default_us\": %.1f}",
-           log_h, row_words, cand.size(), pm, cf, best_ms * 1e3, default_ms * 1e3);
+           log_h, row_words, jobs_class(jobs), cand.size(), pm, cf, best_ms * 1e3, default_ms * 1e3);
   std::lock_guard<std::mutex> g(c->tune_mu);
   c->merkle_tuned[key] = best;
   c->tune_log.push_back(buf);
@@ -1007,11 +1030,11 @@ static void prover_autotune(whir_prover* P) {
   const int k = pl.k, f = pl.f;
   const uint32_t W = 1u << f;
   const int log_h0 = k + pl.rate - f;
-  tune_ntt(c, P->input, 1ull << (k - f), log_h0, f, P->cw0, P->ntt_tmp);
-  tune_merkle(c, P->cw0, log_h0, W, P->tree0);
+  tune_ntt(c, P->input, 1ull << (k - f), log_h0, f, P->cw0, P->ntt_tmp, P->jobs);
+  tune_merkle(c, P->cw0, log_h0, W, P->tree0, P->jobs);
   for (const RoundPlan& rp : pl.rounds) {
-    tune_ntt(c, (const uint32_t*)P->ep[0], 1ull << (rp.kc - f), rp.log_rows, f + 2, P->cw[0], P->ntt_tmp);
-    tune_merkle(c, P->cw[0], rp.log_rows, 4 * W, P->tree[0]);
+    tune_ntt(c, (const uint32_t*)P->ep[0], 1ull << (rp.kc - f), rp.log_rows, f + 2, P->cw[0], P->ntt_tmp, P->jobs);
+    tune_merkle(c, P->cw[0], rp.log_rows, 4 * W, P->tree[0], P->jobs);
   }
   (void)hipGetLastError();
 }
@@ -1162,7 +1185,7 @@ static int enqueue_commit(whir_prover* P, const uint32_t* poly, hipStream_t s, c
   if (P->timers) HIP_TRY(hipEventRecord(P->ev[WHIR_TIMER_DFT_BASE][1], s));
   uint32_t* top = nullptr;
   uint32_t top_n = 0;
-  if ((rc = merkle_enqueue(P->cw0, log_h0, W, P->tree0, s, merkle_config(c, log_h0, W),
+  if ((rc = merkle_enqueue(P->cw0, log_h0, W, P->tree0, s, merkle_config(c, log_h0, W, P->jobs),
                            P->timers ? P->ev[WHIR_TIMER_LEAVES_BASE] : nullptr, &top, &top_n, P->jobs)))
     return rc;
   P->timer_pending = P->timers;
@@ -1241,7 +1264,7 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
                           nullptr, P->jobs)))
       return rc;
     const bool small = r > 0 && small_round_fits(rp.kc, rp.n_ood, rp.n_stir);
-    if ((rc = merkle_enqueue(cw, rp.log_rows, 4 * W, tree, s, merkle_config(c, rp.log_rows, 4 * W), nullptr,
+    if ((rc = merkle_enqueue(cw, rp.log_rows, 4 * W, tree, s, merkle_config(c, rp.log_rows, 4 * W, P->jobs), nullptr,
                              small ? nullptr : &top, &top_n, P->jobs)))
       return rc;
     const uint64_t Hr = 1ull << rp.log_rows;
