@@ -188,7 +188,7 @@ def _drain(srv, specs, bufs):
     return [srv.take(j, 1 << 22)[0] for j in ids]
 
 
-def batched_leg(ctx, slots, reps=3, n_jobs=64, seed=4, streams=12):
+def batched_leg(ctx, slots, reps=3, n_jobs=64, seed=4, streams=12, queue_groups=(1, 2)):
     """64 independent proofs of mixed sizes: grouped by shape into lockstep
     batches side by side, the whole set one captured hipGraph (whir_multi);
     beside it the proof server (one graph per job over `slots` streams) and the
@@ -203,15 +203,22 @@ def batched_leg(ctx, slots, reps=3, n_jobs=64, seed=4, streams=12):
     plain = PlainPath(ctx)
     srv = whir_amd.Server(slots, 64 << 30, ctx=ctx)
     multi = whir_amd.MultiBatch(specs, streams=streams, ctx=ctx, graph=1)
+    # the same set with the commit work queue: every job's base LDE, leaves and
+    # wide tree layers from persistent kernels over one device job table
+    multi_q = {g: whir_amd.MultiBatch(specs, streams=streams, ctx=ctx, graph=1, commit_queue=g)
+               for g in queue_groups}
     ref = [plain.prove(p, b.ptr) for p, b in zip(specs, bufs)]  # warm-up: workspaces
     _drain(srv, specs, bufs)  # warm-up: workspaces + graph capture
 
-    def run_multi():
-        multi.enqueue([b.ptr for b in bufs])
-        return [multi.result(i)[0] for i in range(n_jobs)]
+    def run_multi(m):
+        m.enqueue([b.ptr for b in bufs])
+        return [m.result(i)[0] for i in range(n_jobs)]
 
-    run_multi()  # warm-up: graph capture
+    run_multi(multi)  # warm-up: graph capture
+    for m in multi_q.values():
+        run_multi(m)
     t_plain, t_srv, t_multi, same = [], [], [], True
+    t_queue = {g: [] for g in multi_q}
     for _ in range(reps):
         whir_amd.lib().whir_device_synchronize()
         t0 = time.perf_counter()
@@ -223,13 +230,23 @@ def batched_leg(ctx, slots, reps=3, n_jobs=64, seed=4, streams=12):
         t_srv.append(time.perf_counter() - t0)
         same &= got == ref
         t0 = time.perf_counter()
-        got = run_multi()
+        got = run_multi(multi)
         t_multi.append(time.perf_counter() - t0)
         same &= got == ref
+        for g, m in multi_q.items():
+            t0 = time.perf_counter()
+            got = run_multi(m)
+            t_queue[g].append(time.perf_counter() - t0)
+            same &= got == ref
+    queue_jobs = {g: m.queue_jobs for g, m in multi_q.items()}
+    for m in multi_q.values():
+        m.close()
     multi.close()
     srv.close()
     plain.close()
     tp, ts, tm = min(t_plain), min(t_srv), min(t_multi)
+    tq = {g: min(v) for g, v in t_queue.items()}
+    gbest = min(tq, key=tq.get)
     return {
         "workload": f"{n_jobs} jobs, k uniform in [16,22] (seed {seed}), rates 1/2+1/4 mixed, folding 4, 1 claim",
         "jobs_by_k": {int(k): int((ks == k).sum()) for k in sorted(set(ks.tolist()))},
@@ -240,6 +257,17 @@ def batched_leg(ctx, slots, reps=3, n_jobs=64, seed=4, streams=12):
         "speedup": round(tp / tm, 3),
         "plain_ms": round(tp * 1e3, 2),
         "batched_ms": round(tm * 1e3, 2),
+        "commit_queue": {
+            "method": ("the same set with the commit work queue: every job's base LDE, leaf hashes and wide "
+                       "tree layers from persistent kernels draining a device job table (one launch per "
+                       "stage), the batches in G groups whose queues run one after another, each group's "
+                       "openings overlapping the next group's commit; one captured hipGraph"),
+            "jobs_in_queue": queue_jobs[gbest],
+            "groups": gbest,
+            "proofs_per_s": round(n_jobs / tq[gbest], 1),
+            "ms": round(tq[gbest] * 1e3, 2),
+            "proofs_per_s_by_groups": {g: round(n_jobs / t, 1) for g, t in sorted(tq.items())},
+        },
         "server_slots": slots,
         "server_proofs_per_s": round(n_jobs / ts, 1),
         "server_ms": round(ts * 1e3, 2),

@@ -391,6 +391,19 @@ int whir_multi_create(whir_ctx* ctx, const whir_params* params, int n_jobs, int 
 int whir_multi_destroy(whir_multi* multi);
 uint32_t* whir_multi_input(whir_multi* multi, int job);
 int whir_multi_set_graph(whir_multi* multi, int mode);
+/* Commit work queue (replaces, for the whole set, the per-job calls of
+ * Whir::commit's encode and tree, crates/whir/src/whir.rs:19-36 and
+ * crates/merkle/src/poseidon_packed.rs:96-143, 268-304): with groups >= 1 every
+ * job of folding 4 has its base LDE, leaf hashes and wide tree layers computed
+ * by persistent kernels that drain a device job table (one launch per LDE pass
+ * and per tree layer, items dequeued with an atomic counter); the batches go
+ * on from the cooperative top of each tree. The batches are split into
+ * `groups` groups of about equal commit work whose queues run one after
+ * another, each group's batches starting when its queue is done (the openings
+ * of one group overlap the next group's commit). groups = 0 turns it off.
+ * Bytes are unchanged. Drops captured graphs. *n_jobs (may be NULL): the
+ * number of jobs the queue covers. */
+int whir_multi_set_commit_queue(whir_multi* multi, int groups, int* n_jobs);
 int whir_multi_enqueue(whir_multi* multi, const uint32_t* const* d_polys, void* stream);
 int whir_multi_result(whir_multi* multi, int job, uint8_t* proof, size_t cap, size_t* len, uint32_t* checkpoint,
                       uint32_t* root8);

@@ -132,6 +132,28 @@ def test_batch_matches_oracle(bb_ctx, bb):
     assert got == [bb.whir_prove(*cfg, p)[0] for p in polys]
 
 
+def test_multi_commit_queue_matches_oracle(bb_ctx, bb):
+    """A mixed-shape set with the commit work queue over the BabyBear build."""
+    import whir_amd
+
+    cfgs = [(13, 4, 1, 3, 1, 123, 1), (15, 4, 2, 3, 1, 123, 1), (13, 4, 1, 3, 1, 123, 1), (14, 4, 1, 3, 1, 123, 2)]
+    polys = [bb.rand_poly(80 + i, c[0]) for i, c in enumerate(cfgs)]
+    bufs = []
+    for p in polys:
+        d = whir_amd.api.DeviceBuffer(p.nbytes)
+        d.upload(p)
+        bufs.append(d)
+    print("babybear mixed-shape set with the commit queue", flush=True)
+    m = whir_amd.MultiBatch([whir_amd.params(*c) for c in cfgs], streams=2, ctx=bb_ctx, graph=1, commit_queue=1)
+    try:
+        assert m.queue_jobs == len(cfgs)
+        m.enqueue([d.ptr for d in bufs])
+        got = [m.result(i)[0] for i in range(len(cfgs))]
+    finally:
+        m.close()
+    assert got == [bb.whir_prove(*c, p)[0] for c, p in zip(cfgs, polys)]
+
+
 def test_session_with_prefix_and_given_point(bb_ctx, bb):
     """Whir::commit / ProverStack::eval / Whir::open inside a caller's
     BabyBear transcript (crates/whir/src/whir.rs:127-180): the GPU's bytes

@@ -172,3 +172,41 @@ def test_copying_results_equal_the_views(gpu_ctx, oracle):
         pr.close()
         b.close()
         m.close()
+
+
+@pytest.mark.parametrize("graph,groups", [(0, 1), (1, 2), (2, 3), (1, 8)])
+def test_multi_commit_queue_matches_oracle(gpu_ctx, oracle, graph, groups):
+    """The commit work queue (persistent NTT / leaf / layer kernels draining a
+    device job table of every folding-4 job, uneven sizes and rates; the
+    batches in `groups` pipelined groups) ahead of the batches: every job's
+    bytes equal the oracle's, replayed twice; the job of folding 2 keeps its
+    own commit; switching the queue off and on again gives the same bytes."""
+    import whir_amd
+
+    cfgs = [(12, 4, 1, 3, 1, 123, 1), (16, 4, 2, 3, 1, 123, 1), (13, 4, 1, 3, 1, 123, 2), (12, 4, 1, 3, 1, 123, 1),
+            (17, 4, 1, 3, 1, 123, 1), (10, 2, 1, 1, 1, 32, 3), (15, 4, 2, 3, 1, 123, 1), (14, 4, 1, 3, 1, 123, 1)]
+    polys = [oracle.rand_poly(1300 + i, c[0]) for i, c in enumerate(cfgs)]
+    want = [oracle.whir_prove(*c, p)[0] for c, p in zip(cfgs, polys)]
+    bufs = []
+    for p in polys:
+        d = whir_amd.api.DeviceBuffer(p.nbytes)
+        d.upload(p)
+        bufs.append(d)
+    print(f"commit queue: {len(cfgs)} jobs, graph={graph}, groups={groups}", flush=True)
+    m = whir_amd.MultiBatch([whir_amd.params(*c) for c in cfgs], streams=3, ctx=gpu_ctx, graph=graph,
+                            commit_queue=groups)
+    try:
+        assert m.queue_jobs == len(cfgs) - 1  # every job but the folding-2 one
+        for rep in range(2):
+            m.enqueue([d.ptr for d in bufs])
+            got = [m.result(i)[0] for i in range(len(cfgs))]
+            for i in range(len(cfgs)):
+                assert got[i] == want[i], (rep, i, cfgs[i])
+        assert m.set_commit_queue(0) == 0
+        m.enqueue([d.ptr for d in bufs])
+        assert [m.result(i)[0] for i in range(len(cfgs))] == want
+        assert m.set_commit_queue(groups) == len(cfgs) - 1
+        m.enqueue([d.ptr for d in bufs])
+        assert [m.result(i)[0] for i in range(len(cfgs))] == want
+    finally:
+        m.close()

@@ -174,3 +174,24 @@ def test_served_stream_against_oracle_and_alone(gpu_ctx, oracle):
         finally:
             alone.close()
         assert proof == oracle.whir_prove(k, 4, rate, 3, 1, 123, n, poly)[0], (k, rate, n)
+
+
+def test_batched_mix_commit_queue_against_oracle(gpu_ctx, oracle):
+    """The 64-job workload's shape mix (k 16..22, rates 1/2 and 1/4, shapes
+    repeated) through a mixed-shape set with the commit work queue, one
+    captured graph: every job equals the oracle's proof."""
+    import whir_amd
+
+    cfgs = [(16, 1, 1), (22, 2, 1), (17, 2, 1), (21, 1, 1), (18, 1, 1), (20, 2, 1), (19, 1, 1), (16, 2, 1),
+            (22, 1, 1), (18, 2, 1), (16, 1, 1), (19, 1, 1)]
+    specs, bufs, polys = _jobs(oracle, cfgs, 800)
+    print(f"batched mix, commit queue: {len(cfgs)} jobs", flush=True)
+    m = whir_amd.MultiBatch(specs, streams=6, ctx=gpu_ctx, graph=1, commit_queue=2)
+    try:
+        assert m.queue_jobs == len(cfgs)
+        m.enqueue([d.ptr for d in bufs])
+        got = [m.result(i)[0] for i in range(len(cfgs))]
+    finally:
+        m.close()
+    for (k, rate, n), poly, proof in zip(cfgs, polys, got):
+        assert proof == oracle.whir_prove(k, 4, rate, 3, 1, 123, n, poly)[0], (k, rate)

@@ -141,6 +141,7 @@ SIGNATURES = [
     ("whir_multi_destroy", ctypes.c_int, [_vp]),
     ("whir_multi_input", _vp, [_vp, ctypes.c_int]),
     ("whir_multi_set_graph", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("whir_multi_set_commit_queue", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     ("whir_multi_enqueue", ctypes.c_int, [_vp, ctypes.POINTER(_vp), _vp]),
     ("whir_multi_result", ctypes.c_int,
      [_vp, ctypes.c_int, _u8p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _u32p, _u32p]),

@@ -481,8 +481,11 @@ class MultiBatch:
     lockstep batches that run side by side on `streams` streams (graph: the
     whole set as one captured hipGraph)."""
 
-    def __init__(self, params_list, streams=8, ctx=None, graph=1):
-        """graph: 0 eager, 1 one captured graph for the whole set, 2 one per batch."""
+    def __init__(self, params_list, streams=8, ctx=None, graph=1, commit_queue=0):
+        """graph: 0 eager, 1 one captured graph for the whole set, 2 one per batch.
+        commit_queue: number of job groups whose base LDEs, leaf hashes and
+        wide tree layers come from persistent kernels over a device job table
+        (whir_multi_set_commit_queue); 0 off."""
         self.ctx = ctx or default_context()
         self.n = len(params_list)
         arr = (whir_params * self.n)(*params_list)
@@ -490,6 +493,18 @@ class MultiBatch:
         check(self.ctx.L.whir_multi_create(self.ctx.ptr, arr, self.n, streams, ctypes.byref(p)), "whir_multi_create")
         self.ptr = p.value
         check(self.ctx.L.whir_multi_set_graph(self.ptr, int(graph)), "whir_multi_set_graph")
+        self.queue_jobs = 0
+        if commit_queue:
+            self.set_commit_queue(int(commit_queue))
+
+    def set_commit_queue(self, groups):
+        """groups: 0 off, else the number of pipelined job groups. Returns the
+        number of jobs the commit work queue covers."""
+        n = ctypes.c_int(0)
+        check(self.ctx.L.whir_multi_set_commit_queue(self.ptr, int(groups), ctypes.byref(n)),
+              "whir_multi_set_commit_queue")
+        self.queue_jobs = n.value
+        return n.value
 
     def input_ptr(self, job):
         return self.ctx.L.whir_multi_input(self.ptr, job)

@@ -10,6 +10,7 @@
 // jobs' serial chains side by side. Each job's transcript, arithmetic and
 // proof bytes are exactly those of the job proven alone.
 #pragma once
+#include "jobtable.h"
 
 struct whir_batch {
   whir_prover* P = nullptr;  // job 0's layout; its workspace pointers carry the stride
@@ -227,6 +228,13 @@ struct whir_multi {
   hipGraphExec_t gexec = nullptr;
   hipEvent_t done = nullptr;
   bool result_ok = false;
+  // commit work queue (jobtable.h): every eligible job's base LDE, leaves and
+  // wide tree layers in persistent launches ahead of the batches, in groups
+  // of batches whose queues run one after another on the caller's stream
+  std::vector<JobTable> jt;         // one table per group
+  std::vector<hipEvent_t> jt_done;  // per group: its queue has finished
+  std::vector<int> batch_group;     // per batch: its group, -1 outside the queue
+  int queue_jobs = 0;
 };
 
 extern "C" int whir_multi_destroy(whir_multi* M) {
@@ -235,6 +243,9 @@ extern "C" int whir_multi_destroy(whir_multi* M) {
   for (hipStream_t s : M->streams) release_step(rc, hipStreamSynchronize(s));
   if (M->gexec) release_step(rc, hipGraphExecDestroy(M->gexec));
   if (M->graph) release_step(rc, hipGraphDestroy(M->graph));
+  for (JobTable& t : M->jt)
+    if (t.dev) release_step(rc, hipFree(t.dev));
+  for (hipEvent_t e : M->jt_done) release_step(rc, hipEventDestroy(e));
   for (whir_batch* b : M->batches) {
     const int rb = whir_batch_destroy(b);
     if (rc == WHIR_OK) rc = rb;
@@ -311,15 +322,130 @@ extern "C" int whir_multi_set_graph(whir_multi* M, int mode) {
   return WHIR_OK;
 }
 
+// Drops the captured graphs of the set and of its batches (their launch
+// sequences change with the commit mode).
+static int multi_drop_graphs(whir_multi* M) {
+  for (hipStream_t st : M->streams) HIP_TRY(hipStreamSynchronize(st));
+  if (M->done) HIP_TRY(hipEventSynchronize(M->done));
+  if (M->gexec) HIP_TRY(hipGraphExecDestroy(M->gexec));
+  if (M->graph) HIP_TRY(hipGraphDestroy(M->graph));
+  M->gexec = nullptr;
+  M->graph = nullptr;
+  for (whir_batch* B : M->batches) {
+    if (B->done) HIP_TRY(hipEventSynchronize(B->done));
+    if (B->gexec) HIP_TRY(hipGraphExecDestroy(B->gexec));
+    if (B->graph) HIP_TRY(hipGraphDestroy(B->graph));
+    B->gexec = nullptr;
+    B->graph = nullptr;
+  }
+  M->result_ok = false;
+  return WHIR_OK;
+}
+
+static void multi_free_queue(whir_multi* M) {
+  for (JobTable& t : M->jt)
+    if (t.dev) (void)hipFree(t.dev);
+  for (hipEvent_t e : M->jt_done) (void)hipEventDestroy(e);
+  M->jt.clear();
+  M->jt_done.clear();
+  M->batch_group.clear();
+  M->queue_jobs = 0;
+  for (whir_batch* B : M->batches) B->P->ext_commit_levels = -1;
+}
+
+// Commit work queue. groups = 0: off. groups >= 1: every job whose shape the
+// queue kernels take (folding 4: 16-word codeword rows; an LDE plan of
+// 256-thread passes) gets its base LDE, leaf hashes and wide tree layers from
+// persistent launches over a device job table (jobs largest first), and its
+// batch starts its base tree at the cooperative stages. The eligible batches
+// are split into `groups` groups of about equal commit work, one job table
+// each; the groups' queues run one after another on the caller's stream and a
+// group's batches start when its queue is done, so the openings of one group
+// overlap the next group's commit. The proofs are byte-for-byte those of the
+// batches alone. *n_jobs (optional): the number of jobs the queue covers.
+extern "C" int whir_multi_set_commit_queue(whir_multi* M, int groups, int* n_jobs) {
+  if (!M || groups < 0 || groups > 16) return WHIR_EINVAL;
+  if (n_jobs) *n_jobs = 0;
+  int rc = multi_drop_graphs(M);
+  if (rc) return rc;
+  multi_free_queue(M);
+  if (groups == 0) return WHIR_OK;
+  // eligible batches, largest commit work first, dealt to the least-loaded group
+  struct Elig {
+    int b;
+    uint64_t work;
+    int log_h, wide;
+  };
+  std::vector<Elig> el;
+  for (size_t b = 0; b < M->batches.size(); ++b) {
+    whir_prover* P = M->batches[b]->P;
+    const Plan& pl = P->plan;
+    if (pl.f != 4) continue;
+    const int log_h = pl.k + pl.rate - pl.f;
+    JtJobSpec sp{P->input, P->cw0, P->ntt_tmp, P->tree0, 1ull << (pl.k - pl.f), log_h, 0};
+    std::vector<NttPass> probe;
+    std::vector<uint32_t> kinds;
+    if (!jt_ntt_passes(sp, M->ctx->tw_lo(), M->ctx->tw_hi(), probe, kinds)) continue;
+    const int wide = merkle_wide_levels(log_h, merkle_config(M->ctx, log_h, 16, M->batches[b]->jobs));
+    el.push_back({(int)b, (uint64_t)M->batches[b]->jobs << log_h, log_h, wide});
+  }
+  if (el.empty()) return WHIR_OK;
+  std::stable_sort(el.begin(), el.end(), [](const Elig& x, const Elig& y) { return x.work > y.work; });
+  const int G = std::min<int>(groups, (int)el.size());
+  std::vector<uint64_t> load((size_t)G, 0);
+  std::vector<std::vector<std::pair<uint64_t, JtJobSpec>>> specs((size_t)G);
+  M->batch_group.assign(M->batches.size(), -1);
+  for (const Elig& e : el) {
+    const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[g] += e.work;
+    M->batch_group[e.b] = g;
+    whir_batch* B = M->batches[e.b];
+    whir_prover* P = B->P;
+    for (uint32_t j = 0; j < B->jobs; ++j)
+      specs[g].push_back({1ull << e.log_h,
+                          JtJobSpec{job_addr(P->input, j), job_addr(P->cw0, j), job_addr(P->ntt_tmp, j),
+                                    job_addr(P->tree0, j), 1ull << (P->plan.k - P->plan.f), e.log_h, e.wide}});
+  }
+  int n_cu = 0;
+  HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, M->ctx->device));
+  M->jt.resize((size_t)G);
+  for (int g = 0; g < G; ++g) {
+    std::stable_sort(specs[g].begin(), specs[g].end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+    std::vector<JtJobSpec> jobs;
+    for (auto& x : specs[g]) jobs.push_back(x.second);
+    const int brc = jt_build(jobs, M->ctx->tw_lo(), M->ctx->tw_hi(), n_cu, M->jt[g]);
+    hipEvent_t ev = nullptr;
+    if (brc || hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      multi_free_queue(M);
+      return brc == 3 ? WHIR_EOOM : WHIR_EDEVICE;
+    }
+    M->jt_done.push_back(ev);
+    M->queue_jobs += (int)jobs.size();
+  }
+  for (const Elig& e : el) M->batches[e.b]->P->ext_commit_levels = e.wide;
+  if (n_jobs) *n_jobs = M->queue_jobs;
+  return WHIR_OK;
+}
+
 // Fork from s onto the internal streams, every batch's proofs, join back into
 // s. Outside a capture each batch's completion is recorded on its own stream,
-// so its proofs can be read while later batches still run.
+// so its proofs can be read while later batches still run. With the commit
+// work queue on, the groups' queues run on s after the fork, each group's
+// batches waiting for their group's queue.
 static int multi_enqueue_all(whir_multi* M, hipStream_t s) {
   HIP_TRY(hipEventRecord(M->events[0], s));
   for (hipStream_t st : M->streams) HIP_TRY(hipStreamWaitEvent(st, M->events[0], 0));
+  for (size_t g = 0; g < M->jt.size(); ++g) {
+    jt_enqueue(M->jt[g], s);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(M->jt_done[g], s));
+  }
   for (size_t b = 0; b < M->batches.size(); ++b) {
     whir_batch* B = M->batches[b];
     hipStream_t bs = M->streams[M->batch_stream[b]];
+    if (!M->batch_group.empty() && M->batch_group[b] >= 0)
+      HIP_TRY(hipStreamWaitEvent(bs, M->jt_done[M->batch_group[b]], 0));
     const int rc = M->graph_mode == 2 ? whir_batch_enqueue(B, nullptr, bs) : batch_enqueue_proofs(B, bs);
     if (rc) return rc;
     if (M->graph_mode != 1) {

@@ -17,14 +17,11 @@ namespace kb {
 
 constexpr int MERKLE_THREADS = 256;
 
-// One sponge per lane; ROW_WORDS is the flattened row length (base words).
-template <int ROW_WORDS>
-__global__ void __launch_bounds__(MERKLE_THREADS) merkle_leaves_kernel(JP<const uint32_t> rows_, uint64_t n_rows,
-                                                                   JP<uint32_t> leaves_) {
-  const uint32_t* __restrict__ rows = rows_;
-  uint32_t* __restrict__ leaves = leaves_;
-  const uint64_t i = (uint64_t)blockIdx.x * MERKLE_THREADS + threadIdx.x;
-  if (i >= n_rows) return;
+// Leaf i: the sponge of row i (ROW_WORDS flattened base words), on this lane.
+// OPAQUE: the round constants are loaded inside the caller's loop (opaque_ref).
+template <int ROW_WORDS, bool OPAQUE = false>
+__device__ __forceinline__ void leaf_hash_row(const uint32_t* __restrict__ rows, uint64_t i,
+                                              uint32_t* __restrict__ leaves) {
   const uint32_t* row = rows + i * ROW_WORDS;
   uint32_t s[24];
 #pragma unroll
@@ -49,30 +46,45 @@ __global__ void __launch_bounds__(MERKLE_THREADS) merkle_leaves_kernel(JP<const 
 #pragma unroll
       for (int j = 0; j < FULL; ++j) s[j] = row[c * 16 + j];
     }
-    permute<24, OUT>(s);
+    permute<24, OUT, OPAQUE>(s);
   }
   uint4* out = reinterpret_cast<uint4*>(leaves + i * 8);
   out[0] = make_uint4(s[0], s[1], s[2], s[3]);
   out[1] = make_uint4(s[4], s[5], s[6], s[7]);
 }
 
-// parent[j] = C(prev[j], prev[j + half]) for j < half.
-__global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_kernel(JP<const uint32_t> prev_, uint64_t half,
-                                                                  JP<uint32_t> next_) {
-  const uint32_t* __restrict__ prev = prev_;
-  uint32_t* __restrict__ next = next_;
-  const uint64_t j = (uint64_t)blockIdx.x * MERKLE_THREADS + threadIdx.x;
-  if (j >= half) return;
+// One sponge per lane.
+template <int ROW_WORDS>
+__global__ void __launch_bounds__(MERKLE_THREADS) merkle_leaves_kernel(JP<const uint32_t> rows_, uint64_t n_rows,
+                                                                   JP<uint32_t> leaves_) {
+  const uint64_t i = (uint64_t)blockIdx.x * MERKLE_THREADS + threadIdx.x;
+  if (i >= n_rows) return;
+  leaf_hash_row<ROW_WORDS>(rows_, i, leaves_);
+}
+
+// parent[j] = C(prev[j], prev[j + half]), on this lane (OPAQUE as for
+// leaf_hash_row).
+template <bool OPAQUE = false>
+__device__ __forceinline__ void layer_node(const uint32_t* __restrict__ prev, uint64_t half, uint64_t j,
+                                           uint32_t* __restrict__ next) {
   uint32_t l[8], r[8], o[8];
   const uint4* pl = reinterpret_cast<const uint4*>(prev + j * 8);
   const uint4* pr = reinterpret_cast<const uint4*>(prev + (j + half) * 8);
   uint4 a = pl[0], b = pl[1], c = pr[0], d = pr[1];
   l[0] = a.x; l[1] = a.y; l[2] = a.z; l[3] = a.w; l[4] = b.x; l[5] = b.y; l[6] = b.z; l[7] = b.w;
   r[0] = c.x; r[1] = c.y; r[2] = c.z; r[3] = c.w; r[4] = d.x; r[5] = d.y; r[6] = d.z; r[7] = d.w;
-  compress(l, r, o);
+  compress<OPAQUE>(l, r, o);
   uint4* po = reinterpret_cast<uint4*>(next + j * 8);
   po[0] = make_uint4(o[0], o[1], o[2], o[3]);
   po[1] = make_uint4(o[4], o[5], o[6], o[7]);
+}
+
+// parent[j] = C(prev[j], prev[j + half]) for j < half, one compression per lane.
+__global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_kernel(JP<const uint32_t> prev_, uint64_t half,
+                                                                  JP<uint32_t> next_) {
+  const uint64_t j = (uint64_t)blockIdx.x * MERKLE_THREADS + threadIdx.x;
+  if (j >= half) return;
+  layer_node(prev_, half, j, next_);
 }
 
 #if !defined(WHIR_FIELD_BABYBEAR)  // two lanes per permutation: KoalaBear lazy arithmetic (p2pair.h)

@@ -191,9 +191,12 @@ KB_HD uint32_t tw24(const uint32_t* lo, const uint32_t* hi, uint32_t e) {
   return mul(lo[e & 4095u], hi[(e >> 12) & 4095u]);
 }
 
+// Workgroup `blk` of one pass (ntt_kernel runs it as blockIdx.x; the
+// work-queue kernels of jobtable.h as the block of a dequeued item). `lds`
+// holds the pass's tile; the pass has one workgroup barrier (none for a
+// register-only pass).
 template <int LR1, int LR2, int V>
-__global__ void __launch_bounds__(256) ntt_kernel(NttArgs a) {
-  extern __shared__ uint32_t lds[];
+__device__ __forceinline__ void ntt_block(const NttArgs& a, uint32_t blk, uint32_t* lds) {
   constexpr int R1 = 1 << LR1, R2 = 1 << LR2, LL = LR1 + LR2;
   constexpr int LV = V == 4 ? 2 : V == 2 ? 1 : 0;
   const uint32_t QV = 1u << a.log_qv;
@@ -201,7 +204,7 @@ __global__ void __launch_bounds__(256) ntt_kernel(NttArgs a) {
   const uint32_t cq = tid & (QV - 1);
   const uint32_t jb = tid >> a.log_qv;  // stage A: j_b; stage B: first t_a
   const uint32_t log_lpb = a.log_m + a.log_b - LV;  // lines per block
-  const uint32_t line = (blockIdx.x << a.log_qv) + cq;
+  const uint32_t line = (blk << a.log_qv) + cq;
   const uint32_t b = line >> log_lpb;
   const uint32_t w = (line & ((1u << log_lpb) - 1)) << LV;  // m B + col
   const uint32_t m = w >> a.log_b;
@@ -310,6 +313,12 @@ __global__ void __launch_bounds__(256) ntt_kernel(NttArgs a) {
       vstore<V>(p, z[tb]);
     }
   }
+}
+
+template <int LR1, int LR2, int V>
+__global__ void __launch_bounds__(256) ntt_kernel(NttArgs a) {
+  extern __shared__ uint32_t lds[];
+  ntt_block<LR1, LR2, V>(a, blockIdx.x, lds);
 }
 
 // ---- passes of L = R1 R2 R3 rows: three register stages, two LDS exchanges ----

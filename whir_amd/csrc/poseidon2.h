@@ -89,6 +89,21 @@ KB_HD uint32_t canon2(uint32_t t) {  // [0, 2p) -> [0, p)
   return t < u ? t : u;
 }
 
+// The same reference, passed through an empty asm statement: loads through it
+// cannot be hoisted out of a loop around the caller. A persistent kernel that
+// runs one permutation per loop iteration uses it for the round constants,
+// which LLVM would otherwise load once before the loop and keep in more
+// scalar registers than there are (spilling them).
+// The pointer stays in the constant address space (the reference is to
+// __constant__ memory), so the loads remain scalar loads.
+template <class T>
+__device__ __forceinline__ const T& opaque_ref(const T& r) {
+  typedef const __attribute__((address_space(4))) T CT;
+  CT* q = (CT*)&r;
+  asm volatile("" : "+s"(q));
+  return *(const T*)q;
+}
+
 }  // namespace kb
 
 #if defined(WHIR_FIELD_BABYBEAR)
@@ -417,9 +432,10 @@ KB_HD void permute_lazy(uint32_t (&s)[W], const Poseidon2Lazy<W>& C) {
     if ((OUT >> i) & 1u) s[i] = canon2(mulr(s[i], C.fix));
 }
 
-template <int W, uint32_t OUT = (1u << W) - 1u>
+template <int W, uint32_t OUT = (1u << W) - 1u, bool OPAQUE = false>
 __device__ __forceinline__ void permute(uint32_t (&s)[W]) {
-  permute_lazy<W, OUT>(s, lazy_consts<W>());
+  if constexpr (OPAQUE) permute_lazy<W, OUT>(s, opaque_ref(lazy_consts<W>()));
+  else permute_lazy<W, OUT>(s, lazy_consts<W>());
 }
 
 // Host: the constants of permute_lazy from the plain round constants
@@ -476,6 +492,7 @@ inline void lazy_consts_build(const uint32_t (&ext_init)[4][W], const uint32_t (
 }
 
 // TruncatedPermutation<P16, 2, 8, 16>: first 8 words of P16(l || r).
+template <bool OPAQUE = false>
 __device__ __forceinline__ void compress(const uint32_t (&l)[8], const uint32_t (&r)[8], uint32_t (&out)[8]) {
   uint32_t s[16];
 #pragma unroll
@@ -483,7 +500,7 @@ __device__ __forceinline__ void compress(const uint32_t (&l)[8], const uint32_t 
     s[i] = l[i];
     s[8 + i] = r[i];
   }
-  permute<16, 0xFFu>(s);
+  permute<16, 0xFFu, OPAQUE>(s);
 #pragma unroll
   for (int i = 0; i < 8; ++i) out[i] = s[i];
 }

@@ -108,12 +108,14 @@ KB_HD void permute_plain(uint32_t (&s)[W], const Poseidon2Plain<W>& C) {
 
 // The same interface as the KoalaBear build's lazily reduced permutation:
 // every output word is the reference's (OUT is accepted and ignored).
-template <int W, uint32_t OUT = (1u << W) - 1u>
+template <int W, uint32_t OUT = (1u << W) - 1u, bool OPAQUE = false>
 __device__ __forceinline__ void permute(uint32_t (&s)[W]) {
-  permute_plain<W>(s, plain_consts<W>());
+  if constexpr (OPAQUE) permute_plain<W>(s, opaque_ref(plain_consts<W>()));
+  else permute_plain<W>(s, plain_consts<W>());
 }
 
 // TruncatedPermutation<P16, 2, 8, 16>: first 8 words of P16(l || r).
+template <bool OPAQUE = false>
 __device__ __forceinline__ void compress(const uint32_t (&l)[8], const uint32_t (&r)[8], uint32_t (&out)[8]) {
   uint32_t s[16];
 #pragma unroll
@@ -121,7 +123,7 @@ __device__ __forceinline__ void compress(const uint32_t (&l)[8], const uint32_t 
     s[i] = l[i];
     s[8 + i] = r[i];
   }
-  permute<16>(s);
+  permute<16, (1u << 16) - 1u, OPAQUE>(s);
 #pragma unroll
   for (int i = 0; i < 8; ++i) out[i] = s[i];
 }

@@ -369,16 +369,26 @@ static int leaves_enqueue(const uint32_t* rows, uint64_t n_rows, uint32_t row_wo
 // With `top` set, the last cooperative launch (the top <= COOP_MAX_NODES nodes
 // down to the root) is left to the caller: *top / *top_n = that layer and its
 // node count (tree_top_claim_kernel runs it with the transcript step).
+// With `done_levels` >= 0 the leaves and that many wide layers are already in
+// `layers` (the commit work queue of a set built them), and the launches start
+// at the next layer.
 static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, uint32_t* layers, hipStream_t s,
                           const MerkleConfig& mc, hipEvent_t* leaf_ev = nullptr, uint32_t** top = nullptr,
-                          uint32_t* top_n = nullptr, uint32_t jobs = 1) {
+                          uint32_t* top_n = nullptr, uint32_t jobs = 1, int done_levels = -1) {
   const uint64_t H = 1ull << log_h;
-  if (leaf_ev) HIP_TRY(hipEventRecord(leaf_ev[0], s));
-  int rc = leaves_enqueue(rows, H, row_words, layers, s, mc, jobs);
-  if (rc) return rc;
-  if (leaf_ev) HIP_TRY(hipEventRecord(leaf_ev[1], s));
   uint32_t* cur = layers;
   uint64_t n = H;
+  if (done_levels < 0) {
+    if (leaf_ev) HIP_TRY(hipEventRecord(leaf_ev[0], s));
+    int rc = leaves_enqueue(rows, H, row_words, layers, s, mc, jobs);
+    if (rc) return rc;
+    if (leaf_ev) HIP_TRY(hipEventRecord(leaf_ev[1], s));
+  } else {
+    for (int l = 0; l < done_levels; ++l) {
+      cur += n * 8;
+      n /= 2;
+    }
+  }
   while (n > std::max<uint64_t>(mc.coop_from, COOP_MAX_NODES)) {
     const uint64_t half = n / 2;
 #if !defined(WHIR_FIELD_BABYBEAR)
@@ -409,6 +419,14 @@ static int merkle_enqueue(const uint32_t* rows, int log_h, uint32_t row_words, u
     *top_n = (uint32_t)n;
   }
   return WHIR_OK;
+}
+
+// Layers merkle_enqueue computes with one or two lanes per compression before
+// its cooperative stages (the ones a commit work queue may take over).
+static int merkle_wide_levels(int log_h, const MerkleConfig& mc) {
+  int l = 0;
+  for (uint64_t n = 1ull << log_h; n > std::max<uint64_t>(mc.coop_from, COOP_MAX_NODES); n /= 2) ++l;
+  return l;
 }
 
 extern "C" int whir_merkle_commit(whir_ctx* c, const uint32_t* d_rows, uint32_t log_h, uint32_t row_words,
@@ -535,6 +553,9 @@ struct whir_prover {
   hipStream_t last_stream = nullptr;
   bool graph_mode = false;
   bool checks = false;  // device self-checks (checks.h)
+  // >= 0: the base LDE, the leaves and this many wide layers of the base tree
+  // come from a set's commit work queue (jobtable.h), not from this sequence
+  int ext_commit_levels = -1;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -1180,15 +1201,19 @@ static int enqueue_commit(whir_prover* P, const uint32_t* poly, hipStream_t s, c
   int rc;
   if (import) hipLaunchKernelGGL(tx_import_kernel, zg(dim3(1), P->jobs), dim3(64), 0, s, P->tx, P->js, *import);
   const int log_h0 = k + pl.rate - f;
-  if (P->timers) HIP_TRY(hipEventRecord(P->ev[WHIR_TIMER_DFT_BASE][0], s));
-  if ((rc = ntt_enqueue(c, poly, 1ull << (k - f), log_h0, f, P->cw0, P->ntt_tmp, s, nullptr, P->jobs))) return rc;
-  if (P->timers) HIP_TRY(hipEventRecord(P->ev[WHIR_TIMER_DFT_BASE][1], s));
+  const bool ext = P->ext_commit_levels >= 0;
+  const bool timers = P->timers && !ext;
+  if (timers) HIP_TRY(hipEventRecord(P->ev[WHIR_TIMER_DFT_BASE][0], s));
+  if (!ext && (rc = ntt_enqueue(c, poly, 1ull << (k - f), log_h0, f, P->cw0, P->ntt_tmp, s, nullptr, P->jobs)))
+    return rc;
+  if (timers) HIP_TRY(hipEventRecord(P->ev[WHIR_TIMER_DFT_BASE][1], s));
   uint32_t* top = nullptr;
   uint32_t top_n = 0;
   if ((rc = merkle_enqueue(P->cw0, log_h0, W, P->tree0, s, merkle_config(c, log_h0, W, P->jobs),
-                           P->timers ? P->ev[WHIR_TIMER_LEAVES_BASE] : nullptr, &top, &top_n, P->jobs)))
+                           timers ? P->ev[WHIR_TIMER_LEAVES_BASE] : nullptr, &top, &top_n, P->jobs,
+                           P->ext_commit_levels)))
     return rc;
-  P->timer_pending = P->timers;
+  P->timer_pending = timers;
   // the tree's top levels, transcript.write(root) and the first group's draw
   hipLaunchKernelGGL(tree_top_claim_kernel, zg(dim3(1), P->jobs), dim3(SERIAL_THREADS), 0, s, top, top_n, P->tx, P->log, P->js,
                      first, import ? 0u : (uint32_t)pl.log_words);
