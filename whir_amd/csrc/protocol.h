@@ -51,8 +51,9 @@ struct QueryArrays {
 
 __device__ __forceinline__ bool is_wave0() { return threadIdx.x < 64; }
 
-__device__ __forceinline__ void tx_begin(WaveTx& tx, DevTx* t, uint32_t* log, JobScalars* js) {
-  tx.load(t, log);
+__device__ __forceinline__ void tx_begin(WaveTx& tx, DevTx* t, uint32_t* log, JobScalars* js,
+                                         const CoopLane* pre = nullptr) {
+  tx.load(t, log, pre);
   const uint32_t pend = js->pending_words;
   tx.log_words += pend;
   if (tx.log_words > tx.log_cap) tx.overflow = 1;
@@ -165,15 +166,13 @@ __global__ void __launch_bounds__(SERIAL_THREADS) tree_top_claim_kernel(JP<uint3
   JobScalars* js = js_;
   __shared__ uint32_t s_node[COOP_MAX_NODES * 8];
   __shared__ ClaimScratch sc;
-  {
-    CoopP16 K;
-    K.load();
-    coop_subtrees_body(layer, n0, 1, 0, K, s_node);
-  }
+  CoopP16 K;  // the tree's compressions and then the transcript's
+  K.load();
+  coop_subtrees_body(layer, n0, 1, 0, K, s_node);
   WaveTx tx;
   if (is_wave0()) {
-    if (fresh_cap) tx.fresh(fresh_cap, log);
-    else tx_begin(tx, t, log, js);
+    if (fresh_cap) tx.fresh(fresh_cap, log, &K);
+    else tx_begin(tx, t, log, js, &K);
     write_root_body(tx, s_node);
   }
   if (cb.slot) claim_begin_body(tx, cb, sc);

@@ -103,51 +103,85 @@ __device__ __noinline__ uint32_t permute16_coop_shared(uint32_t s, CoopLane L) {
 #else
 // Cooperative Poseidon2-16: lane i (mod 16) holds state word i, with the
 // lazy reduction and scaled state of permute_lazy (poseidon2.h). The per-lane
-// constants are loaded into registers once per kernel (CoopP16::load), so a
-// permutation reads no memory.
+// constants are loaded into registers once per kernel (CoopP16::load); the
+// wave-uniform ones (internal round constants, kappa, the final scale) are
+// scalar operands read from constant memory.
 struct CoopP16 {
-  uint32_t la[9];   // lin_add[k][lane]
-  uint32_t ia[20];  // int_add[r] on lane 0, 0 elsewhere
-  uint32_t d;       // V_lane as an integer; 2 on lane 0, applied to 2p - x0: V_0 = -2, and the small
-                    // product keeps lane 0's REDC near-canonical for the next S-box's square
-  uint32_t kappa[20];
-  uint32_t fix;
+  uint32_t la[9];  // lin_add[k][lane]
+  uint32_t d;      // V_lane as an integer; 2 on lane 0, applied to 2p - x0: V_0 = -2, and the small
+                   // product keeps lane 0's REDC near-canonical for the next S-box's square
+  uint32_t m0;     // 1 on lane 0 of the row, else 0: lane 0's internal round constant enters as m0 * int_add
 
   __device__ __forceinline__ void load() {
     const Poseidon2Lazy<16>& C = c_p2l.w16;
     const uint32_t i = threadIdx.x & 15;
 #pragma unroll
     for (int k = 0; k < 9; ++k) la[k] = C.lin_add[k][i];
-#pragma unroll
-    for (int r = 1; r < 20; ++r) ia[r] = i == 0 ? C.int_add[r] : 0u;
-    ia[0] = 0;
     uint32_t dv = 2;
 #pragma unroll
     for (int j = 1; j < 16; ++j) dv = i == (uint32_t)j ? Poseidon2Lazy<16>::diag(j) : dv;
     d = dv;
-#pragma unroll
-    for (int r = 0; r < 20; ++r) kappa[r] = C.kappa[r];
-    fix = C.fix;
+    m0 = i == 0 ? 1u : 0u;
   }
 };
+using CoopLane = CoopP16;
+
+// The cross-lane sums below are exact 64-bit integers: a DPP add of the low
+// words whose carry (vcc) enters the DPP add of the high words. Each asm
+// block starts with two wait states, which a DPP read of a VGPR that the
+// previous VALU instruction wrote needs; inside, one independent instruction
+// and one s_nop separate a write from its DPP read.
+#define KB_DPP_CTL " row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+
+// Sum of x (< 2^32) over the 16 lanes of each row, on every lane.
+__device__ __forceinline__ uint64_t row_sum64(uint32_t x) {
+  uint32_t lo, hi;
+  asm("s_nop 1\n"
+      "v_add_co_u32_dpp %0, vcc, %2, %2 row_ror:8" KB_DPP_CTL
+      "v_addc_co_u32_e64 %1, vcc, 0, 0, vcc\n"
+      "s_nop 0\n"
+      "v_add_co_u32_dpp %0, vcc, %0, %0 row_ror:4" KB_DPP_CTL
+      "v_addc_co_u32_dpp %1, vcc, %1, %1, vcc row_ror:4" KB_DPP_CTL
+      "s_nop 0\n"
+      "v_add_co_u32_dpp %0, vcc, %0, %0 row_ror:2" KB_DPP_CTL
+      "v_addc_co_u32_dpp %1, vcc, %1, %1, vcc row_ror:2" KB_DPP_CTL
+      "s_nop 0\n"
+      "v_add_co_u32_dpp %0, vcc, %0, %0 row_ror:1" KB_DPP_CTL
+      "v_addc_co_u32_dpp %1, vcc, %1, %1, vcc row_ror:1" KB_DPP_CTL
+      : "=&v"(lo), "=&v"(hi)
+      : "v"(x)
+      : "vcc");
+  return ((uint64_t)hi << 32) | lo;
+}
 
 // REDC(MDS-light(x) + add) across the 16 lanes of a row; x < 2p, result
-// < p + 2^7. Per chunk (quad) row a of M4 is S4 + x_a + 2 x_{a+1}; then every
-// lane adds the sum over the four chunks at its position (rotations by 4, 8).
-// The cross-lane sums run on the two 16-bit limbs of x (x & 0xffff, x >> 16) as
-// independent 32-bit DPP adds (every partial sum < 2^22): no carries and no
-// 64-bit moves, which shortens the dependency chain a single wavefront waits on.
+// < p + 2^7. Per chunk (quad) row a of M4 is S4 + x_a + 2 x_{a+1} (S4 the
+// quad's sum); then every lane adds the sum over the four chunks at its
+// position (rotations by 4 and 8).
 __device__ __forceinline__ uint32_t mds16_coop_lazy(uint32_t x, uint32_t add) {
-  const uint32_t a = x & 0xffffu, b = x >> 16;
-  const uint32_t an = dpp<DPP_QUAD_NEXT>(a), bn = dpp<DPP_QUAD_NEXT>(b);
-  uint32_t sa = a + dpp<DPP_QUAD_XOR1>(a), sb = b + dpp<DPP_QUAD_XOR1>(b);
-  sa += dpp<DPP_QUAD_XOR2>(sa);
-  sb += dpp<DPP_QUAD_XOR2>(sb);
-  const uint32_t ya = sa + (a + 2u * an), yb = sb + (b + 2u * bn);  // S4 + x_a + 2 x_{a+1}
-  uint32_t ua = ya + dpp<DPP_ROR(4)>(ya), ub = yb + dpp<DPP_ROR(4)>(yb);
-  ua += dpp<DPP_ROR(8)>(ua);
-  ub += dpp<DPP_ROR(8)>(ub);
-  return redc(mad64(yb + ub, 65536u, (uint64_t)(ya + ua + add)));  // y + column sums + add
+  uint32_t slo, shi, xn;
+  asm("s_nop 1\n"
+      "v_mov_b32_dpp %2, %3 quad_perm:[1,2,3,0]" KB_DPP_CTL
+      "v_add_co_u32_dpp %0, vcc, %3, %3 quad_perm:[1,0,3,2]" KB_DPP_CTL
+      "v_addc_co_u32_e64 %1, vcc, 0, 0, vcc\n"
+      "s_nop 0\n"
+      "v_add_co_u32_dpp %0, vcc, %0, %0 quad_perm:[2,3,0,1]" KB_DPP_CTL
+      "v_addc_co_u32_dpp %1, vcc, %1, %1, vcc quad_perm:[2,3,0,1]" KB_DPP_CTL
+      : "=&v"(slo), "=&v"(shi), "=&v"(xn)
+      : "v"(x)
+      : "vcc");
+  const uint64_t y = mad_k<2>(xn, mad_k<1>(x, ((uint64_t)shi << 32) | slo));  // S4 + x_a + 2 x_{a+1}
+  uint32_t ulo, uhi;
+  asm("s_nop 1\n"
+      "v_add_co_u32_dpp %0, vcc, %2, %2 row_ror:4" KB_DPP_CTL
+      "v_addc_co_u32_dpp %1, vcc, %3, %3, vcc row_ror:4" KB_DPP_CTL
+      "s_nop 0\n"
+      "v_add_co_u32_dpp %0, vcc, %0, %0 row_ror:8" KB_DPP_CTL
+      "v_addc_co_u32_dpp %1, vcc, %1, %1, vcc row_ror:8" KB_DPP_CTL
+      : "=&v"(ulo), "=&v"(uhi)
+      : "v"((uint32_t)y), "v"((uint32_t)(y >> 32))
+      : "vcc");
+  return redc(mad_k<1>(add, y + (((uint64_t)uhi << 32) | ulo)));  // y + column sums + add
 }
 
 // REDC(a k) for a constant k with kp = k * (-p^-1) mod 2^32: the Montgomery
@@ -158,36 +192,42 @@ __device__ __forceinline__ uint32_t mulr_k(uint32_t a, uint32_t k, uint32_t kp) 
 }
 
 // In: s = the lane's state word in [0, p). Out: the permuted word in [0, p).
-// Internal rounds: lane 0's S-box (times kappa_r), then the row sum on 16-bit
-// limbs, then x_i <- REDC(sum + V_i x_i + c) as in internal_lazy.
+// Internal rounds: lane 0's S-box (times kappa_r), then the row sum, then
+// x_i <- REDC(sum + V_i x_i + c) as in internal_lazy (c on lane 0 only).
+// Measured on one wavefront (100 chained calls): 1.84 us per permutation
+// against 1.96 us with 16-bit limb sums. One wavefront pays about one issue
+// slot (~4 cycles) per instruction, s_nop included, and ~7 cycles per
+// instruction that waits on the one before; variants that took lanes 1..15's
+// sum off lane 0's S-box chain (row_newbcast of its result) or filled the DPP
+// wait states with the lane-0 select measured slower (2.10, 1.92 us).
 __device__ __forceinline__ void permute16_coop(uint32_t& s, const CoopP16& K) {
-  const bool lane0 = (threadIdx.x & 15) == 0;
+  const Poseidon2Lazy<16>& C = c_p2l.w16;
+  // the wave-uniform constants in one run of scalar loads
+  uint32_t kappa[20], iadd[20];
+#pragma unroll
+  for (int r = 0; r < 20; ++r) {
+    kappa[r] = C.kappa[r];
+    iadd[r] = C.int_add[r];
+  }
+  const uint32_t fix = C.fix;
+  const bool lane0 = K.m0 != 0;
   uint32_t x = mds16_coop_lazy(s, K.la[0]);
 #pragma unroll
   for (int r = 1; r < 5; ++r) x = mds16_coop_lazy(mulr(mulr(x, x), x), K.la[r]);
 #pragma unroll
   for (int r = 0; r < 20; ++r) {
-    const uint32_t c = mulr_k(mulr(mulr(x, x), x), K.kappa[r], K.kappa[r] * NPINV);
+    const uint32_t c = mulr_k(mulr(mulr(x, x), x), kappa[r], kappa[r] * NPINV);
     x = lane0 ? c : x;
-    uint32_t a = x & 0xffffu, b = x >> 16;
-    a += dpp<DPP_ROR(8)>(a);
-    b += dpp<DPP_ROR(8)>(b);
-    a += dpp<DPP_ROR(4)>(a);
-    b += dpp<DPP_ROR(4)>(b);
-    a += dpp<DPP_ROR(2)>(a);
-    b += dpp<DPP_ROR(2)>(b);
-    a += dpp<DPP_ROR(1)>(a);
-    b += dpp<DPP_ROR(1)>(b);
+    const uint64_t sum = row_sum64(x);
     const uint32_t y = lane0 ? 2u * P - x : x;
-    const uint32_t ia = r < 19 ? K.ia[r + 1] : K.la[5];
-    // sum = b 2^16 + a (a, b < 2^20); t = y d + sum + c < 2 p^2 + 2^37
-    const uint64_t t = mad64(y, K.d, mad64(b, 65536u, (uint64_t)(a + ia)));
+    const uint64_t s2 = r < 19 ? mad_u(K.m0, iadd[r + 1], sum) : mad_k<1>(K.la[5], sum);
+    const uint64_t t = mad64(y, K.d, s2);  // < 2 p^2 + 2^37
     x = r < 19 ? redc(t) : canon2(redc(t));
   }
 #pragma unroll
   for (int r = 6; r < 9; ++r) x = mds16_coop_lazy(mulr(mulr(x, x), x), K.la[r]);
   x = mds16_coop_lazy(mulr(mulr(x, x), x), 0u);
-  s = canon2(mulr(x, K.fix));
+  s = canon2(mulr(x, fix));
 }
 
 // The transcript's permutation, out of line. A transcript kernel is one
@@ -199,33 +239,8 @@ __device__ __forceinline__ void permute16_coop(uint32_t& s, const CoopP16& K) {
 // their instruction count). One shared copy is fetched once per kernel (and
 // stays in L2 between kernels). Its arguments are the lane's state word and
 // lane constants; the wave-uniform constants come from constant memory.
-struct CoopLane {
-  uint32_t la[9];  // lin_add[k][lane]
-  uint32_t d;      // V_lane as an integer; 2 on lane 0
-
-  __device__ __forceinline__ void load() {
-    CoopP16 K;
-    K.load();
-#pragma unroll
-    for (int k = 0; k < 9; ++k) la[k] = K.la[k];
-    d = K.d;
-  }
-};
-
 __device__ __noinline__ uint32_t permute16_coop_shared(uint32_t s, CoopLane L) {
-  const Poseidon2Lazy<16>& C = c_p2l.w16;
-  const bool lane0 = (threadIdx.x & 15) == 0;
-  CoopP16 K;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) K.la[k] = L.la[k];
-  K.d = L.d;
-  K.ia[0] = 0;
-#pragma unroll
-  for (int r = 1; r < 20; ++r) K.ia[r] = lane0 ? C.int_add[r] : 0u;
-#pragma unroll
-  for (int r = 0; r < 20; ++r) K.kappa[r] = C.kappa[r];
-  K.fix = C.fix;
-  permute16_coop(s, K);
+  permute16_coop(s, L);
   return s;
 }
 
@@ -247,7 +262,8 @@ struct WaveTx {
 
   // A fresh transcript: zero sponge state, empty buffers, an empty log of
   // `cap` words.
-  __device__ __forceinline__ void fresh(uint32_t cap, uint32_t* log_ptr) {
+  // pre: lane constants the kernel has loaded already (else loaded here)
+  __device__ __forceinline__ void fresh(uint32_t cap, uint32_t* log_ptr, const CoopLane* pre = nullptr) {
     s = 0u;
     pin = 0u;
     in_len = out_len = 0;
@@ -255,9 +271,10 @@ struct WaveTx {
     log_cap = cap;
     overflow = 0;
     log = log_ptr;
-    k.load();
+    if (pre) k = *pre;
+    else k.load();
   }
-  __device__ __forceinline__ void load(const DevTx* t, uint32_t* log_ptr) {
+  __device__ __forceinline__ void load(const DevTx* t, uint32_t* log_ptr, const CoopLane* pre = nullptr) {
     const int lane = threadIdx.x & 63;
     s = t->state[lane & 15];
     pin = t->in_buf[lane & 7];
@@ -267,7 +284,8 @@ struct WaveTx {
     log_cap = t->log_cap;
     overflow = t->overflow;
     log = log_ptr;
-    k.load();
+    if (pre) k = *pre;
+    else k.load();
   }
   __device__ __forceinline__ void store(DevTx* t) {
     const int lane = threadIdx.x & 63;
