@@ -77,8 +77,10 @@ int whir_ctx_set_ntt_shape(whir_ctx* ctx, int first_pass_log, int log_vec);
 
 /* NTT pass count of this context (no counterpart in the reference): with
  * two_pass = 1 a transform of 2^17..2^20 rows runs in two passes of up to 1024
- * rows (three register stages each), with 0 in three passes of <= 256 rows.
- * Results do not depend on it. Default 0. */
+ * rows (three register stages each), with 2 a transform of 2^18..2^20 rows of
+ * at least 16 columns in two passes of 512 or 1024 rows (two register stages of
+ * up to 32 points each, 256-thread workgroups), with 0 in three passes of
+ * <= 256 rows. Results do not depend on it. Default 0. */
 int whir_ctx_set_ntt_passes(whir_ctx* ctx, int two_pass);
 
 /* Merkle tree shape of this context (no counterpart in the reference, whose

@@ -1,7 +1,8 @@
 """The autotuner (whir_ctx_set_autotune): every candidate it may choose gives the
 same words, and proofs on an autotuned context are the oracle's bytes.
 
-* NTT candidates: two or three passes x first-pass radix x workgroup size x
+* NTT candidates: two passes (three or two register stages each) or three
+  passes x first-pass radix x workgroup size x
   vector width (the space ntt_candidates enumerates), forced through the
   context knobs, each DFT equal to the oracle's (Radix2DFTSmallBatch
   semantics, crates/whir/src/whir.rs:34,59);
@@ -33,7 +34,7 @@ def test_ntt_candidates_identical(oracle, n_in, log_h, width):
     want = oracle.dft_batch(padded)
     ctx = whir_amd.Context(0, 1000)
     try:
-        for two in (0, 1):
+        for two in (0, 1, 2):
             for first in (4, 3, 0):
                 for lt in (8, 7):
                     for lv in (2, 1):

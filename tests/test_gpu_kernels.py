@@ -76,13 +76,16 @@ def test_dft_batch(gpu_ctx, oracle, n_in, log_h, width):
 NTT_LAYOUTS = [(5, 8, 4, 2), (5, 8, 3, 2), (5, 7, 2, 1), (4, 6, 4, 0), (8, 8, 4, 2), (6, 8, 0, 1)]
 LAYOUT_CASES = [(1 << 12, 13, 16), (1 << 11, 13, 64), (1 << 13, 14, 8), (1 << 10, 12, 4), (1 << 17, 18, 16)]
 
-# two passes of up to 1024 rows (three register stages per pass) against three
-# passes, for the transform sizes where the choice exists (2^17..2^20 rows)
+# two passes of up to 1024 rows (1: three register stages per pass; 2: two
+# register stages of up to 32 points, 256-thread workgroups, for >= 16 columns)
+# against three passes, for the transform sizes where the choice exists
+# (2^17..2^20 rows)
 TWO_PASS_CASES = [(1 << 17, 18, 16), (1 << 18, 19, 4), ((1 << 18) - 5, 19, 16), (1 << 19, 21, 2), (1 << 20, 21, 16),
-                  (1 << 20, 20, 1), (1 << 17, 19, 64)]
+                  (1 << 20, 20, 1), (1 << 17, 19, 64), (1 << 19, 20, 16), (1 << 18, 20, 16), ((1 << 20) - 3, 21, 16),
+                  (1 << 19, 21, 64)]
 
 
-@pytest.mark.parametrize("two_pass", [0, 1])
+@pytest.mark.parametrize("two_pass", [0, 1, 2])
 def test_dft_two_pass_plan(oracle, two_pass):
     import whir_amd
     from whir_amd._lib import lib

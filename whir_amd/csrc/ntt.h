@@ -50,21 +50,21 @@ constexpr uint32_t cx_pow(uint32_t a, uint64_t e) {
   }
   return r;
 }
-struct Omega16 {
-  uint32_t w[16];
+struct Omega32 {
+  uint32_t w[32];
 };
-constexpr Omega16 make_omega16() {
-  Omega16 t{};
+constexpr Omega32 make_omega32() {
+  Omega32 t{};
   const uint32_t g = (uint32_t)(((uint64_t)GENERATOR << 32) % P);  // the generator in Montgomery form
-  const uint32_t w16 = cx_pow(g, (uint64_t)(P - 1) >> 4);          // two_adic_generator(4)
+  const uint32_t w32 = cx_pow(g, (uint64_t)(P - 1) >> 5);          // two_adic_generator(5)
   uint32_t x = ONE;
-  for (int i = 0; i < 16; ++i) {
+  for (int i = 0; i < 32; ++i) {
     t.w[i] = x;
-    x = cx_mul(x, w16);
+    x = cx_mul(x, w32);
   }
   return t;
 }
-constexpr Omega16 kOmega16 = make_omega16();
+constexpr Omega32 kOmega32 = make_omega32();
 
 template <int V>
 struct Vec {
@@ -116,9 +116,9 @@ constexpr bool dft_lazy(int LR, int s, int i, uint32_t lazy_out) {
   return (i % (2 * half)) >= half && (i % half) != 0;
 }
 
-// x <- DFT_R(x) (natural order in and out, R = 2^LR <= 16): radix-2 DIT on the
+// x <- DFT_R(x) (natural order in and out, R = 2^LR <= 32): radix-2 DIT on the
 // bit-reversed registers; twiddles w_{2^(s+1)}^j are compile-time constants and
-// j = 0 multiplies are skipped (17 multiplies for R = 16). Inputs are in
+// j = 0 multiplies are skipped (17 multiplies for R = 16, 49 for 32). Inputs are in
 // [0, p). A sum or difference that is next only multiplied by a twiddle is
 // left unreduced in [0, 2p) (u + t, or u - t + p): the Montgomery product of
 // a factor < 2p and a twiddle < p is below p 2^32 and comes out in [0, p), so
@@ -128,6 +128,7 @@ constexpr bool dft_lazy(int LR, int s, int i, uint32_t lazy_out) {
 template <int LR, int V, uint32_t LAZY_OUT = 0>
 KB_HD void dft_regs(Vec<V> (&x)[1 << LR]) {
   constexpr int R = 1 << LR;
+  static_assert(LR >= 0 && LR <= 5, "register DFT of up to 32 points");
   Vec<V> y[R];
 #pragma unroll
   for (int i = 0; i < R; ++i) y[i] = x[bitrev(i, LR)];
@@ -139,7 +140,7 @@ KB_HD void dft_regs(Vec<V> (&x)[1 << LR]) {
 #pragma unroll
       for (int j = 0; j < half; ++j) {
         const int i0 = g + j, i1 = g + j + half;
-        const uint32_t w = kOmega16.w[j << (3 - s)];
+        const uint32_t w = kOmega32.w[j << (4 - s)];  // w_{2^(s+1)}^j
         const bool lazy0 = dft_lazy(LR, s, i0, LAZY_OUT), lazy1 = dft_lazy(LR, s, i1, LAZY_OUT);
 #pragma unroll
         for (int c = 0; c < V; ++c) {
@@ -319,6 +320,35 @@ template <int LR1, int LR2, int V>
 __global__ void __launch_bounds__(256) ntt_kernel(NttArgs a) {
   extern __shared__ uint32_t lds[];
   ntt_block<LR1, LR2, V>(a, blockIdx.x, lds);
+}
+
+// Workgroup i of a grid of n (n a multiple of 16) takes tile
+// 16 (i / 16) + 2 (i mod 8) + (i / 8 mod 2): tiles 2t and 2t + 1 go to launch
+// slots 8 apart. Workgroups are dealt to the 8 XCDs in turn, so the two run on
+// one XCD at about the same time and the second finds in that XCD's L2 the
+// 128-byte lines the first brought in. Other grids keep the identity.
+__host__ __device__ __forceinline__ uint32_t xcd_pair_tile(uint32_t i, uint32_t n) {
+  if (n & 15u) return i;
+  return (i & ~15u) | ((i & 7u) << 1) | ((i >> 3) & 1u);
+}
+
+// Passes of 512 or 1024 rows in two register stages of up to 32 points
+// (ntt_block with 2^LR1 x 2^LR2, LR1 = 5), 256 threads, two words per lane and
+// line: a workgroup's tile is 1024 rows x 16 columns (64 KiB of LDS, two
+// workgroups per CU) or 512 rows x 32 columns. A 1024-row tile covers half of
+// each 128-byte row of a 32-column work array; the tile of the other half runs
+// beside it on the same XCD (xcd_pair_tile), so each line is fetched from HBM
+// once (FETCH_SIZE of a pass = its 128 MiB). The base LDE of a 2^24-coefficient
+// commitment (2^20 x 16 -> 2^21 rows) in two such passes measured 201 us
+// against 189 us for the three-pass plan (50 launches, host-timed; one word
+// per lane: 235 us, 74.6 VALU instructions per element per pass against 48.3
+// for an 8-bit pass): the passes save HBM traffic (2.3x the algorithmic bytes
+// instead of 3.7x) but not VALU work, which bounds the transform, and their
+// tiles leave two waves per SIMD. It stays a tuner candidate.
+template <int LR1, int LR2, int V>
+__global__ void __launch_bounds__(256) ntt_wide_kernel(NttArgs a) {
+  extern __shared__ uint32_t lds[];
+  ntt_block<LR1, LR2, V>(a, xcd_pair_tile(blockIdx.x, gridDim.x), lds);
 }
 
 // ---- passes of L = R1 R2 R3 rows: three register stages, two LDS exchanges ----

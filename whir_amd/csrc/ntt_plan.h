@@ -19,6 +19,7 @@ struct NttPlan {
   int log_nn = 0, log_c = 0, log_n = 0, log_b = 0, lv = 0, P = 0;
   int logl[3] = {0, 0, 0};
   bool tri[3] = {false, false, false};  // pass p is an ntt3_kernel pass (logl[p] = 9 or 10)
+  bool wide[3] = {false, false, false}; // pass p is an ntt_wide_kernel pass (logl[p] = 9 or 10)
   bool reg_first = false;  // pass 0 is ntt_kernel<logl[0], 0, V> (no LDS)
 };
 
@@ -35,7 +36,19 @@ inline bool ntt_plan(uint64_t n_in, int log_nn, int log_c, NttPlan& pl, int max_
   pl.log_b = log_c + (log_nn - log_n);
   pl.lv = std::min(std::min(2, max_lv), log_c);
   pl.reg_first = false;
-  if (two_pass && log_n >= 17 && log_n <= 20) {
+  pl.tri[0] = pl.tri[1] = pl.tri[2] = false;
+  pl.wide[0] = pl.wide[1] = pl.wide[2] = false;
+  if (two_pass == 2 && log_n >= 18 && log_n <= 20 && log_c >= 4) {
+    // two passes of 9-10 bits, each two register stages of up to 32 points,
+    // two words per lane and line (ntt_wide_kernel)
+    pl.P = 2;
+    pl.lv = 1;
+    pl.logl[0] = (log_n + 1) / 2;
+    pl.logl[1] = log_n / 2;
+    pl.wide[0] = pl.wide[1] = true;
+    return true;
+  }
+  if (two_pass == 1 && log_n >= 17 && log_n <= 20) {
     pl.P = 2;
     pl.logl[0] = (log_n + 1) / 2;
     pl.logl[1] = log_n / 2;
@@ -64,7 +77,9 @@ inline bool ntt_plan(uint64_t n_in, int log_nn, int log_c, NttPlan& pl, int max_
   return true;
 }
 
-enum { NTT_PASS_LDS = 0, NTT_PASS_REG = 1, NTT_PASS_TRI = 2 };
+enum { NTT_PASS_LDS = 0, NTT_PASS_REG = 1, NTT_PASS_TRI = 2, NTT_PASS_WIDE = 3 };
+constexpr int NTT_WIDE_LR1 = 5;  // first register stage of an ntt_wide_kernel pass
+constexpr int NTT_WIDE_LT = 8;   // log2 threads of an ntt_wide_kernel workgroup
 
 struct NttPass {
   NttArgs a;
@@ -80,8 +95,14 @@ inline void ntt_pass_setup(const NttPlan& pl, int p, uint64_t n_in, int log_thre
   int used = 0;
   for (int q = 0; q <= p; ++q) used += pl.logl[q];
   const int l = pl.logl[p];
-  ps.kind = (pl.reg_first && p == 0) ? NTT_PASS_REG : pl.tri[p] ? NTT_PASS_TRI : NTT_PASS_LDS;
-  const int lr2 = ps.kind == NTT_PASS_REG ? 0 : ps.kind == NTT_PASS_TRI ? NTT3_LT : l / 2;  // log2 threads per line
+  ps.kind = (pl.reg_first && p == 0) ? NTT_PASS_REG
+            : pl.tri[p]                ? NTT_PASS_TRI
+            : pl.wide[p]               ? NTT_PASS_WIDE
+                                       : NTT_PASS_LDS;
+  const int lr2 = ps.kind == NTT_PASS_REG    ? 0
+                  : ps.kind == NTT_PASS_TRI  ? NTT3_LT
+                  : ps.kind == NTT_PASS_WIDE ? l - NTT_WIDE_LR1
+                                             : l / 2;  // log2 threads per line
   NttArgs& a = ps.a;
   memset(&a, 0, sizeof(a));
   a.tw_lo = tw_lo;
@@ -93,7 +114,9 @@ inline void ntt_pass_setup(const NttPlan& pl, int p, uint64_t n_in, int log_thre
   a.log_n = (uint32_t)pl.log_n;
   a.log_m = (uint32_t)(pl.log_n - used);
   const int log_lines = pl.log_n + pl.log_b - pl.lv - l;
-  const int lt = ps.kind == NTT_PASS_TRI ? log_threads : std::min(log_threads, 8);  // ntt_kernel: <= 256 threads
+  const int lt = ps.kind == NTT_PASS_TRI    ? log_threads
+                 : ps.kind == NTT_PASS_WIDE ? NTT_WIDE_LT
+                                            : std::min(log_threads, 8);  // ntt_kernel: <= 256 threads
   a.log_qv = (uint32_t)std::max(0, std::min(lt - lr2, log_lines));
   a.first = p == 0;
   a.twiddle_out = p + 1 < pl.P;

@@ -52,7 +52,9 @@ static __device__ __align__(256) unsigned char scratch[WHIR_NOALLOC_SCRATCH_BYTE
 // 2^log_threads threads per workgroup; the first pass of a transform of >= 3
 // passes as an LDS-free radix-2^first_log register pass (0: all passes even);
 // 2^max_lv words per lane and line; transforms of 2^17..2^20 rows in two passes
-// of up to 1024 rows (ntt3_kernel) when two_pass is set.
+// of up to 1024 rows with three register stages (ntt3_kernel) when two_pass is
+// 1, transforms of 2^18..2^20 rows in two passes of two register stages of up
+// to 32 points (ntt_wide_kernel) when it is 2.
 struct NttConfig {
   int max_logl = 8, log_threads = 8, first_log = 4, max_lv = 1, two_pass = 0;
 };
@@ -230,7 +232,7 @@ extern "C" int whir_ctx_set_ntt_tuning(whir_ctx* c, int max_log_pass, int log_th
 }
 
 extern "C" int whir_ctx_set_ntt_passes(whir_ctx* c, int two_pass) {
-  if (!c || two_pass < 0 || two_pass > 1) return WHIR_EINVAL;
+  if (!c || two_pass < 0 || two_pass > 2) return WHIR_EINVAL;
   c->ntt.two_pass = two_pass;
   return WHIR_OK;
 }
@@ -274,6 +276,15 @@ static NttKernelFn ntt3_kernel_for(int logl) {
 }
 
 template <int V>
+static NttKernelFn ntt_wide_kernel_for(int logl) {
+  switch (logl) {
+    case 9: return ntt_wide_kernel<NTT_WIDE_LR1, 4, V>;
+    case 10: return ntt_wide_kernel<NTT_WIDE_LR1, 5, V>;
+    default: return nullptr;
+  }
+}
+
+template <int V>
 static NttKernelFn ntt_kernel_for(int logl) {
   switch (logl) {
     case 1: return ntt_kernel<1, 0, V>;
@@ -307,6 +318,7 @@ static int ntt_enqueue(whir_ctx* c, const uint32_t* in, uint64_t n_in, int log_n
     NttKernelFn fn = ps.kind == NTT_PASS_REG
                          ? (pl.lv == 2 ? ntt_reg_kernel_for<4>(l) : pl.lv == 1 ? ntt_reg_kernel_for<2>(l)
                                                                                : ntt_reg_kernel_for<1>(l))
+                     : ps.kind == NTT_PASS_WIDE ? (pl.lv == 1 ? ntt_wide_kernel_for<2>(l) : nullptr)
                      : ps.kind == NTT_PASS_TRI
                          ? (pl.lv == 2 ? ntt3_kernel_for<4>(l) : pl.lv == 1 ? ntt3_kernel_for<2>(l)
                                                                             : ntt3_kernel_for<1>(l))
@@ -899,7 +911,7 @@ static double time_min_ms(hipStream_t s, const std::function<int()>& run, int re
 static std::vector<NttConfig> ntt_candidates(whir_ctx* c, uint64_t n_in, int log_nn, int log_c) {
   std::vector<NttConfig> out;
   std::vector<std::string> seen;
-  for (int two : {0, 1})
+  for (int two : {0, 1, 2})
     for (int fl : {4, 3, 0})
       for (int lt : {8, 7, 9})  // 9: 512-thread three-stage passes (others are capped at 256 threads)
         for (int lv : {2, 1}) {
