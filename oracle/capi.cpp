@@ -47,6 +47,10 @@ void oracle_rng_u32(uint64_t seed, size_t n, uint32_t* out) {
   StdRng r = StdRng::seed_from_u64(seed);
   for (size_t i = 0; i < n; ++i) out[i] = r.next_u32();
 }
+void oracle_stdrng_key(uint64_t seed, uint32_t* out) {  // the ChaCha key seed_from_u64 derives
+  StdRng r = StdRng::seed_from_u64(seed);
+  memcpy(out, r.key, 32);
+}
 void oracle_rand_poly(uint64_t seed, int k, uint32_t* out) {
   std::vector<F> v = rand_poly(seed, k);
   memcpy(out, v.data(), v.size() * 4);

@@ -70,6 +70,7 @@ def lib():
         L.oracle_two_adic_generator.argtypes = [ctypes.c_uint]
         L.oracle_chacha_block.argtypes = [u32p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, u32p]
         L.oracle_rng_u32.argtypes = [ctypes.c_uint64, ctypes.c_size_t, u32p]
+        L.oracle_stdrng_key.argtypes = [ctypes.c_uint64, u32p]
         L.oracle_rand_poly.argtypes = [ctypes.c_uint64, ctypes.c_int, u32p]
         L.oracle_rand_ext.argtypes = [ctypes.c_uint64, ctypes.c_size_t, u32p]
         L.oracle_poseidon2_constants.argtypes = [ctypes.c_int, ctypes.c_uint64, u32p, u32p, u32p, u32p]
@@ -172,6 +173,13 @@ def chacha_block(key, counter, stream, rounds):
 def rng_u32(seed, n):
     out = np.zeros(n, np.uint32)
     lib().oracle_rng_u32(seed, n, _p(out))
+    return out
+
+
+def stdrng_key(seed):
+    """The 8-word ChaCha key StdRng::seed_from_u64(seed) derives (PCG32 expansion)."""
+    out = np.zeros(8, np.uint32)
+    lib().oracle_stdrng_key(seed, _p(out))
     return out
 
 

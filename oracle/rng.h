@@ -10,10 +10,12 @@
 //   * Sampling a field element (Plonky3 StandardUniform for MontyField31):
 //     loop { v = next_u32() >> 1; if v < p return from_monty(v) }.
 // Pinning: the ChaCha block function (20-round form) is checked against the
-// RFC 8439 / RFC 7539 known-answer vectors in tests/test_oracle_rng.py. The
-// 12-round variant, the PCG32 expansion and the sampling loop are restated from
-// the crates' documented algorithms: parity unpinned (no reference vectors exist
-// and the crates are absent).
+// RFC 8439 / RFC 7539 known-answer vectors, and the PCG32 multiplier and XSH-RR
+// output function against the PCG reference implementation's demo outputs
+// (seed 42, sequence 54), in tests/test_oracle_rng.py. The 12-round variant,
+// seed_from_u64's increment and step-before-output order, and the sampling loop
+// are restated from the crates' documented algorithms: parity unpinned (no
+// reference vectors exist and the crates are absent).
 #pragma once
 #include <cstdint>
 #include "field.h"

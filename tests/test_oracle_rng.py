@@ -96,6 +96,48 @@ def test_stdrng_seed_from_u64_stream(oracle):
         assert [int(x) for x in oracle.rng_u32(seed, 100)] == stdrng_words(seed, 100)
 
 
+# pcg32_random_r of the PCG reference implementation (pcg-random.org, the
+# pcg32-demo program) seeded with pcg32_srandom_r(42, 54): its published first
+# outputs. PCG32 outputs XSH-RR of the state *before* each step; rand_core's
+# seed_from_u64 steps first and outputs XSH-RR of the new state, with its own
+# increment. The demo's vectors pin the multiplier and the XSH-RR output
+# function that both share.
+PCG32_DEMO_42_54 = [0xA15C02B7, 0x7B47F409, 0xBA1D3330, 0x83D2F293, 0xBFA4784B, 0xCBED606E]
+PCG_MUL = 6364136223846793005
+M64 = (1 << 64) - 1
+
+
+def xsh_rr(st):
+    xs = (((st >> 18) ^ st) >> 27) & MASK
+    rot = st >> 59
+    return ((xs >> rot) | (xs << ((32 - rot) & 31))) & MASK
+
+
+def pcg32_demo_states(initstate, initseq, n):
+    inc = ((initseq << 1) | 1) & M64
+    st = (0 * PCG_MUL + inc) & M64  # pcg32_srandom_r: state 0, one step, add initstate, one step
+    st = (st + initstate) & M64
+    st = (st * PCG_MUL + inc) & M64
+    out = []
+    for _ in range(n):
+        out.append(st)  # the state whose XSH-RR is the next output
+        st = (st * PCG_MUL + inc) & M64
+    return out
+
+
+def test_pcg32_known_answers(oracle):
+    states = pcg32_demo_states(42, 54, len(PCG32_DEMO_42_54))
+    assert [xsh_rr(s) for s in states] == PCG32_DEMO_42_54
+    # the oracle's seed_from_u64 takes key word 0 = XSH-RR(seed * MUL + INC):
+    # for each demo state s, the seed with seed * MUL + INC = s gives the demo's output
+    inc = 11634580027462260723
+    mul_inv = pow(PCG_MUL, -1, 1 << 64)
+    for s, want in zip(states, PCG32_DEMO_42_54):
+        seed = ((s - inc) * mul_inv) & M64
+        assert int(oracle.stdrng_key(seed)[0]) == want
+        assert [int(x) for x in oracle.stdrng_key(seed)] == pcg32_key(seed)
+
+
 def test_field_sampling_rejection(oracle):
     words = stdrng_words(1, 5000)
     want, i = [], 0
