@@ -122,7 +122,10 @@ int whir_dft_batch(whir_ctx* ctx, const uint32_t* d_in, uint64_t n_in_rows, uint
 int whir_merkle_commit(whir_ctx* ctx, const uint32_t* d_rows, uint32_t log_h, uint32_t row_words,
                        uint32_t* d_layers, uint32_t* root_host, void* stream);
 
-/* Poseidon2 permutation of n independent states (width 16 or 24), in place. */
+/* Poseidon2 permutation of n independent states (width 16 or 24), in place.
+ * width -16 / -64: width 16 in the cooperative forms, one wavefront per state
+ * (-16: 16-lane rows, as in the Merkle tree tops; -64: the device
+ * transcript's one-wavefront form). */
 int whir_poseidon2_permute(whir_ctx* ctx, int width, uint32_t* d_states, uint64_t n, void* stream);
 
 /* Replaces MerkleTree::query / MerkleTreeExt::query (crates/merkle/src/lib.rs:30-37,

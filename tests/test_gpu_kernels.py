@@ -39,6 +39,22 @@ def test_poseidon2_cooperative16(gpu_ctx, oracle):
     assert np.array_equal(got, want)
 
 
+def test_poseidon2_wave16(gpu_ctx, oracle):
+    """The transcript's one-wavefront P16 (linearised internal rounds), on
+    random states and on states with words near p and 0 (the accumulators'
+    reduction bounds)."""
+    import whir_amd
+
+    rng = np.random.default_rng(11)
+    states = _rand_field(rng, (96, 16))
+    states[64:80] = P - 1 - rng.integers(0, 4, (16, 16), dtype=np.uint32)
+    states[80:96] = rng.integers(0, 4, (16, 16), dtype=np.uint32)
+    print("poseidon2 wave16", flush=True)
+    got = whir_amd.poseidon2_permute(states, 16, wave=True, ctx=gpu_ctx)
+    want = np.stack([oracle.permute(16, s) for s in states])
+    assert np.array_equal(got, want)
+
+
 DFT_CASES = [
     # (n_in rows, log_h, width)
     (1, 1, 1),

@@ -132,13 +132,15 @@ def merkle_commit(rows, ctx=None):
     return root, layers
 
 
-def poseidon2_permute(states, width, cooperative=False, ctx=None):
-    """Poseidon2-{16,24} of each row of `states` (n x width)."""
+def poseidon2_permute(states, width, cooperative=False, wave=False, ctx=None):
+    """Poseidon2-{16,24} of each row of `states` (n x width). Width 16 only:
+    `cooperative` runs the 16-lane row form of the Merkle tree tops, `wave`
+    the device transcript's one-wavefront form."""
     ctx = ctx or default_context()
     s = _u32(states).reshape(-1, width)
     d = DeviceBuffer(s.nbytes)
     d.upload(s)
-    w = -16 if cooperative else width
+    w = -64 if wave else -16 if cooperative else width
     check(ctx.L.whir_poseidon2_permute(ctx.ptr, w, d.ptr, s.shape[0], None), "whir_poseidon2_permute")
     check(ctx.L.whir_device_synchronize(), "sync")
     return d.download(np.uint32, s.size).reshape(s.shape)

@@ -170,6 +170,22 @@ struct Poseidon2LazyConsts {
 
 __constant__ Poseidon2LazyConsts c_p2l;
 
+// Per-lane coefficient columns of the one-wavefront P16 (permute16_wave in
+// tx.h), whose 20 internal rounds run as one chain of S-boxes over a linear
+// recurrence. With the state u entering the internal rounds, x_r the S-box
+// input of round r and y_r = x_r^3 its output, and A = M_I with column 0
+// zeroed, b = column 0 of M_I (M_I = 1 1^T + diag(V)):
+//   u_{r+1} = A u_r + b y_r,  x_r = e0^T A^r u + rc_r + sum_{j<r} e0^T A^{r-1-j} b y_j,
+//   u_20 = A^20 u + sum_j A^{19-j} b y_j.
+// Lane 16 + r (r = 1..19) accumulates R * (scaled x_r); lane i < 16 accumulates
+// R * (scaled u_20[i] + the first terminal round constant). Column layout,
+// lane-major: col[l][j] (j < 20) multiplies y_j, col[l][20 + k - 1] (k = 1..15)
+// multiplies input word k, col[l][35] is the accumulator's starting value.
+struct Poseidon2Wave16 {
+  uint32_t col[64][36];
+};
+__constant__ Poseidon2Wave16 c_p2w;
+
 template <int W>
 __device__ __forceinline__ const Poseidon2Lazy<W>& lazy_consts() {
   if constexpr (W == 16) return c_p2l.w16;
@@ -489,6 +505,58 @@ inline void lazy_consts_build(const uint32_t (&ext_init)[4][W], const uint32_t (
     a = mm(a, Rinv);
   }
   C.fix = mm(minv(a), R);
+}
+
+// Host: the columns of Poseidon2Wave16 for the lazy scales of
+// lazy_consts_build (sigma: the scale of the state entering the internal
+// rounds; a_t: the scale leaving them). Words are integers mod p: the
+// internal layer is the integer matrix M_I, the S-box w -> w^3 R^-2.
+inline void wave16_consts_build(const uint32_t (&ext_term)[4][16], const uint32_t* internal,
+                                Poseidon2Wave16& Wc) {
+  using namespace lazy_host;
+  constexpr int N = 16, RP = 20;
+  const uint32_t R = (uint32_t)((1ull << 32) % P), Rinv = minv(R);
+  uint32_t a = mm(1, Rinv);
+  for (int r = 0; r < 4; ++r) a = mm(mm(mm(a, a), a), Rinv);
+  const uint32_t sigma = a;
+  for (int r = 0; r < RP; ++r) a = mm(a, Rinv);
+  const uint32_t a_t = a;
+  const uint32_t si1 = minv(sigma), si2 = mm(si1, si1), si3 = mm(si2, si1);
+  typedef uint32_t Mat[N][N];
+  static Mat pw[RP + 1];  // A^n
+  Mat A;
+  uint32_t b[N];
+  for (int i = 0; i < N; ++i) {
+    for (int k = 0; k < N; ++k) A[i][k] = k == 0 ? 0u : (uint32_t)((1ull + (i == k ? Poseidon2Lazy<16>::diag(i) : 0u)) % P);
+    b[i] = (uint32_t)((1ull + (i == 0 ? Poseidon2Lazy<16>::diag(0) : 0u)) % P);
+  }
+  for (int i = 0; i < N; ++i)
+    for (int k = 0; k < N; ++k) pw[0][i][k] = i == k ? 1u : 0u;
+  for (int n = 1; n <= RP; ++n)
+    for (int i = 0; i < N; ++i)
+      for (int k = 0; k < N; ++k) {
+        uint64_t s = 0;
+        for (int j = 0; j < N; ++j) s = (s + (uint64_t)A[i][j] * pw[n - 1][j][k]) % P;
+        pw[n][i][k] = (uint32_t)s;
+      }
+  auto apply_b = [&](int n, int i) {  // (A^n b)[i]
+    uint64_t s = 0;
+    for (int j = 0; j < N; ++j) s = (s + (uint64_t)pw[n][i][j] * b[j]) % P;
+    return (uint32_t)s;
+  };
+  memset(&Wc, 0, sizeof(Wc));
+  for (int i = 0; i < N; ++i) {
+    const uint32_t ra = mm(R, a_t);
+    for (int j = 0; j < RP; ++j) Wc.col[i][j] = mm(mm(ra, apply_b(RP - 1 - j, i)), si3);
+    for (int k = 1; k < N; ++k) Wc.col[i][RP + k - 1] = mm(mm(ra, pw[RP][i][k]), si1);
+    Wc.col[i][35] = mm(ra, ext_term[0][i]);
+  }
+  for (int r = 1; r < RP; ++r) {
+    uint32_t* c = Wc.col[16 + r];
+    for (int j = 0; j < r; ++j) c[j] = mm(mm(R, apply_b(r - 1 - j, 0)), si2);
+    for (int k = 1; k < N; ++k) c[RP + k - 1] = mm(R, pw[r][0][k]);
+    c[35] = mm(mm(R, sigma), internal[r]);
+  }
 }
 
 // TruncatedPermutation<P16, 2, 8, 16>: first 8 words of P16(l || r).

@@ -161,6 +161,9 @@ static int upload_constants(uint64_t seed) {
     lazy_consts_build<16>(c.ext16_init, c.ext16_term, c.int16, lz.w16);
     lazy_consts_build<24>(c.ext24_init, c.ext24_term, c.int24, lz.w24);
     HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_p2l), &lz, sizeof(lz)));
+    static Poseidon2Wave16 wz;
+    wave16_consts_build(c.ext16_term, c.int16, wz);
+    HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_p2w), &wz, sizeof(wz)));
   }
   return WHIR_OK;
 }
@@ -468,14 +471,21 @@ __global__ void permute_kernel(uint32_t* st, uint64_t n) {
   for (int j = 0; j < W; ++j) st[i * W + j] = s[j];
 }
 
-// Cooperative (one wavefront per state) P16, as used by the device transcript.
+// Cooperative P16, one wavefront per state: WAVE = 0, the 16-lane row form of
+// the Merkle tree tops (every row a copy); WAVE = 1, the device transcript's
+// one-wavefront form (KoalaBear; BabyBear's transcript uses the row form).
+template <int WAVE>
 __global__ void permute16_coop_kernel(uint32_t* st, uint64_t n) {
   const uint64_t i = blockIdx.x;
   if (i >= n) return;
   uint32_t s = st[i * 16 + (threadIdx.x & 15)];
   CoopP16 K;
   K.load();
-  permute16_coop(s, K);
+#if !defined(WHIR_FIELD_BABYBEAR)
+  if constexpr (WAVE) permute16_wave(s, K);
+  else
+#endif
+    permute16_coop(s, K);
   if (threadIdx.x < 16) st[i * 16 + threadIdx.x] = s;
 }
 
@@ -485,7 +495,8 @@ extern "C" int whir_poseidon2_permute(whir_ctx* c, int width, uint32_t* d_states
   const dim3 grid((unsigned)((n + 255) / 256)), block(256);
   if (width == 16) hipLaunchKernelGGL(permute_kernel<16>, grid, block, 0, s, d_states, n);
   else if (width == 24) hipLaunchKernelGGL(permute_kernel<24>, grid, block, 0, s, d_states, n);
-  else if (width == -16) hipLaunchKernelGGL(permute16_coop_kernel, dim3((unsigned)n), dim3(64), 0, s, d_states, n);
+  else if (width == -16) hipLaunchKernelGGL(permute16_coop_kernel<0>, dim3((unsigned)n), dim3(64), 0, s, d_states, n);
+  else if (width == -64) hipLaunchKernelGGL(permute16_coop_kernel<1>, dim3((unsigned)n), dim3(64), 0, s, d_states, n);
   else return WHIR_EINVAL;
   HIP_TRY(hipGetLastError());
   return WHIR_OK;
