@@ -243,29 +243,18 @@ __device__ __forceinline__ uint64_t hi_reduce(uint64_t t) {
   return ((uint64_t)(h < h2 ? h : h2) << 32) | (uint32_t)t;
 }
 
-// x^3 R^-2 (the S-box on a word) for a wave-uniform x < p + 2^7 held in an
-// SGPR, on the VALU: result < 1.75 p. (Written as plain C++, the compiler
-// moves the uniform chain to the scalar unit: 1.74 instead of 1.41 us per
-// permutation.)
-__device__ __forceinline__ uint32_t cube_uniform(uint32_t x) {
-  uint64_t t;
-  asm("v_mad_u64_u32 %0, vcc, %1, %1, 0" : "=v"(t) : "s"(x) : "vcc");
-  const uint32_t sq = redc(t);  // < 1.5 p
-  asm("v_mad_u64_u32 %0, vcc, %1, %2, 0" : "=v"(t) : "v"(sq), "s"(x) : "vcc");
-  return redc(t);
-}
-
 // Poseidon2-16 on one wavefront: lane i < 16 holds word i in and out (lanes
 // 16..63 carry no state: they end with unspecified values). External rounds
 // as in permute16_coop; the 20 internal rounds as the recurrence of
 // Poseidon2Wave16: every lane keeps one 64-bit accumulator (lane 16 + r: the
 // S-box input of round r; lane i < 16: output word i), seeded from the 15
-// input words 1..15 (read lane by lane as scalars). Round r reads its S-box
-// input from lane 16 + r as a scalar, cubes it, and adds coefficient x cube
-// to every accumulator in one instruction, so a round's dependent chain is
-// the S-box, one multiply-add, two reductions and a lane read, with no
-// cross-lane sum. Measured on one wavefront (100 chained calls): 1.41 us per
-// permutation against 1.94 us for permute16_coop.
+// input words 1..15 (read lane by lane as scalars). Round r cubes every
+// lane's reduced accumulator, reads the cube of lane 16 + r as a scalar and
+// adds coefficient x cube to every accumulator in one instruction, so a
+// round's dependent chain is one multiply-add, two reductions, the S-box and
+// a lane read, with no cross-lane sum. Measured on one wavefront (100 chained
+// calls): 1.41 us per permutation against 1.94 us for permute16_coop (with
+// the S-box on the scalar read, computed on the scalar unit: 1.74 us).
 __device__ __forceinline__ void permute16_wave(uint32_t& s, const CoopP16& K) {
   const Poseidon2Lazy<16>& C = c_p2l.w16;
   const uint4* cp = reinterpret_cast<const uint4*>(c_p2w.col[threadIdx.x & 63]);
@@ -290,13 +279,15 @@ __device__ __forceinline__ void permute16_wave(uint32_t& s, const CoopP16& K) {
     acc = mad64(cw[19 + k], (uint32_t)__builtin_amdgcn_readlane((int)x, k), acc);
     if (k >= 4 && (k % 2 == 0 || k == 15)) acc = hi_reduce(acc);
   }
-  uint32_t xh = (uint32_t)__builtin_amdgcn_readlane((int)x, 0);  // round 0's S-box input
+  uint32_t xr = x;  // round 0's S-box input is lane 0's word
 #pragma unroll
   for (int r = 0; r < 20; ++r) {
-    acc = hi_reduce(mad64(cw[r], cube_uniform(xh), acc));  // high word < p + 0.87 p before
-    if (r < 19) xh = (uint32_t)__builtin_amdgcn_readlane((int)canon2(redc(acc)), 17 + r);
+    // cube < 1.75 p (inputs < p + 2^7); the high word stays below 2p before the reduction
+    const uint32_t y = (uint32_t)__builtin_amdgcn_readlane((int)mulr(mulr(xr, xr), xr), r == 0 ? 0 : 16 + r);
+    acc = hi_reduce(mad64(cw[r], y, acc));
+    xr = canon2(redc(acc));
   }
-  x = canon2(redc(acc));
+  x = xr;
 #pragma unroll
   for (int r = 6; r < 9; ++r) x = mds16_coop_lazy(mulr(mulr(x, x), x), K.la[r]);
   x = mds16_coop_lazy(mulr(mulr(x, x), x), 0u);
