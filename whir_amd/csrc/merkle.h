@@ -170,7 +170,9 @@ __global__ void __launch_bounds__(MERKLE_THREADS) merkle_layer_pair_kernel(JP<co
 // A workgroup is one wave per SIMD (16 compressions in flight), so a level of
 // at most 16 compressions per workgroup costs one cooperative P16 latency; the
 // host cuts the top of a tree into launches of at most COOP_MAX_NODES nodes
-// per residue class (five levels each).
+// per residue class (five levels each). A level of at most four compressions
+// per workgroup runs them one per wavefront in the lower-latency one-wavefront
+// form (permute16_wave, KoalaBear).
 constexpr int COOP_THREADS = 256;
 constexpr uint32_t COOP_MAX_NODES = 32;  // n0 / G
 
@@ -189,14 +191,27 @@ __device__ __forceinline__ void coop_subtrees_body(uint32_t* layer, uint32_t n0,
     const uint32_t h = kk >> 1;
     // compression t reads nodes t and t + h and overwrites node t: no other
     // compression of this layer touches either, so the update is in place
-    for (uint32_t t = slot; t < h; t += COOP_THREADS / 16) {
-      uint32_t v = li < 8 ? s_node[t * 8 + li] : s_node[(t + h) * 8 + (li - 8)];
-      permute16_coop(v, K);
-      if (li < 8) {
-        s_node[t * 8 + li] = v;
-        next[(uint64_t)(w + t * G) * 8 + li] = v;
+#if !defined(WHIR_FIELD_BABYBEAR)
+    if (h <= COOP_THREADS / 64) {
+      const uint32_t t = threadIdx.x >> 6, lane = threadIdx.x & 63;
+      if (t < h) {  // wavefront t: compression t (words in lanes 0..15)
+        uint32_t v = li < 8 ? s_node[t * 8 + li] : s_node[(t + h) * 8 + (li - 8)];
+        v = permute16_wave_shared(v, K);
+        if (lane < 8) {
+          s_node[t * 8 + lane] = v;
+          next[(uint64_t)(w + t * G) * 8 + lane] = v;
+        }
       }
-    }
+    } else
+#endif
+      for (uint32_t t = slot; t < h; t += COOP_THREADS / 16) {
+        uint32_t v = li < 8 ? s_node[t * 8 + li] : s_node[(t + h) * 8 + (li - 8)];
+        permute16_coop(v, K);
+        if (li < 8) {
+          s_node[t * 8 + li] = v;
+          next[(uint64_t)(w + t * G) * 8 + li] = v;
+        }
+      }
     __syncthreads();
     n >>= 1;
     next += (uint64_t)n * 8;
