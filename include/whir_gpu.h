@@ -57,7 +57,10 @@ typedef struct whir_prover whir_prover; /* device workspace for one proof shape 
 
 /* Creates the context on `device`: uploads the Poseidon2 KoalaBear constants of
  * Poseidon2KoalaBear<16|24>::new_from_rng_128(StdRng::seed_from_u64(seed))
- * (crates/whir/src/test.rs:195-196; the reference uses seed 1000). */
+ * (crates/whir/src/test.rs:195-196; the reference uses seed 1000). The
+ * constants live in the device's constant memory: contexts on one device share
+ * them, so they must use one seed (the last created context's constants are
+ * the ones in effect). Thread-safe. */
 int whir_ctx_create(int device, uint64_t poseidon_seed, whir_ctx** out);
 /* Destroy calls release everything; a failed HIP release returns WHIR_EDEVICE. */
 int whir_ctx_destroy(whir_ctx* ctx);

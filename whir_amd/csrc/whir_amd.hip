@@ -115,6 +115,11 @@ static MerkleConfig merkle_config(whir_ctx* c, int log_h, uint32_t row_words, ui
   return it == c->merkle_tuned.end() ? c->merkle : it->second;
 }
 
+// The constant tables are per-device globals built in static buffers:
+// contexts created from several host threads build and upload them one at a
+// time.
+static std::mutex g_constants_mu;
+
 #if defined(WHIR_FIELD_BABYBEAR)
 // Poseidon2BabyBear<16|24>::new_from_rng_128(StdRng::seed_from_u64(seed)): per
 // width a fresh generator, 4 initial external rounds of W constants, 4
@@ -131,6 +136,7 @@ static void plain_consts_draw(uint64_t seed, Poseidon2Plain<W>& C) {
 }
 
 static int upload_constants(uint64_t seed) {
+  std::lock_guard<std::mutex> g(g_constants_mu);
   static Poseidon2PlainConsts pc;
   plain_consts_draw<16>(seed, pc.w16);
   plain_consts_draw<24>(seed, pc.w24);
@@ -139,6 +145,7 @@ static int upload_constants(uint64_t seed) {
 }
 #else
 static int upload_constants(uint64_t seed) {
+  std::lock_guard<std::mutex> g(g_constants_mu);
   Poseidon2Consts c;
   {
     whir_host::ChaChaRng r(seed);
