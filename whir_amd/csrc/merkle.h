@@ -53,13 +53,18 @@ __device__ __forceinline__ void leaf_hash_row(const uint32_t* __restrict__ rows,
   out[1] = make_uint4(s[4], s[5], s[6], s[7]);
 }
 
-// One sponge per lane.
+// One sponge per lane. Rows of several chunks run the permutation in a loop,
+// out of which LLVM would hoist the round constants into more scalar
+// registers than there are (221 spilled through VGPR lanes, ~450 extra
+// instructions per permutation), so they read the constants inside the loop
+// (OPAQUE): 3 864 instead of 4 307 instructions per loop body and 90 instead
+// of 104 VGPRs.
 template <int ROW_WORDS>
 __global__ void __launch_bounds__(MERKLE_THREADS) merkle_leaves_kernel(JP<const uint32_t> rows_, uint64_t n_rows,
                                                                    JP<uint32_t> leaves_) {
   const uint64_t i = (uint64_t)blockIdx.x * MERKLE_THREADS + threadIdx.x;
   if (i >= n_rows) return;
-  leaf_hash_row<ROW_WORDS>(rows_, i, leaves_);
+  leaf_hash_row<ROW_WORDS, (ROW_WORDS > 16)>(rows_, i, leaves_);
 }
 
 // parent[j] = C(prev[j], prev[j + half]), on this lane (OPAQUE as for
