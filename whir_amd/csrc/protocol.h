@@ -523,49 +523,108 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(JP<DevTx> t_, JP<ui
   }
 }
 
+// Bitonic sort of s_v[0..np) (np a power of two, values at and past n read
+// as 0xFFFFFFFF) with the values in registers: thread t holds elements
+// t + k T (k < E, E T >= np). A compare-exchange partner x ^ stride is in the
+// same thread when stride >= T (another register), in the same wavefront when
+// stride < 64 (a lane shuffle) and in another wavefront otherwise (through
+// s_v, two workgroup barriers). Writes the sorted values back to s_v.
+template <int E>
+__device__ void bitonic_regs(uint32_t* s_v, uint32_t n, uint32_t np) {
+  const uint32_t tid = threadIdx.x, T = blockDim.x;
+  uint32_t v[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const uint32_t x = tid + (uint32_t)k * T;
+    v[k] = x < n ? s_v[x] : 0xFFFFFFFFu;  // > any index (bits <= 31)
+  }
+  for (uint32_t size = 2; size <= np; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride >= T) {
+        const uint32_t kx = stride / T;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+          const uint32_t kp = (uint32_t)k ^ kx;
+          if ((uint32_t)k < kp && kp < (uint32_t)E) {
+            const uint32_t x = tid + (uint32_t)k * T;
+            const bool up = (x & size) == 0;
+            const uint32_t a = v[k], b = v[kp];
+            if ((a > b) == up) {
+              v[k] = b;
+              v[kp] = a;
+            }
+          }
+        }
+        continue;
+      }
+      uint32_t o[E];
+      if (stride < 64) {
+#pragma unroll
+        for (int k = 0; k < E; ++k) o[k] = (uint32_t)__shfl_xor((int)v[k], (int)stride, 64);
+      } else {
+#pragma unroll
+        for (int k = 0; k < E; ++k) s_v[tid + (uint32_t)k * T] = v[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < E; ++k) o[k] = s_v[(tid + (uint32_t)k * T) ^ stride];
+        __syncthreads();
+      }
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const uint32_t x = tid + (uint32_t)k * T;
+        const bool keep_min = ((x & stride) == 0) == ((x & size) == 0);
+        v[k] = keep_min ? (v[k] < o[k] ? v[k] : o[k]) : (v[k] > o[k] ? v[k] : o[k]);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const uint32_t x = tid + (uint32_t)k * T;
+    if (x < np) s_v[x] = v[k];
+  }
+}
+
 // Sort s_v[0..n) and write its distinct values in ascending order to out[],
-// with the whole workgroup (s_scan: blockDim.x words). A bitonic network over
-// the next power of two (padding sorts last), then a workgroup prefix count
-// over "first of its value" flags gives each distinct value its rank. Returns
-// the number of distinct values (in every thread). n <= MAX_STIR.
+// with the whole workgroup (s_scan: blockDim.x words; blockDim.x a multiple
+// of 64 with 4 blockDim.x >= n). A bitonic network over the next power of two
+// (bitonic_regs; padding sorts last), then a workgroup prefix count over
+// "first of its value" flags gives each distinct value its rank. Returns the
+// number of distinct values (in every thread). n <= MAX_STIR. Measured on
+// MI355X for n = 287 in 256 threads: 10.2 us with every stage a round trip
+// through LDS and a workgroup barrier.
 __device__ uint32_t sort_dedup_block(uint32_t* s_v, uint32_t n, uint32_t* s_scan, uint32_t* out) {
   const uint32_t tid = threadIdx.x, T = blockDim.x;
   uint32_t np = 1;
   while (np < n) np <<= 1;
-  for (uint32_t i = n + tid; i < np; i += T) s_v[i] = 0xFFFFFFFFu;  // > any index (bits <= 31)
+  __syncthreads();  // s_v complete
+  if (np <= T) bitonic_regs<1>(s_v, n, np);
+  else if (np <= 2 * T) bitonic_regs<2>(s_v, n, np);
+  else bitonic_regs<4>(s_v, n, np);
   __syncthreads();
-  for (uint32_t size = 2; size <= np; size <<= 1) {
-    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      for (uint32_t i = tid; i < np / 2; i += T) {
-        const uint32_t lo = 2 * i - (i & (stride - 1));  // index with bit `stride` clear
-        const uint32_t hi = lo + stride;
-        const bool up = (lo & size) == 0;
-        const uint32_t a = s_v[lo], b = s_v[hi];
-        if ((a > b) == up) {
-          s_v[lo] = b;
-          s_v[hi] = a;
-        }
-      }
-      __syncthreads();
-    }
-  }
   // per thread: a contiguous chunk of the sorted values; count the firsts
   const uint32_t chunk = (n + T - 1) / T;
   const uint32_t b0 = tid * chunk, b1 = b0 + chunk < n ? b0 + chunk : n;
   uint32_t cnt = 0;
   for (uint32_t i = b0; i < b1; ++i) cnt += (i == 0 || s_v[i] != s_v[i - 1]) ? 1u : 0u;
-  s_scan[tid] = cnt;
-  __syncthreads();
-  for (uint32_t off = 1; off < T; off <<= 1) {
-    const uint32_t add = tid >= off ? s_scan[tid - off] : 0u;
-    __syncthreads();
-    s_scan[tid] += add;
-    __syncthreads();
+  // exclusive prefix: an inclusive scan in each wavefront, then the
+  // wavefronts' totals through LDS
+  const uint32_t lane = tid & 63, wave = tid >> 6, nw = T >> 6;
+  uint32_t inc = cnt;
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t o = (uint32_t)__shfl_up((int)inc, d, 64);
+    if (lane >= d) inc += o;
   }
-  uint32_t rank = s_scan[tid] - cnt;  // exclusive prefix
+  if (lane == 63) s_scan[wave] = inc;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+  for (uint32_t w = 0; w < nw; ++w) {
+    const uint32_t c = s_scan[w];
+    total += c;
+    if (w < wave) before += c;
+  }
+  uint32_t rank = before + inc - cnt;
   for (uint32_t i = b0; i < b1; ++i)
     if (i == 0 || s_v[i] != s_v[i - 1]) out[rank++] = s_v[i];
-  const uint32_t total = s_scan[T - 1];
   __syncthreads();
   return total;
 }
