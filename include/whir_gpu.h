@@ -131,6 +131,13 @@ int whir_merkle_commit(whir_ctx* ctx, const uint32_t* d_rows, uint32_t log_h, ui
  * transcript's one-wavefront form). */
 int whir_poseidon2_permute(whir_ctx* ctx, int width, uint32_t* d_states, uint64_t n, void* stream);
 
+/* Replaces the sorted().dedup() of stir_indexes (crates/whir/src/whir.rs:63-77):
+ * the distinct values of d_in[0..n) in ascending order into d_out, their
+ * number into d_count (device words). n <= 1024; values below 2^32 - 1 (STIR
+ * indices have at most 31 bits). */
+int whir_sort_dedup(whir_ctx* ctx, const uint32_t* d_in, uint32_t n, uint32_t* d_out, uint32_t* d_count,
+                    void* stream);
+
 /* Replaces MerkleTree::query / MerkleTreeExt::query (crates/merkle/src/lib.rs:30-37,
  * poseidon_packed.rs:145-233) for n leaf indices: per index, the row (row_words
  * words of d_rows) then the log_h sibling digests bottom-up (sibling of the

@@ -124,3 +124,27 @@ def test_merkle_open_paths_verify(gpu_ctx, oracle, log_h, row_words):
             pos = cur & ((mid << 1) - 1)
             node = oracle.compress16(node, sib) if pos < mid else oracle.compress16(sib, node)
         assert np.array_equal(node, root)
+
+
+SORT_NS = [0, 1, 2, 3, 63, 64, 65, 127, 128, 129, 255, 256, 257, 287, 511, 512, 513, 1000, 1024]
+
+
+@pytest.mark.parametrize("n", SORT_NS)
+def test_sort_dedup(gpu_ctx, n):
+    """sorted().dedup() of stir_indexes (whir.rs:63-77) is np.unique of the
+    values: random indices with duplicates, all equal, ascending, descending
+    and values next to the padding word."""
+    import whir_amd
+
+    rng = np.random.default_rng(7000 + n)
+    cases = [
+        rng.integers(0, max(1, n // 2), n, dtype=np.uint64).astype(np.uint32),
+        rng.integers(0, 1 << 31, n, dtype=np.uint64).astype(np.uint32),
+        np.full(n, 12345, np.uint32),
+        np.arange(n, dtype=np.uint32),
+        np.arange(n, dtype=np.uint32)[::-1].copy(),
+        (0xFFFFFFFE - rng.integers(0, 3, n, dtype=np.uint64)).astype(np.uint32),
+    ]
+    print(f"sort_dedup n={n}", flush=True)
+    for v in cases:
+        assert np.array_equal(whir_amd.api.sort_dedup(v, ctx=gpu_ctx), np.unique(v))

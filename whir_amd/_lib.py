@@ -80,6 +80,7 @@ SIGNATURES = [
      [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp]),
     ("whir_eq_table", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, _vp, _vp]),
     ("whir_eval_ext", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp, _vp]),
+    ("whir_sort_dedup", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp, _vp]),
     ("whir_compress_hi", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp]),
     ("whir_compress_lo", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp]),
     ("whir_compress_hi_noalloc", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp]),

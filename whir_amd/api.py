@@ -180,6 +180,21 @@ def merkle_open(layers, rows, log_h, indices, ctx=None):
     return out.reshape(len(idx), per)
 
 
+def sort_dedup(values, ctx=None):
+    """The sorted().dedup() of stir_indexes on the device (len(values) <= 1024)."""
+    ctx = ctx or default_context()
+    v = np.asarray(values, np.uint32).ravel()
+    dv = DeviceBuffer(max(4, v.size * 4))
+    if v.size:
+        dv.upload(v)
+    do = DeviceBuffer(max(4, v.size * 4))
+    dc = DeviceBuffer(4)
+    check(ctx.L.whir_sort_dedup(ctx.ptr, dv.ptr, v.size, do.ptr, dc.ptr, None), "whir_sort_dedup")
+    check(lib().whir_device_synchronize(), "sync")
+    count = int(dc.download(np.uint32, 1)[0])
+    return do.download(np.uint32, max(count, 1))[:count]
+
+
 def eq_table(z, scale, ctx=None):
     """Point::eq: 2^k x 4 words for a point z (k x 4 words)."""
     ctx = ctx or default_context()

@@ -125,6 +125,20 @@ __global__ void __launch_bounds__(SC_THREADS) merkle_open_kernel(const uint32_t*
   }
 }
 
+// The sorted().dedup() of stir_indexes (crates/whir/src/whir.rs:63-77) for n
+// given values: one workgroup of SERIAL_THREADS threads running
+// sort_dedup_block, as stir_begin_kernel does on the drawn indices.
+__global__ void __launch_bounds__(SERIAL_THREADS) sort_dedup_kernel(const uint32_t* __restrict__ in, uint32_t n,
+                                                                   uint32_t* __restrict__ out, uint32_t* count) {
+  __shared__ uint32_t s_v[MAX_STIR];
+  __shared__ uint32_t s_scan[SERIAL_THREADS];
+  __shared__ uint32_t s_out[MAX_STIR];
+  for (uint32_t i = threadIdx.x; i < n; i += SERIAL_THREADS) s_v[i] = in[i];
+  const uint32_t c = sort_dedup_block(s_v, n, s_scan, s_out);
+  for (uint32_t i = threadIdx.x; i < c; i += SERIAL_THREADS) out[i] = s_out[i];
+  if (threadIdx.x == 0) *count = c;
+}
+
 }  // namespace kb
 
 // ============================ component C ABI ============================
@@ -366,6 +380,15 @@ extern "C" int whir_merkle_open(whir_ctx* c, const uint32_t* d_layers, uint32_t 
   hipStream_t s = pick_stream(c, stream);
   hipLaunchKernelGGL(merkle_open_kernel, dim3(n), dim3(SC_THREADS), 0, s, d_layers, log_h, d_rows, row_words, d_idx,
                      d_out);
+  HIP_TRY(hipGetLastError());
+  return WHIR_OK;
+}
+
+extern "C" int whir_sort_dedup(whir_ctx* c, const uint32_t* d_in, uint32_t n, uint32_t* d_out, uint32_t* d_count,
+                               void* stream) {
+  if (!c || (!d_in && n) || (!d_out && n) || !d_count || n > (uint32_t)MAX_STIR) return WHIR_EINVAL;
+  hipStream_t s = pick_stream(c, stream);
+  hipLaunchKernelGGL(sort_dedup_kernel, dim3(1), dim3(SERIAL_THREADS), 0, s, d_in, n, d_out, d_count);
   HIP_TRY(hipGetLastError());
   return WHIR_OK;
 }
