@@ -96,11 +96,8 @@ __device__ __forceinline__ void permute16_coop(uint32_t& s, const CoopP16& K) {
 
 using CoopLane = CoopP16;
 
-__device__ __noinline__ uint32_t permute16_coop_shared(uint32_t s, CoopLane L) {
-  permute16_coop(s, L);
-  return s;
-}
-// The transcript's permutation: the row form here (every row a copy).
+// The transcript's permutation, out of line: the row form here (every row a
+// copy).
 __device__ __noinline__ uint32_t permute16_wave_shared(uint32_t s, CoopLane L) {
   permute16_coop(s, L);
   return s;
@@ -297,16 +294,13 @@ __device__ __forceinline__ void permute16_wave(uint32_t& s, const CoopP16& K) {
 // The transcript's permutation, out of line. A transcript kernel is one
 // workgroup running straight-line code once, so its time goes to fetching
 // instructions into a cold instruction cache as much as to executing them; with
-// the permutation (~900 instructions) inlined at every observe/sample site, each
+// the permutation (~600 instructions) inlined at every observe/sample site, each
 // site is a copy of its own and each executed copy is fetched cold (measured on
 // MI355X: a claim's draw, eq tables and value took ~14 us in one workgroup, 3-5x
 // their instruction count). One shared copy is fetched once per kernel (and
 // stays in L2 between kernels). Its arguments are the lane's state word and
-// lane constants; the wave-uniform constants come from constant memory.
-__device__ __noinline__ uint32_t permute16_coop_shared(uint32_t s, CoopLane L) {
-  permute16_coop(s, L);
-  return s;
-}
+// lane constants; the wave-uniform constants and the coefficient columns come
+// from constant memory.
 __device__ __noinline__ uint32_t permute16_wave_shared(uint32_t s, CoopLane L) {
   permute16_wave(s, L);
   return s;
