@@ -683,11 +683,17 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_begin_kernel(JP<DevTx> t_
   }
   if (!a.query_scales) return;
   // query scales: thread t takes q = t + 256 i as alpha^(n_eq + t) alpha^(256 i)
-  Ext sq = epow(s_alpha, (uint64_t)a.n_eq + tid), step = s_alpha;
-  for (int i = 0; i < 8; ++i) step = esqr(step);  // alpha^256 (SERIAL_THREADS)
-  for (uint32_t q = tid; q < count; q += SERIAL_THREADS) {
-    a.qa.scale[q] = sq;
-    sq = emul(sq, step);
+  // (alpha^256 only when there are more than 256 queries: its eight squarings
+  // would otherwise follow the power on every thread's chain)
+  Ext sq = epow(s_alpha, (uint64_t)a.n_eq + tid);
+  if (tid < count) a.qa.scale[tid] = sq;
+  if (count > SERIAL_THREADS) {
+    Ext step = s_alpha;
+    for (int i = 0; i < 8; ++i) step = esqr(step);  // alpha^256 (SERIAL_THREADS)
+    for (uint32_t q = tid + SERIAL_THREADS; q < count; q += SERIAL_THREADS) {
+      sq = emul(sq, step);
+      a.qa.scale[q] = sq;
+    }
   }
 }
 
