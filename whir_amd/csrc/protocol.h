@@ -802,7 +802,8 @@ __global__ void __launch_bounds__(SERIAL_THREADS) stir_query_kernel(JP<const Dev
       const uint32_t om = two_adic_generator(depth);
       qa.index[q] = index;
       qa.ws[q] = pow(om, index);
-      qa.ts[q] = inv(pow(om, index & ((1u << (depth - 1)) - 1)));
+      // omega^-(index mod mid) = omega^(2^depth - index mod mid): om has order 2^depth
+      qa.ts[q] = pow(om, (1ull << depth) - (index & ((1u << (depth - 1)) - 1)));
     }
   } else if (mode == 2) {
     const uint32_t var = pow(two_adic_generator(depth), index);
