@@ -492,11 +492,13 @@ __global__ void __launch_bounds__(SERIAL_THREADS) svo_kernel(JP<DevTx> t_, JP<ui
       if (tid == 0) {
         s_rs[ri] = r;
         js->rs[ri] = r;
-        Ext l[3];
-        lagrange012(r, l);
-        s_lag[ri][0] = l[0];
-        s_lag[ri][1] = l[1];
-        s_lag[ri][2] = l[2];
+      }
+      if (tid < 3) {  // lagrange012(r), one weight per lane
+        const Ext one = eone(), two = efrom(TWO);
+        const Ext a = tid == 0 ? esub(r, one) : r;
+        const Ext b = tid == 0 ? esub(r, two) : tid == 1 ? esub(two, r) : esub(r, one);
+        const Ext m = emul(a, b);
+        s_lag[ri][tid] = tid == 1 ? m : emul_base(m, HALF);
       }
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
