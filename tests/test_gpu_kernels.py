@@ -69,6 +69,11 @@ DFT_CASES = [
     (1 << 16, 17, 16),
     (1 << 13, 16, 32),
     (1 << 20, 21, 16),  # the base encode of 2^24 coefficients (default plan: register pass + two LDS passes)
+    # ragged inputs: first-pass lines whose rows run past n_in (clamped loads, zeroed rows)
+    (3, 5, 8),
+    (1000, 12, 16),
+    ((1 << 16) - 7, 18, 16),
+    ((1 << 20) - 1, 21, 16),
 ]
 
 

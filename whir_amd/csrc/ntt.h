@@ -71,29 +71,53 @@ struct Vec {
   uint32_t v[V];
 };
 
+// The transform's arrays are device memory. Their pointers reach the kernels
+// through JP (job-tagged addresses), which hides the address space from the
+// compiler; on the device the loads and stores name the global space, so they
+// are global_load / global_store (in-order vmcnt waits) rather than flat ones
+// (which also count on lgkmcnt and are waited for as a whole).
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gptr(const uint32_t* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gptr(uint32_t* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+#else
+template <class T>
+KB_HD const T* gptr(const uint32_t* p) {
+  return reinterpret_cast<const T*>(p);
+}
+template <class T>
+KB_HD T* gptr(uint32_t* p) {
+  return reinterpret_cast<T*>(p);
+}
+#endif
 template <int V>
 KB_HD Vec<V> vload(const uint32_t* p) {
   Vec<V> r;
   if constexpr (V == 4) {
-    const uint4 t = *reinterpret_cast<const uint4*>(p);
+    const uint4 t = *gptr<uint4>(p);
     r.v[0] = t.x;
     r.v[1] = t.y;
     r.v[2] = t.z;
     r.v[3] = t.w;
   } else if constexpr (V == 2) {
-    const uint2 t = *reinterpret_cast<const uint2*>(p);
+    const uint2 t = *gptr<uint2>(p);
     r.v[0] = t.x;
     r.v[1] = t.y;
   } else {
-    r.v[0] = *p;
+    r.v[0] = *gptr<uint32_t>(p);
   }
   return r;
 }
 template <int V>
 KB_HD void vstore(uint32_t* p, const Vec<V>& x) {
-  if constexpr (V == 4) *reinterpret_cast<uint4*>(p) = make_uint4(x.v[0], x.v[1], x.v[2], x.v[3]);
-  else if constexpr (V == 2) *reinterpret_cast<uint2*>(p) = make_uint2(x.v[0], x.v[1]);
-  else *p = x.v[0];
+  if constexpr (V == 4) *gptr<uint4>(p) = make_uint4(x.v[0], x.v[1], x.v[2], x.v[3]);
+  else if constexpr (V == 2) *gptr<uint2>(p) = make_uint2(x.v[0], x.v[1]);
+  else *gptr<uint32_t>(p) = x.v[0];
 }
 template <int V>
 KB_HD void vscale(Vec<V>& x, uint32_t w) {
@@ -189,7 +213,63 @@ struct NttArgs {
 };
 
 KB_HD uint32_t tw24(const uint32_t* lo, const uint32_t* hi, uint32_t e) {
-  return mul(lo[e & 4095u], hi[(e >> 12) & 4095u]);
+  return mul(*gptr<uint32_t>(lo + (e & 4095u)), *gptr<uint32_t>(hi + ((e >> 12) & 4095u)));
+}
+
+// The first pass's R input rows r = r0 + j 2^log_step (j < R) of its line, in
+// coset e and input column c (col = e C + c), times the coset twist w_N^(e r);
+// rows >= n_in read as zero. Every row's load is issued before any is used:
+// a load under a per-row branch, followed by the row's twist multiply, waited
+// for each row in turn (16 round trips to HBM per thread in the base LDE's
+// radix-16 first pass). Rows past the input load row 0 and are zeroed after.
+template <int R, int V>
+KB_HD void load_input_rows(const NttArgs& a, uint32_t r0, uint32_t log_step, uint32_t col,
+                           Vec<V> (&x)[R]) {
+  const uint32_t e = col >> a.log_c, c = col & ((1u << a.log_c) - 1);
+  if (a.n_in == 0) {
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+#pragma unroll
+      for (int q = 0; q < V; ++q) x[j].v[q] = 0u;
+    return;
+  }
+  // the twist's table entries first, so that they are in flight with the rows
+  const uint32_t sh = 24 - a.log_nn, i0 = (e * r0) << sh, i1 = (e << log_step) << sh;
+  uint32_t t0[2] = {ONE, ONE}, t1[2] = {ONE, ONE};
+  if (e) {
+    const uint32_t* lo = a.tw_lo;
+    const uint32_t* hi = a.tw_hi;
+    t0[0] = *gptr<uint32_t>(lo + (i0 & 4095u));
+    t0[1] = *gptr<uint32_t>(hi + ((i0 >> 12) & 4095u));
+    t1[0] = *gptr<uint32_t>(lo + (i1 & 4095u));
+    t1[1] = *gptr<uint32_t>(hi + ((i1 >> 12) & 4095u));
+  }
+  const uint32_t* in = a.src + c;
+  if (r0 + ((uint32_t)(R - 1) << log_step) < a.n_in) {  // every row inside the input
+#pragma unroll
+    for (int j = 0; j < R; ++j) x[j] = vload<V>(in + ((r0 + ((uint32_t)j << log_step)) << a.log_c));
+  } else {
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const uint32_t r = r0 + ((uint32_t)j << log_step);
+      x[j] = vload<V>(in + ((r < a.n_in ? r : 0u) << a.log_c));
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const bool past = r0 + ((uint32_t)j << log_step) >= a.n_in;
+#pragma unroll
+      for (int q = 0; q < V; ++q) x[j].v[q] = past ? 0u : x[j].v[q];
+    }
+  }
+  if (e) {
+    uint32_t tw = mul(t0[0], t0[1]);  // tw24 of i0 and i1
+    const uint32_t st = mul(t1[0], t1[1]);
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      vscale<V>(x[j], tw);
+      tw = mul(tw, st);
+    }
+  }
 }
 
 // Workgroup `blk` of one pass (ntt_kernel runs it as blockIdx.x; the
@@ -220,28 +300,8 @@ __device__ __forceinline__ void ntt_block(const NttArgs& a, uint32_t blk, uint32
 #pragma unroll
     for (int ja = 0; ja < R1; ++ja) x[ja] = vload<V>(a.src + base + ((uint32_t)ja * R2 + jb) * jstride);
   } else {
-    // input row r = j M_1 + m (one block), coset e, input column c; twist w_N^(e r)
-    const uint32_t e = col >> a.log_c, c = col & ((1u << a.log_c) - 1);
-    uint32_t tw = ONE, st = ONE;
-    if (e) {
-      const uint32_t sh = 24 - a.log_nn;
-      tw = tw24(a.tw_lo, a.tw_hi, (e * ((jb << a.log_m) + m)) << sh);
-      st = tw24(a.tw_lo, a.tw_hi, (e << (LR2 + a.log_m)) << sh);
-    }
-#pragma unroll
-    for (int ja = 0; ja < R1; ++ja) {
-      const uint32_t r = ((((uint32_t)ja * R2 + jb)) << a.log_m) + m;
-      if (r < a.n_in) {
-        x[ja] = vload<V>(a.src + (r << a.log_c) + c);
-      } else {
-#pragma unroll
-        for (int q = 0; q < V; ++q) x[ja].v[q] = 0u;
-      }
-      if (e) {
-        vscale<V>(x[ja], tw);
-        tw = mul(tw, st);
-      }
-    }
+    // input row r = (j_a R2 + j_b) M_1 + m (one block)
+    load_input_rows<R1, V>(a, (jb << a.log_m) + m, LR2 + a.log_m, col, x);
   }
   if constexpr (LR2 == 0) {
     // register-only pass (an early pass of a multi-pass transform): output t
@@ -267,7 +327,7 @@ __device__ __forceinline__ void ntt_block(const NttArgs& a, uint32_t blk, uint32
   dft_regs<LR1, V, lazy_but_first<LR1>()>(x);
   if (R2 > 1 && jb) {  // inner twiddles w_L^(j_b t_a) = tw_hi[(j_b t_a mod L) 2^(12 - LL)] (L <= 4096)
 #pragma unroll
-    for (int ta = 1; ta < R1; ++ta) vscale<V>(x[ta], a.tw_hi[((jb * (uint32_t)ta) & ((1u << LL) - 1)) << (12 - LL)]);
+    for (int ta = 1; ta < R1; ++ta) vscale<V>(x[ta], *gptr<uint32_t>(a.tw_hi + (((jb * (uint32_t)ta) & ((1u << LL) - 1)) << (12 - LL))));
   } else {
 #pragma unroll
     for (int ta = 1; ta < R1; ++ta) vcanon<V>(x[ta]);
@@ -397,33 +457,13 @@ struct Ntt3 {
       #pragma unroll
       for (int j1 = 0; j1 < R1; ++j1) x[j1] = vload<V>(a.src + l.base + (((uint32_t)j1 << LT) + l.jb) * l.jstride);
     } else {
-      // input row r = j M + m, coset e, input column c; twist w_N^(e r)
-      const uint32_t e = l.col >> a.log_c, c = l.col & ((1u << a.log_c) - 1);
-      uint32_t tw = ONE, st = ONE;
-      if (e) {
-        const uint32_t sh = 24 - a.log_nn;
-        tw = tw24(a.tw_lo, a.tw_hi, (e * ((l.jb << a.log_m) + l.m)) << sh);
-        st = tw24(a.tw_lo, a.tw_hi, (e << (LT + a.log_m)) << sh);
-      }
-      #pragma unroll
-      for (int j1 = 0; j1 < R1; ++j1) {
-        const uint32_t r = ((((uint32_t)j1 << LT) + l.jb) << a.log_m) + l.m;
-        if (r < a.n_in) {
-          x[j1] = vload<V>(a.src + (r << a.log_c) + c);
-        } else {
-          #pragma unroll
-          for (int q = 0; q < V; ++q) x[j1].v[q] = 0u;
-        }
-        if (e) {
-          vscale<V>(x[j1], tw);
-          tw = mul(tw, st);
-        }
-      }
+      // input row r = (j1 T + jb) M + m
+      load_input_rows<R1, V>(a, (l.jb << a.log_m) + l.m, LT + a.log_m, l.col, x);
     }
     dft_regs<LR1, V, lazy_but_first<LR1>()>(x);
     if (l.jb) {  // w_L^(jb t1) = tw_hi[(jb t1 mod L) 2^(12 - LL)] (L <= 4096)
       #pragma unroll
-      for (int t1 = 1; t1 < R1; ++t1) vscale<V>(x[t1], a.tw_hi[((l.jb * (uint32_t)t1) & ((1u << LL) - 1)) << (12 - LL)]);
+      for (int t1 = 1; t1 < R1; ++t1) vscale<V>(x[t1], *gptr<uint32_t>(a.tw_hi + (((l.jb * (uint32_t)t1) & ((1u << LL) - 1)) << (12 - LL))));
     } else {
       #pragma unroll
       for (int t1 = 1; t1 < R1; ++t1) vcanon<V>(x[t1]);
@@ -443,7 +483,7 @@ struct Ntt3 {
       dft_regs<LR2, V, lazy_but_first<LR2>()>(y[k]);
       if (j3) {  // w_T^(j3 t2) = tw_hi[(j3 t2 mod T) 2^(12 - LT)]
         #pragma unroll
-        for (int t2 = 1; t2 < R2; ++t2) vscale<V>(y[k][t2], a.tw_hi[((j3 * (uint32_t)t2) & ((1u << LT) - 1)) << (12 - LT)]);
+        for (int t2 = 1; t2 < R2; ++t2) vscale<V>(y[k][t2], *gptr<uint32_t>(a.tw_hi + (((j3 * (uint32_t)t2) & ((1u << LT) - 1)) << (12 - LT))));
       } else {
         #pragma unroll
         for (int t2 = 1; t2 < R2; ++t2) vcanon<V>(y[k][t2]);
