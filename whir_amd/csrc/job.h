@@ -41,10 +41,18 @@ struct JP {
   // Device: this workgroup's job's address (blockIdx.z). Host: the pointer as
   // it is, tag included, to be passed on to a kernel (job_addr gives a job's
   // address to host code).
+  // On the device the address is formed as a global-space pointer and then
+  // converted to a generic one: the compiler then knows that every access
+  // through it is global memory and emits global_load / global_store (waited
+  // for with in-order vmcnt counts) instead of flat accesses (which also count
+  // on lgkmcnt, so a wait on LDS or scalar loads waits for them too). Every
+  // JP holds device (or host-mapped) memory, never LDS or scratch.
   __host__ __device__ __forceinline__ T* get() const {
 #if defined(__HIP_DEVICE_COMPILE__)
+    typedef __attribute__((address_space(1))) char gchar;
     const uint64_t v = (uint64_t)raw;
-    return (T*)((v & JOB_ADDR_MASK) + (uint64_t)blockIdx.z * ((v >> JOB_TAG_SHIFT) << JOB_STRIDE_UNIT_LOG));
+    gchar* g = (gchar*)(v & JOB_ADDR_MASK);
+    return (T*)(g + (uint64_t)blockIdx.z * ((v >> JOB_TAG_SHIFT) << JOB_STRIDE_UNIT_LOG));
 #else
     return raw;
 #endif
