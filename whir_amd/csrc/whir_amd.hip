@@ -565,6 +565,7 @@ struct whir_prover {
   uint32_t full0 = 0, n_full = 0;  // claims spanning the whole stacked vector
   StackBlocks sb{};
   uint32_t* host_log = nullptr;  // pinned: the log, LOG_TAIL status words, the transcript (tx_off)
+  uint32_t* host_log_dev = nullptr;  // host_log's device-mapped address (nullptr: copy with the DMA engine)
   size_t tx_off = 0;             // byte offset of the DevTx in the log allocation (and in host_log)
   size_t host_log_bytes = 0;     // needed by this shape
   size_t host_log_cap = 0;       // allocated
@@ -797,6 +798,10 @@ static int prover_bind(whir_prover* P, const std::vector<ArenaItem>& items) {
     p += align_up(it.bytes);
   }
   P->tx = (DevTx*)((char*)P->log + P->tx_off);
+  void* host_dev = nullptr;
+  P->host_log_dev = P->host_log && hipHostGetDevicePointer(&host_dev, P->host_log, 0) == hipSuccess
+                        ? (uint32_t*)host_dev : nullptr;
+  (void)hipGetLastError();
   const struct {
     void* dst;
     const void* src;
@@ -1479,7 +1484,37 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
 
 // The log, the status words final_phase_kernel leaves after it and the
 // transcript state, in one device-to-host copy.
+// The finished proof into the pinned host buffer (`host`, its device-mapped
+// address): the log's used words (the first status word after the log's
+// capacity `cap`; more than cap only on overflow), the status words and the
+// transcript state at word tx_word. A kernel starts right after the proof's
+// last kernel, where a DMA copy waited ~9 us for the copy engine, and only
+// the written part of the log crosses PCIe, not the whole capacity.
+static_assert(sizeof(DevTx) % 4 == 0, "the transcript state is copied in words");
+constexpr uint32_t LOG_COPY_THREADS = 256;
+__global__ void __launch_bounds__(LOG_COPY_THREADS) log_to_host_kernel(const uint32_t* __restrict__ log, uint32_t cap,
+                                                                       uint32_t tx_word, uint32_t* __restrict__ host) {
+  const uint32_t used = min(log[cap], cap);
+  const uint32_t i = (blockIdx.x * LOG_COPY_THREADS + threadIdx.x) * 4;  // log and host are 256-byte aligned
+  if (i + 4 <= used) {
+    *reinterpret_cast<uint4*>(host + i) = *reinterpret_cast<const uint4*>(log + i);
+  } else {
+    for (uint32_t j = i; j < used && j < i + 4; ++j) host[j] = log[j];
+  }
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < LOG_TAIL) host[cap + threadIdx.x] = log[cap + threadIdx.x];
+    for (uint32_t w = threadIdx.x; w < sizeof(DevTx) / 4; w += LOG_COPY_THREADS) host[tx_word + w] = log[tx_word + w];
+  }
+}
+
 static int enqueue_result_copy(whir_prover* P, hipStream_t s) {
+  if (P->jobs == 1 && P->host_log_dev) {
+    const uint32_t cap = (uint32_t)P->plan.log_words;
+    hipLaunchKernelGGL(log_to_host_kernel, dim3((cap + 4 * LOG_COPY_THREADS - 1) / (4 * LOG_COPY_THREADS)),
+                       dim3(LOG_COPY_THREADS), 0, s, P->log, cap, (uint32_t)(P->tx_off / 4), P->host_log_dev);
+    HIP_TRY(hipGetLastError());
+    return WHIR_OK;
+  }
   HIP_TRY(hipMemcpyAsync(P->host_log, P->log, P->host_log_bytes, hipMemcpyDeviceToHost, s));
   return WHIR_OK;
 }
