@@ -1484,18 +1484,22 @@ static int enqueue_open(whir_prover* P, const uint32_t* poly, hipStream_t s) {
 
 // The log, the status words final_phase_kernel leaves after it and the
 // transcript state, in one device-to-host copy.
-// The finished proof into the pinned host buffer (`host`, its device-mapped
-// address): the log's used words (the first status word after the log's
-// capacity `cap`; more than cap only on overflow), the status words and the
-// transcript state at word tx_word. A kernel starts right after the proof's
-// last kernel, where a DMA copy waited ~9 us for the copy engine, and only
-// the written part of the log crosses PCIe, not the whole capacity.
-static_assert(sizeof(DevTx) % 4 == 0, "the transcript state is copied in words");
+// Finished proofs into pinned host memory (`host`, its device-mapped address;
+// job z of a batch, blockIdx.z, at host + z * host_job_words): the log's used
+// words (the first status word after the log's capacity `cap`; more than cap
+// only on overflow), the status words and the transcript state at word
+// tx_word. A kernel starts right after the proof's last kernel, where a DMA
+// copy waited ~9 us for the copy engine, and only the written part of the log
+// crosses PCIe, not the whole capacity.
+static_assert(sizeof(DevTx) % 16 == 0, "the transcript state is copied in words; job slots stay 16-byte aligned");
 constexpr uint32_t LOG_COPY_THREADS = 256;
-__global__ void __launch_bounds__(LOG_COPY_THREADS) log_to_host_kernel(const uint32_t* __restrict__ log, uint32_t cap,
-                                                                       uint32_t tx_word, uint32_t* __restrict__ host) {
+__global__ void __launch_bounds__(LOG_COPY_THREADS) log_to_host_kernel(JP<const uint32_t> log_, uint32_t cap,
+                                                                       uint32_t tx_word, uint32_t* __restrict__ host,
+                                                                       uint64_t host_job_words) {
+  const uint32_t* __restrict__ log = log_;
+  host += (uint64_t)blockIdx.z * host_job_words;
   const uint32_t used = min(log[cap], cap);
-  const uint32_t i = (blockIdx.x * LOG_COPY_THREADS + threadIdx.x) * 4;  // log and host are 256-byte aligned
+  const uint32_t i = (blockIdx.x * LOG_COPY_THREADS + threadIdx.x) * 4;  // log and host slots are 16-byte aligned
   if (i + 4 <= used) {
     *reinterpret_cast<uint4*>(host + i) = *reinterpret_cast<const uint4*>(log + i);
   } else {
@@ -1507,11 +1511,18 @@ __global__ void __launch_bounds__(LOG_COPY_THREADS) log_to_host_kernel(const uin
   }
 }
 
+// The copy of `jobs` proofs' logs (P's layout) into host (device-mapped); a batch keeps
+// its per-job DMA copies, which measured the same as one such launch.
+static void enqueue_log_to_host(const whir_prover* P, uint32_t* host_dev, uint32_t jobs, hipStream_t s) {
+  const uint32_t cap = (uint32_t)P->plan.log_words;
+  hipLaunchKernelGGL(log_to_host_kernel, dim3((cap + 4 * LOG_COPY_THREADS - 1) / (4 * LOG_COPY_THREADS), 1, jobs),
+                     dim3(LOG_COPY_THREADS), 0, s, JP<const uint32_t>(P->log), cap, (uint32_t)(P->tx_off / 4), host_dev,
+                     (uint64_t)(P->host_log_bytes / 4));
+}
+
 static int enqueue_result_copy(whir_prover* P, hipStream_t s) {
   if (P->jobs == 1 && P->host_log_dev) {
-    const uint32_t cap = (uint32_t)P->plan.log_words;
-    hipLaunchKernelGGL(log_to_host_kernel, dim3((cap + 4 * LOG_COPY_THREADS - 1) / (4 * LOG_COPY_THREADS)),
-                       dim3(LOG_COPY_THREADS), 0, s, P->log, cap, (uint32_t)(P->tx_off / 4), P->host_log_dev);
+    enqueue_log_to_host(P, P->host_log_dev, 1, s);
     HIP_TRY(hipGetLastError());
     return WHIR_OK;
   }
